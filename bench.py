@@ -1,0 +1,193 @@
+"""Benchmark: single-slice GD ptychography position-updates/s on one MI355X.
+
+Workload (synthetic, seeded): 128x128 scan at 0.4 A, 128x128-pixel float32
+diffraction patterns (Poisson, 1.6e5 counts each), 720x720 complex64 object,
+300 keV / 20 mrad / defocus 200 A probe, mini-batches of 1024 positions in
+seeded order, probe update on.  One step = one full GD iteration (16,384
+position updates + constraints) of ``reconstruct()``, amplitudes resident in
+HBM.  Prints one JSON line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds T]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_update(S, Px, Py, B):
+    """20 S^2 (amplitude read, object-patch gather, object-gradient scatter)
+    + object and probe read+written once per mini-batch, amortised."""
+    return 20 * S * S + 16 * (Px * Py + S * S) / B
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(p, batch, seconds):
+    """Oracle (NumPy, one core) forward+adjoint on mini-batches of the same
+    workload until ~`seconds` of work; returns (pos-updates/s, sample text)."""
+    from oracle import OracleSingleslicePtychography
+
+    o = OracleSingleslicePtychography(energy=p._energy)
+    o._region_of_interest_shape = p._region_of_interest_shape
+    obj = p._object_initial.copy()
+    probe = p._probe_initial.copy()
+    rng = np.random.default_rng(0)
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        idx = rng.permutation(p._num_diffraction_patterns)[:batch]
+        pos = p._positions_px[idx]
+        sp, op, ew, _ = o.forward(obj, probe, pos, p._amplitudes[idx])
+        o.adjoint(obj, probe, op, sp, pos, ew, 0.5, 1, False)
+        done += batch
+    dt = time.perf_counter() - t0
+    sample = (f"{done // batch} mini-batches of {batch} positions (forward+adjoint, probe update) "
+              f"of the same workload, NumPy complex64 oracle, {dt:.1f} s, single process "
+              f"(numpy FFT/bincount single-threaded); host {cpu_model()}, "
+              f"os.cpu_count()={os.cpu_count()}")
+    return done / dt, sample
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--scan", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); the multi-GPU "
+              "path is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from py4dstem_amd import kernels as K
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import production_workload
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    torch.cuda.set_device(0)
+    t = time.perf_counter()
+    w = production_workload(seed=0, scan=(args.scan, args.scan))
+    log(f"workload generated in {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+    p.preprocess(**w["preprocess"])
+    torch.cuda.synchronize()
+    log(f"preprocess in {time.perf_counter() - t:.1f} s")
+    S = int(p._region_of_interest_shape[0])
+    Px, Py = p._object_shape
+    N = p._num_diffraction_patterns
+    B = w["reconstruct"]["max_batch_size"]
+    recon = dict(w["reconstruct"], progress_bar=False)
+
+    if args.warmup:
+        p.reconstruct(**dict(recon, num_iter=args.warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.reconstruct(**dict(recon, num_iter=args.steps))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pos_per_s = N * args.steps / dt
+    log(f"{args.steps} iterations in {dt:.3f} s -> {pos_per_s:,.0f} pos-updates/s; "
+        f"errors {p.error_iterations[-3:]}")
+
+    # live timing of the dominant kernel (fused forward/adjoint, one launch = B
+    # positions) with HIP events on the stream it is launched on
+    dev = torch.device("cuda", 0)
+    obj = p._object_dev.clone()
+    phat = K.probe_spectrum(p._probe_dev)
+    order = torch.from_numpy(np.random.default_rng(1).permutation(N)[:B].astype(np.int32)).to(dev)
+    stream = torch.cuda.current_stream()
+    ps = torch.empty((B, S, S), dtype=torch.complex64, device=dev)
+    K.forward_adjoint(phat, obj, p._positions_dev, p._amplitudes_dev, order)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.kernel_reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        K.forward_adjoint(phat, obj, p._positions_dev, p._amplitudes_dev, order)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    del ps
+    k_avg = float(np.mean(times))
+    per_update = algorithmic_bytes_per_update(S, Px, Py, B)
+    achieved = per_update * B / k_avg / 1e9
+    log(f"k_fused avg {k_avg * 1e6:.1f} us per launch of {B} positions")
+
+    cpu_rate, sample = cpu_baseline(p, B, args.cpu_seconds)
+    log(f"cpu baseline {cpu_rate:,.1f} pos-updates/s")
+
+    out = {
+        "metric": "position-updates/s",
+        "value": round(pos_per_s, 1),
+        "unit": "position-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (complex64)",
+        "data": "synthetic (seeded Gaussian-atom object, simulated Poisson patterns)",
+        "config": {
+            "workload": f"single-slice GD ptychography, {args.scan}x{args.scan} scan, "
+                        f"{S}x{S} px patterns, {Px}x{Py} object, mini-batch {B}, probe update",
+            "scan_positions": N, "frame_px": S, "object_px": [Px, Py], "mini_batch": B,
+            "iterations_timed": args.steps,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_fused",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel_avg_us": round(k_avg * 1e6, 2),
+            "bytes_per_update": per_update,
+        },
+        "cpu_baseline": {
+            "value": round(cpu_rate, 2),
+            "unit": "position-updates/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": sample,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,146 @@
+/* C-ABI of libpty_hip.so: MI355X (gfx950) kernels for py4DSTEM's single-slice
+ * gradient-descent ptychography.
+ *
+ * The reference has no native interface for this path: its seam is the
+ * Python array module swap (py4DSTEM/process/phase/phase_base_class.py:46-84)
+ * under SingleslicePtychography.reconstruct()
+ * (py4DSTEM/process/phase/singleslice_ptychography.py:606-996).  Each entry
+ * point below replaces the reference function(s) cited beside it; the Python
+ * package py4dstem_amd binds them with ctypes.
+ *
+ * Conventions
+ *  - Complex arrays are complex64: interleaved float32 (re, im), 8-byte aligned.
+ *  - All buffers are device memory owned by the caller; the library keeps no
+ *    pointer after a call returns and allocates nothing.
+ *  - Every call is asynchronous on `stream` (a hipStream_t, NULL = default
+ *    stream) and returns 0 on success or a negative PTY_E* code; the message
+ *    is available from pty_last_error() on the calling thread.
+ *  - Frame edge S must be 64 or 128 (square frames).  Objects need Px >= S
+ *    and Py >= S.  "Kernel layout" amplitudes / probe spectra are stored in
+ *    the FFT's permuted frequency order (see pty_permute_frames).
+ */
+#ifndef PTY_HIP_H
+#define PTY_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* pty_stream_t; /* hipStream_t */
+
+enum {
+  PTY_OK = 0,
+  PTY_E_ARG = -1,      /* bad size, null pointer or misalignment */
+  PTY_E_UNSUPPORTED = -2, /* frame size not compiled */
+  PTY_E_LAUNCH = -3,   /* HIP launch / runtime error */
+  PTY_E_WORKSPACE = -4 /* workspace too small */
+};
+
+int pty_version(void);
+const char* pty_last_error(void);
+
+/* Batched 2-D FFT of `batch` S x S frames, numpy.fft.fft2 / ifft2 semantics
+ * (inverse scaled by 1/S^2), natural order in and out.  Replaces np.fft.fft2
+ * / ifft2 at phase/utils.py:423, 526, 533 and ptychographic_methods.py:1600,
+ * 1633. */
+int pty_fft2_c64(const void* in, void* out, int batch, int S, int inverse, pty_stream_t stream);
+
+/* Reorder `count` real S x S frames from natural to kernel layout
+ * (direction 0) or back (direction 1). */
+int pty_permute_frames(const float* in, float* out, int count, int S, int direction,
+                       pty_stream_t stream);
+
+/* probe_hat = fft2(probe) in kernel layout (the fft2 inside fft_shift,
+ * phase/utils.py:526). */
+int pty_probe_spectrum(const void* probe, void* probe_hat, int S, pty_stream_t stream);
+
+/* Fused forward operator and exit-wave update for one mini-batch
+ * (ptychographic_methods.py:1482-1527 _return_shifted_probes /
+ * _overlap_projection, 1576-1635 _gradient_descent_fourier_projection,
+ * 1736-1816 _forward).  For slot b < B with position idx = order[b]:
+ *   ps[b]  = P_shift = ifft2(probe_hat * ramp(frac))    (natural order)
+ *   chi[b] = ifft2(A e^{i angle F} - F), F = fft2(P_shift * O_patch)
+ *   err[b] = sum (A - |F|)^2                             (float64)
+ *   far[b] = F in kernel layout                          (skipped if NULL)
+ * Patches are S x S, corner-centred. */
+int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px, int Py,
+                              const float* positions, int N, const float* amplitudes,
+                              const int* order, int B, int S, void* ps, void* chi,
+                              double* err, void* far, pty_stream_t stream);
+
+/* Deterministic scatter-add of staged patches into object-shaped sums, in
+ * float64 and in batch order per pixel, cast to float32
+ * (phase_base_class.py:1905-1968, _sum_overlapping_patches_bincounts, as
+ * called at ptychographic_methods.py:1859-1886):
+ *   grad = scatter(conj(ps) * chi), spsum = scatter(|ps|^2);
+ *   tilemax[t] = max spsum over 32 x 32 tile t (ceil(Px/32)*ceil(Py/32)). */
+int pty_scatter_patches(const void* ps, const void* chi, const float* positions, int N,
+                        const int* order, int B, int S, int Px, int Py, void* grad, float* spsum,
+                        float* tilemax, pty_stream_t stream);
+
+/* object += step * grad / sqrt(1e-16 + ((1-n) spsum)^2 + (n max spsum)^2)
+ * (ptychographic_methods.py:1864-1886). */
+int pty_object_update(void* object, const void* grad, const float* spsum, const float* tilemax,
+                      int num_tiles, int Px, int Py, float step_size, float normalization_min,
+                      pty_stream_t stream);
+
+/* Probe step sum_b conj(O_patch) * chi normalised by sum_b |O_patch|^2
+ * (ptychographic_methods.py:1888-1906); reads the pre-update object.
+ * `partials` needs pty_probe_partials_bytes(S, B) bytes. */
+size_t pty_probe_partials_bytes(int S, int B);
+int pty_probe_update(void* probe, const void* chi, const void* object, int Px, int Py,
+                     const float* positions, int N, const int* order, int B, int S,
+                     float step_size, float normalization_min, void* partials,
+                     pty_stream_t stream);
+
+/* Object constraints for a complex object (ptychographic_constraints.py:
+ * 29-97, 411-464): optional shrinkage by the mean phase over `mask` (uint8,
+ * NULL = none) plus shrinkage_rad, then |O| <= 1 (or |O| = 1 when
+ * pure_phase_object).  `partials` needs pty_constraint_partials_bytes(). */
+size_t pty_constraint_partials_bytes(void);
+int pty_object_constraints(void* object, int Px, int Py, const unsigned char* mask,
+                           float shrinkage_rad, int pure_phase_object, void* partials,
+                           pty_stream_t stream);
+
+/* Probe centre-of-mass recentring (ptychographic_constraints.py:894-918). */
+int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream);
+
+/* ---- whole reconstruction loop ------------------------------------------ */
+
+typedef struct pty_problem {
+  int S, Px, Py, N;
+  void* object;                          /* complex64 [Px][Py], updated in place */
+  void* probe;                           /* complex64 [S][S], updated in place */
+  const float* positions;                /* float32 [N][2] object pixels */
+  const float* amplitudes;               /* float32 [N][S][S], kernel layout */
+  const unsigned char* fov_mask_inverse; /* uint8 [Px][Py] or NULL */
+  float mean_intensity;
+} pty_problem;
+
+typedef struct pty_gd_params {
+  int num_iter;
+  int max_batch;        /* positions per mini-batch, 1..N */
+  int fix_probe;
+  int fix_probe_com;    /* applied only when the probe is updated */
+  int pure_phase_object;
+  float step_size;
+  float normalization_min;
+  float shrinkage_rad;
+} pty_gd_params;
+
+size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);
+
+/* num_iter iterations of phase/singleslice_ptychography.py:847-983 (GD):
+ * `orders` holds num_iter permutations of 0..N-1 (int32, device), split into
+ * batches by phase/utils.py:539-589; `errors` receives the normalised
+ * per-iteration error (float64, device, num_iter entries). */
+int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const int* orders,
+               double* errors, void* workspace, size_t workspace_bytes, pty_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTY_HIP_H */

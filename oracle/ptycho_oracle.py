@@ -1,0 +1,743 @@
+"""NumPy restatement of py4DSTEM's single-slice gradient-descent ptychography.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Reference: py4DSTEM 0.14.18.  Abbreviations in citations:
+``phase/`` = ``py4DSTEM/process/phase/``.
+
+Precision modes
+---------------
+``mode="c64"`` is the NumPy >= 2 behaviour (``np.fft`` keeps complex64), which
+is what a reference run on this image's NumPy 2.2 computes.  ``mode="c128"``
+reproduces NumPy < 2 (the version pinned by the reference's ``setup.py:26``),
+where every ``np.fft`` call upcasts to complex128 while the object stays
+complex64.  The two modes differ only in ``_fft2`` / ``_ifft2`` below; every
+other dtype follows from NumPy's own promotion rules, as in the reference.
+"""
+
+from __future__ import annotations
+
+import math
+import warnings
+
+import numpy as np
+import scipy.ndimage
+
+# Which check in tests/ pins each function (see the package docstring).
+PINNING = {
+    "electron_wavelength_angstrom": "closed form: lambda(300 keV) = 0.019687 A",
+    "ComplexProbeOracle": "closed-form aperture/rolloff and defocus chi; unit norm",
+    "fourier_translation_operator": "integer shift == np.roll; shifts compose",
+    "fft_shift": "integer shift == np.roll; shifts compose; Parseval",
+    "subdivide_into_batches": "hand-derived vectors and sum/ordering invariants",
+    "patch_indices": "brute-force loops on tiny grids, round-half-even",
+    "scatter_bincount": "brute-force python loop scatter, float64 order",
+    "get_shifted_ar": "integer shift == np.roll; bilinear linearity",
+    "get_CoM": "closed form on single-pixel and symmetric images",
+    "fit_plane": "exact recovery of a noiseless plane",
+    "forward": "Parseval, projection fixed point, A=0 => chi=-psi, angle(0)=0",
+    "adjoint": "noise-free ground truth is a fixed point; FP64 finite differences",
+    "object_constraints": "mean phase outside FOV is 0 after shrinkage; |O|<=1",
+    "probe_com_constraint": "probe CoM ~ 0 after recentring",
+    "reconstruct": "composition of the above; c64 vs c128 sensitivity measured",
+}
+
+
+# ---------------------------------------------------------------------------
+# Numerical utilities
+# ---------------------------------------------------------------------------
+
+def electron_wavelength_angstrom(E_eV):
+    """Relativistic electron wavelength in Angstrom.
+
+    Restates py4DSTEM/process/utils/utils.py:90-97 (same constants and
+    evaluation order).
+    """
+    m = 9.109383 * 10**-31
+    e = 1.602177 * 10**-19
+    c = 299792458
+    h = 6.62607 * 10**-34
+    return (
+        h / math.sqrt(2 * m * e * E_eV) / math.sqrt(1 + e * E_eV / 2 / m / c**2) * 10**10
+    )
+
+
+POLAR_SYMBOLS = (
+    "C10", "C12", "phi12",
+    "C21", "phi21", "C23", "phi23",
+    "C30", "C32", "phi32", "C34", "phi34",
+    "C41", "phi41", "C43", "phi43", "C45", "phi45",
+    "C50", "C52", "phi52", "C54", "phi54", "C56", "phi56",
+)  # phase/utils.py:29-35
+
+POLAR_ALIASES = {
+    "defocus": "C10", "astigmatism": "C12", "astigmatism_angle": "phi12",
+    "coma": "C21", "coma_angle": "phi21", "Cs": "C30", "C5": "C50",
+}  # phase/utils.py:38-43
+
+
+def set_polar_parameters(target: dict, parameters: dict):
+    """Alias handling with the defocus sign flip.
+
+    Restates phase/phase_base_class.py:1788-1809 and phase/utils.py:141-160.
+    """
+    for symbol, value in parameters.items():
+        if symbol in target:
+            target[symbol] = value
+        elif symbol == "defocus":
+            target[POLAR_ALIASES[symbol]] = -value
+        elif symbol in POLAR_ALIASES:
+            target[POLAR_ALIASES[symbol]] = value
+        else:
+            raise ValueError("{} not a recognized parameter".format(symbol))
+    return target
+
+
+def spatial_frequencies(gpts, sampling):
+    """phase/utils.py:444-462."""
+    return tuple(np.fft.fftfreq(n, d).astype(np.float32) for n, d in zip(gpts, sampling))
+
+
+class ComplexProbeOracle:
+    """Restatement of ``ComplexProbe`` (phase/utils.py:50-426), CPU only.
+
+    Envelopes (focal/angular/gaussian spread) default to zero in the reference
+    and are not used by ``SingleslicePtychography``; they are omitted.
+    """
+
+    def __init__(self, energy, gpts, sampling, semiangle_cutoff=np.inf, rolloff=2.0,
+                 vacuum_probe_intensity=None, parameters=None, mode="c64"):
+        self._energy = energy
+        self._wavelength = electron_wavelength_angstrom(energy)
+        self._gpts = tuple(int(g) for g in gpts)
+        self._sampling = tuple(sampling)
+        self._semiangle_cutoff = semiangle_cutoff
+        self._rolloff = rolloff
+        self._vacuum_probe_intensity = vacuum_probe_intensity
+        self._phase_shift = 0.0
+        self._mode = mode
+        self._parameters = dict(zip(POLAR_SYMBOLS, [0.0] * len(POLAR_SYMBOLS)))
+        set_polar_parameters(self._parameters, dict(parameters or {}))
+
+    def evaluate_aperture(self, alpha):
+        """phase/utils.py:162-191."""
+        semiangle_cutoff = self._semiangle_cutoff / 1000
+        if self._vacuum_probe_intensity is not None:
+            v = np.asarray(self._vacuum_probe_intensity, dtype=np.float32)
+            return np.sqrt(np.maximum(v, 0))
+        if self._semiangle_cutoff == np.inf:
+            return np.ones_like(alpha)
+        if self._rolloff > 0.0:
+            rolloff = self._rolloff / 1000.0
+            array = 0.5 * (1 + np.cos(np.pi * (alpha - semiangle_cutoff + rolloff) / rolloff))
+            array[alpha > semiangle_cutoff] = 0.0
+            array = np.where(alpha > semiangle_cutoff - rolloff, array,
+                             np.ones_like(alpha, dtype=np.float32))
+        else:
+            array = np.array(alpha < semiangle_cutoff).astype(np.float32)
+        return array
+
+    def evaluate_chi(self, alpha, phi):
+        """phase/utils.py:290-368 (including its ``phi41`` duplicate in the
+        fourth-order guard tuple, which only affects when the term is added)."""
+        p = self._parameters
+        alpha2 = alpha**2
+        alpha = np.array(alpha)
+        array = np.zeros(alpha.shape, dtype=np.float32)
+        if any(p[s] != 0.0 for s in ("C10", "C12", "phi12")):
+            array += 1 / 2 * alpha2 * (p["C10"] + p["C12"] * np.cos(2 * (phi - p["phi12"])))
+        if any(p[s] != 0.0 for s in ("C21", "phi21", "C23", "phi23")):
+            array += 1 / 3 * alpha2 * alpha * (
+                p["C21"] * np.cos(phi - p["phi21"]) + p["C23"] * np.cos(3 * (phi - p["phi23"]))
+            )
+        if any(p[s] != 0.0 for s in ("C30", "C32", "phi32", "C34", "phi34")):
+            array += 1 / 4 * alpha2**2 * (
+                p["C30"] + p["C32"] * np.cos(2 * (phi - p["phi32"]))
+                + p["C34"] * np.cos(4 * (phi - p["phi34"]))
+            )
+        if any(p[s] != 0.0 for s in ("C41", "phi41", "C43", "phi43", "C45", "phi41")):
+            array += 1 / 5 * alpha2**2 * alpha * (
+                p["C41"] * np.cos(phi - p["phi41"]) + p["C43"] * np.cos(3 * (phi - p["phi43"]))
+                + p["C45"] * np.cos(5 * (phi - p["phi45"]))
+            )
+        if any(p[s] != 0.0 for s in ("C50", "C52", "phi52", "C54", "phi54", "C56", "phi56")):
+            array += 1 / 6 * alpha2**3 * (
+                p["C50"] + p["C52"] * np.cos(2 * (phi - p["phi52"]))
+                + p["C54"] * np.cos(4 * (phi - p["phi54"]))
+                + p["C56"] * np.cos(6 * (phi - p["phi56"]))
+            )
+        return 2 * np.pi / self._wavelength * array + self._phase_shift
+
+    def get_scattering_angles(self):
+        """phase/utils.py:401-418."""
+        kx, ky = spatial_frequencies(self._gpts, self._sampling)
+        x = kx * self._wavelength
+        y = ky * self._wavelength
+        alpha = np.sqrt(x[:, None] ** 2 + y[None, :] ** 2)
+        phi = np.arctan2(y[None, :], x[:, None])
+        return alpha, phi
+
+    def evaluate_ctf(self):
+        """phase/utils.py:370-399."""
+        alpha, phi = self.get_scattering_angles()
+        array = np.exp(-1.0j * self.evaluate_chi(alpha, phi))
+        if self._semiangle_cutoff < np.inf or self._vacuum_probe_intensity is not None:
+            array *= self.evaluate_aperture(alpha)
+        return array
+
+    def build(self):
+        """Corner-centred probe, unit L2 norm (phase/utils.py:420-426)."""
+        array = _ifft2(self.evaluate_ctf(), self._mode)
+        array = array / np.sqrt((np.abs(array) ** 2).sum())
+        self._array = array
+        return self
+
+
+def _fft2(x, mode):
+    if mode == "c128":
+        return np.fft.fft2(np.asarray(x, dtype=np.complex128))
+    return np.fft.fft2(x)
+
+
+def _ifft2(x, mode):
+    if mode == "c128":
+        return np.fft.ifft2(np.asarray(x, dtype=np.complex128))
+    return np.fft.ifft2(x)
+
+
+def fourier_translation_operator(positions, shape):
+    """Phase ramp(s) exp(-2 pi i (kx x + ky y)) (phase/utils.py:468-505)."""
+    positions_shape = positions.shape
+    if len(positions_shape) == 1:
+        positions = positions[None]
+    kx, ky = spatial_frequencies(shape, (1.0, 1.0))
+    positions = np.asarray(positions, dtype=np.float32)
+    x = positions[:, 0].ravel()[:, None, None]
+    y = positions[:, 1].ravel()[:, None, None]
+    result = np.exp(-2.0j * np.pi * kx[None, :, None] * x) * np.exp(
+        -2.0j * np.pi * ky[None, None, :] * y
+    )
+    if len(positions_shape) == 1:
+        return result[0]
+    return result
+
+
+def fft_shift(array, positions, mode="c64"):
+    """Fourier shift of ``array`` by ``positions`` (phase/utils.py:508-533)."""
+    op = fourier_translation_operator(positions, array.shape[-2:])
+    fourier_array = _fft2(array, mode)
+    if len(op.shape) == 3 and len(fourier_array.shape) == 3:
+        shifted = fourier_array[None] * op[:, None]
+    else:
+        shifted = fourier_array * op
+    return _ifft2(shifted, mode)
+
+
+def subdivide_into_batches(num_items, num_batches=None, max_batch=None):
+    """Near-equal split, larger batches first (phase/utils.py:539-579)."""
+    if (num_batches is not None) & (max_batch is not None):
+        raise RuntimeError()
+    if num_batches is None:
+        if max_batch is not None:
+            num_batches = (num_items + (-num_items % max_batch)) // max_batch
+        else:
+            raise RuntimeError()
+    if num_items < num_batches:
+        raise RuntimeError("num_batches may not be larger than num_items")
+    elif num_items % num_batches == 0:
+        return [num_items // num_batches] * num_batches
+    else:
+        v = []
+        zp = num_batches - (num_items % num_batches)
+        pp = num_items // num_batches
+        for i in range(num_batches):
+            if i >= zp:
+                v = [pp + 1] + v
+            else:
+                v = [pp] + v
+        return v
+
+
+def generate_batches(num_items, num_batches=None, max_batch=None, start=0):
+    """phase/utils.py:582-589."""
+    for batch in subdivide_into_batches(num_items, num_batches, max_batch):
+        end = start + batch
+        yield start, end
+        start = end
+
+
+def affine_rotate(positions, angle, origin):
+    """``AffineTransform(angle=angle)(positions, origin)`` with scale 1, no
+    shear, no translation (phase/utils.py:675-698, 726-731)."""
+    cosx = np.cos(angle)
+    sinx = np.sin(angle)
+    rot = np.array([[+cosx, -sinx], [+sinx, +cosx]], dtype="float32")
+    scale = np.array([[1.0, 0.0], [0.0, 1.0]], dtype="float32")
+    shear = np.array([[1.0, 0.0], [0.0, 1.0]], dtype="float32")
+    tf_matrix = scale @ shear @ rot
+    origin = np.asarray(origin, dtype=np.float32)
+    tf_translation = np.array((0.0, 0.0)) + origin
+    return ((positions - origin) @ tf_matrix) + tf_translation
+
+
+def get_CoM(ar, corner_centered=False):
+    """Centre of mass (py4DSTEM/process/utils/utils.py:166-187)."""
+    ar = np.asarray(ar)
+    nx, ny = ar.shape
+    if corner_centered:
+        ry, rx = np.meshgrid(np.fft.fftfreq(ny, 1 / ny), np.fft.fftfreq(nx, 1 / nx))
+    else:
+        ry, rx = np.meshgrid(np.arange(ny), np.arange(nx))
+    tot = np.sum(ar)
+    return np.sum(rx * ar) / tot, np.sum(ry * ar) / tot
+
+
+def get_shifted_ar(ar, xshift, yshift):
+    """Bilinear, periodic shift (py4DSTEM/preprocess/utils.py:59-121 with
+    ``periodic=True, bilinear=True``)."""
+    ar = np.asarray(ar)
+    xF = np.floor(xshift).astype(int).item()
+    yF = np.floor(yshift).astype(int).item()
+    wx = xshift - xF
+    wy = yshift - yF
+    return (
+        np.roll(ar, (xF, yF), axis=(0, 1)) * ((1 - wx) * (1 - wy))
+        + np.roll(ar, (xF + 1, yF), axis=(0, 1)) * ((wx) * (1 - wy))
+        + np.roll(ar, (xF, yF + 1), axis=(0, 1)) * ((1 - wx) * (wy))
+        + np.roll(ar, (xF + 1, yF + 1), axis=(0, 1)) * ((wx) * (wy))
+    )
+
+
+def fit_plane(data, mask=None):
+    """Least-squares plane ``mx*x + my*y + b`` on coordinates normalised to
+    [0, 1] (py4DSTEM/process/fit/fit.py:38-124, 127-128, non-robust).
+
+    The reference minimises the same linear least-squares objective with
+    ``scipy.optimize.curve_fit``; for a model linear in its parameters the
+    minimiser is the normal-equation solution computed here with ``lstsq``.
+    """
+    shape = data.shape
+    x, y = np.linspace(0, 1, shape[0]), np.linspace(0, 1, shape[1])
+    ry, rx = np.meshgrid(y, x)
+    if mask is None:
+        mask = np.ones(shape, dtype=bool)
+    A = np.stack([rx[mask], ry[mask], np.ones(int(mask.sum()))], axis=1)
+    popt, *_ = np.linalg.lstsq(A, np.asarray(data[mask], dtype=np.float64), rcond=None)
+    return popt[0] * rx + popt[1] * ry + popt[2]
+
+
+def patch_indices(positions_px, roi_shape, obj_shape):
+    """Corner-centred periodic patch indices (phase/phase_base_class.py:1970-2003)."""
+    x0 = np.round(positions_px[:, 0]).astype("int")
+    y0 = np.round(positions_px[:, 1]).astype("int")
+    x_ind = np.fft.fftfreq(roi_shape[0], d=1 / roi_shape[0]).astype("int")
+    y_ind = np.fft.fftfreq(roi_shape[1], d=1 / roi_shape[1]).astype("int")
+    rows = (x0[:, None, None] + x_ind[None, :, None]) % obj_shape[0]
+    cols = (y0[:, None, None] + y_ind[None, None, :]) % obj_shape[1]
+    return rows, cols
+
+
+def _scatter_base(patches, positions_px, roi_shape, obj_shape):
+    """``np.bincount`` scatter-add with float64 accumulation
+    (phase/phase_base_class.py:1905-1940)."""
+    x0 = np.round(positions_px[:, 0]).astype("int")
+    y0 = np.round(positions_px[:, 1]).astype("int")
+    x_ind = np.fft.fftfreq(roi_shape[0], d=1 / roi_shape[0]).astype("int")
+    y_ind = np.fft.fftfreq(roi_shape[1], d=1 / roi_shape[1]).astype("int")
+    flat_weights = patches.ravel()
+    indices = ((y0[:, None, None] + y_ind[None, None, :]) % obj_shape[1]) + (
+        (x0[:, None, None] + x_ind[None, :, None]) % obj_shape[0]
+    ) * obj_shape[1]
+    counts = np.bincount(indices.ravel(), weights=flat_weights,
+                         minlength=int(np.prod(obj_shape)))
+    return np.reshape(counts, obj_shape).astype(np.float32)
+
+
+def scatter_bincount(patches, positions_px, roi_shape, obj_shape):
+    """Real/imag split scatter (phase/phase_base_class.py:1942-1968)."""
+    if np.iscomplexobj(patches):
+        real = _scatter_base(patches.real, positions_px, roi_shape, obj_shape)
+        imag = _scatter_base(patches.imag, positions_px, roi_shape, obj_shape)
+        return real + 1.0j * imag
+    return _scatter_base(patches, positions_px, roi_shape, obj_shape)
+
+
+# ---------------------------------------------------------------------------
+# Minimal data model used by the oracle (duck-typed like py4DSTEM's DataCube)
+# ---------------------------------------------------------------------------
+
+class _Cal:
+    def __init__(self, R_size, R_units, Q_size, Q_units):
+        self._r = (R_size, R_units)
+        self._q = (Q_size, Q_units)
+
+    def get_R_pixel_size(self):
+        return self._r[0]
+
+    def get_R_pixel_units(self):
+        return self._r[1]
+
+    def get_Q_pixel_size(self):
+        return self._q[0]
+
+    def get_Q_pixel_units(self):
+        return self._q[1]
+
+
+# ---------------------------------------------------------------------------
+# The reconstruction class
+# ---------------------------------------------------------------------------
+
+class OracleSingleslicePtychography:
+    """Restatement of ``SingleslicePtychography`` for the GD path.
+
+    Follows phase/singleslice_ptychography.py:118-996 and the mixin and base
+    class methods it calls.  Only the in-scope options are implemented:
+    complex object, square ROI equal to the detector, no resampling/padding,
+    no detector Fourier mask, fixed positions, and the default constraints
+    (shrinkage against the FOV mask, amplitude threshold, probe CoM).
+    """
+
+    def __init__(self, energy, datacube=None, semiangle_cutoff=None,
+                 semiangle_cutoff_pixels=None, rolloff=2.0,
+                 vacuum_probe_intensity=None, polar_parameters=None,
+                 initial_object_guess=None, initial_probe_guess=None,
+                 initial_scan_positions=None, object_padding_px=None,
+                 verbose=False, mode="c64", **kwargs):
+        # singleslice_ptychography.py:142-189
+        for key in kwargs:
+            if key not in POLAR_SYMBOLS and key not in POLAR_ALIASES:
+                raise ValueError("{} not a recognized parameter".format(key))
+        self._polar_parameters = dict(zip(POLAR_SYMBOLS, [0.0] * len(POLAR_SYMBOLS)))
+        polar_parameters = dict(polar_parameters or {})
+        polar_parameters.update(kwargs)
+        set_polar_parameters(self._polar_parameters, polar_parameters)
+        self._datacube = datacube
+        self._object = initial_object_guess
+        self._probe = initial_probe_guess
+        self._vacuum_probe_intensity = vacuum_probe_intensity
+        self._scan_positions = initial_scan_positions
+        self._energy = energy
+        self._semiangle_cutoff = semiangle_cutoff
+        self._semiangle_cutoff_pixels = semiangle_cutoff_pixels
+        self._rolloff = rolloff
+        self._object_padding_px = object_padding_px
+        self._verbose = verbose
+        self._mode = mode
+
+    # -- calibrations -------------------------------------------------------
+    @property
+    def sampling(self):
+        """phase/phase_base_class.py:2202-2212."""
+        return tuple(
+            electron_wavelength_angstrom(self._energy) * 1e3 / dk / n
+            for dk, n in zip(self._angular_sampling, self._amplitudes_shape)
+        )
+
+    def _extract_intensities_and_calibrations(self, datacube):
+        """phase/phase_base_class.py:424-611 (calibrated branch)."""
+        intensities = np.asarray(datacube.data, dtype=np.float32)
+        self._grid_scan_shape = intensities.shape[:2]
+        cal = datacube.calibration
+        r_units = cal.get_R_pixel_units()
+        if r_units == "A":
+            self._scan_sampling = (cal.get_R_pixel_size(),) * 2
+        elif r_units == "nm":
+            self._scan_sampling = (cal.get_R_pixel_size() * 10,) * 2
+        else:
+            raise ValueError(f"Real-space calibrations must be given in 'A', not {r_units}")
+        q_units = cal.get_Q_pixel_units()
+        lam = electron_wavelength_angstrom(self._energy)
+        if q_units == "A^-1":
+            rs = cal.get_Q_pixel_size()
+            self._reciprocal_sampling = (rs,) * 2
+            self._angular_sampling = (rs * lam * 1e3,) * 2
+        elif q_units == "mrad":
+            a = cal.get_Q_pixel_size()
+            self._angular_sampling = (a,) * 2
+            self._reciprocal_sampling = (a / lam / 1e3,) * 2
+        else:
+            raise ValueError("Reciprocal-space calibrations must be given in 'A^-1' or 'mrad'")
+        return intensities
+
+    # -- preprocess ---------------------------------------------------------
+    def preprocess(self, force_com_rotation=None, force_com_transpose=None,
+                   force_com_shifts=None, fit_function="plane",
+                   object_fov_mask=None, center_positions_in_fov=True,
+                   max_batch_size=None, **kwargs):
+        """phase/singleslice_ptychography.py:191-604 (no plotting)."""
+        mode = self._mode
+        dc = self._datacube
+        Rshape = dc.data.shape[:2]
+        if force_com_shifts is not None and np.isscalar(force_com_shifts[0]):
+            # phase_base_class.py:277-282
+            force_com_shifts = (np.ones(Rshape) * force_com_shifts[0],
+                                np.ones(Rshape) * force_com_shifts[1])
+        intensities = self._extract_intensities_and_calibrations(dc)
+        if self._semiangle_cutoff_pixels:
+            self._semiangle_cutoff = self._semiangle_cutoff_pixels * self._angular_sampling[0]
+
+        # CoM (phase_base_class.py:613-767, vectorised branch)
+        kx = np.arange(intensities.shape[-2], dtype=np.float32)
+        ky = np.arange(intensities.shape[-1], dtype=np.float32)
+        kya, kxa = np.meshgrid(ky, kx)
+        isum = np.sum(intensities, axis=(-2, -1))
+        com_mx = np.sum(intensities * kxa[None, None], axis=(-2, -1)) / isum
+        com_my = np.sum(intensities * kya[None, None], axis=(-2, -1)) / isum
+        if force_com_shifts is None:
+            if fit_function is not None:
+                finite = np.isfinite(com_mx)
+                com_fx = np.asarray(fit_plane(com_mx, finite), dtype=np.float32)
+                com_fy = np.asarray(fit_plane(com_my, finite), dtype=np.float32)
+            else:
+                com_fx = np.asarray(com_mx, dtype=np.float32)
+                com_fy = np.asarray(com_my, dtype=np.float32)
+        else:
+            com_fx = np.asarray(force_com_shifts[0], dtype=np.float32)
+            com_fy = np.asarray(force_com_shifts[1], dtype=np.float32)
+        self._com_fitted_x, self._com_fitted_y = com_fx, com_fy
+
+        # rotation (phase_base_class.py:769-1345) -- forced branch only
+        if force_com_rotation is None or force_com_transpose is None:
+            raise NotImplementedError("oracle covers force_com_rotation/transpose only")
+        self._rotation_best_rad = np.deg2rad(force_com_rotation)
+        self._rotation_best_transpose = force_com_transpose
+
+        # normalisation (phase_base_class.py:1347-1460)
+        mean_intensity = 0
+        diff = intensities.copy()
+        for rx in range(diff.shape[0]):
+            for ry in range(diff.shape[1]):
+                I = get_shifted_ar(diff[rx, ry], -com_fx[rx, ry], -com_fy[rx, ry])
+                mean_intensity += np.sum(I)
+                diff[rx, ry] = np.sqrt(np.maximum(I, 0))
+        qx, qy = diff.shape[-2:]
+        diff = diff.reshape((-1, qx, qy))
+        mean_intensity /= diff.shape[0]
+        self._amplitudes = diff
+        self._mean_diffraction_intensity = mean_intensity
+        self._num_diffraction_patterns = diff.shape[0]
+        self._amplitudes_shape = np.array(diff.shape[-2:])
+        self._region_of_interest_shape = np.array(diff.shape[-2:])
+
+        # positions (phase_base_class.py:1811-1903)
+        self._positions_px, self._object_padding_px = self._positions_in_pixels(
+            self._scan_positions, self._object_padding_px)
+
+        # object (ptychographic_methods.py:40-79)
+        if self._object is None:
+            pad_x = self._object_padding_px[0][1]
+            pad_y = self._object_padding_px[1][1]
+            p, q = np.round(np.max(self._positions_px, axis=0))
+            p = np.max([np.round(p + pad_x), self._region_of_interest_shape[0]]).astype("int")
+            q = np.max([np.round(q + pad_y), self._region_of_interest_shape[1]]).astype("int")
+            self._object = np.ones((p, q), dtype=np.complex64)
+        else:
+            self._object = np.asarray(self._object, dtype=np.complex64)
+        self._object_initial = self._object.copy()
+        self._object_shape = self._object.shape
+
+        # singleslice_ptychography.py:462-478
+        self._positions_px = np.asarray(self._positions_px, dtype=np.float32)
+        com = self._positions_px.mean(0)
+        if center_positions_in_fov:
+            self._positions_px -= com - np.array(self._object_shape) / 2
+        self._positions_px_initial = self._positions_px.copy()
+
+        # probe (ptychographic_methods.py:1011-1102)
+        if self._probe is None:
+            probe = ComplexProbeOracle(
+                energy=self._energy, gpts=self._region_of_interest_shape,
+                sampling=self.sampling, semiangle_cutoff=self._semiangle_cutoff,
+                rolloff=self._rolloff, parameters=self._polar_parameters, mode=mode,
+            ).build()._array
+            probe_intensity = np.sum(np.abs(_fft2(probe, mode)) ** 2)
+            probe *= np.sqrt(self._mean_diffraction_intensity / probe_intensity)
+        else:
+            probe = np.asarray(self._probe, dtype=np.complex64)
+        self._probe = probe
+        self._probe_initial = probe.copy()
+
+        # FOV mask (singleslice_ptychography.py:504-537)
+        if object_fov_mask is None:
+            N = self._num_diffraction_patterns
+            if max_batch_size is None:
+                max_batch_size = N
+            overlap = np.zeros(self._object_shape, dtype=np.float32)
+            for start, end in generate_batches(N, max_batch=max_batch_size):
+                pos = self._positions_px[start:end]
+                frac = pos - np.round(pos)
+                sp = fft_shift(self._probe, frac, mode)
+                overlap += scatter_bincount(np.abs(sp) ** 2, pos,
+                                            self._region_of_interest_shape,
+                                            self._object_shape)
+            self._probe_overlap = overlap
+            blurred = scipy.ndimage.gaussian_filter(overlap, 1.0)
+            self._object_fov_mask = blurred > 0.25 * blurred.max()
+        elif object_fov_mask is True:
+            self._object_fov_mask = np.full(self._object_shape, True)
+        else:
+            self._object_fov_mask = np.asarray(object_fov_mask)
+        self._object_fov_mask_inverse = np.invert(self._object_fov_mask)
+        return self
+
+    def _positions_in_pixels(self, positions, object_padding_px):
+        """phase/phase_base_class.py:1811-1903."""
+        sampling = self.sampling
+        if positions is None:
+            nx, ny = self._grid_scan_shape
+            sx, sy = self._scan_sampling
+            x = np.arange(nx) * sx
+            y = np.arange(ny) * sy
+            x, y = np.meshgrid(x, y, indexing="ij")
+            positions = np.stack((x.ravel(), y.ravel()), axis=-1)
+        else:
+            positions = np.array(positions, dtype=np.float64)
+        if self._rotation_best_rad is not None:
+            positions = affine_rotate(positions, self._rotation_best_rad, positions.mean(0))
+        if self._rotation_best_transpose:
+            positions = np.flip(positions, 1)
+            sampling = sampling[::-1]
+        positions -= np.min(positions, axis=0).clip(-np.inf, 0)
+        positions[:, 0] /= sampling[0]
+        positions[:, 1] /= sampling[1]
+        if object_padding_px is None:
+            float_padding = self._region_of_interest_shape / 2
+            object_padding_px = (float_padding, float_padding)
+        elif np.isscalar(object_padding_px[0]):
+            object_padding_px = ((object_padding_px[0],) * 2, (object_padding_px[1],) * 2)
+        positions[:, 0] += object_padding_px[0][0]
+        positions[:, 1] += object_padding_px[1][0]
+        return positions, object_padding_px
+
+    # -- forward / adjoint --------------------------------------------------
+    def forward(self, obj, probe, positions_px, amplitudes):
+        """``_forward`` for GD (ptychographic_methods.py:1482-1527, 1576-1635,
+        1736-1816).  Returns (shifted_probes, object_patches, exit_waves, error)."""
+        mode = self._mode
+        frac = positions_px - np.round(positions_px)
+        rows, cols = patch_indices(positions_px, self._region_of_interest_shape, obj.shape)
+        shifted_probes = fft_shift(probe, frac, mode)
+        object_patches = obj[rows, cols]
+        overlap = shifted_probes * object_patches
+        fourier_overlap = _fft2(overlap, mode)
+        farfield = np.abs(fourier_overlap)
+        error = np.sum(np.abs(amplitudes - farfield) ** 2)
+        modified = amplitudes * np.exp(1j * np.angle(fourier_overlap))
+        modified = modified - fourier_overlap
+        exit_waves = _ifft2(modified, mode)
+        return shifted_probes, object_patches, exit_waves, error
+
+    def adjoint(self, obj, probe, object_patches, shifted_probes, positions_px,
+                exit_waves, step_size, normalization_min, fix_probe):
+        """``_gradient_descent_adjoint`` for a complex object
+        (ptychographic_methods.py:1818-1908).  Updates in place, as the
+        reference's ``+=`` does."""
+        roi = self._region_of_interest_shape
+        shape = obj.shape
+        probe_normalization = scatter_bincount(np.abs(shifted_probes) ** 2,
+                                               positions_px, roi, shape)
+        probe_normalization = 1 / np.sqrt(
+            1e-16
+            + ((1 - normalization_min) * probe_normalization) ** 2
+            + (normalization_min * np.max(probe_normalization)) ** 2
+        )
+        obj += step_size * (
+            scatter_bincount(np.conj(shifted_probes) * exit_waves, positions_px, roi, shape)
+            * probe_normalization
+        )
+        if not fix_probe:
+            object_normalization = np.sum(np.abs(object_patches) ** 2, axis=0)
+            object_normalization = 1 / np.sqrt(
+                1e-16
+                + ((1 - normalization_min) * object_normalization) ** 2
+                + (normalization_min * np.max(object_normalization)) ** 2
+            )
+            probe += step_size * (
+                np.sum(np.conj(object_patches) * exit_waves, axis=0) * object_normalization
+            )
+        return obj, probe
+
+    # -- constraints --------------------------------------------------------
+    @staticmethod
+    def object_constraints(obj, object_mask, shrinkage_rad=0.0, pure_phase_object=False):
+        """ptychographic_constraints.py:29-97, 411-464 (complex object,
+        no smoothing filters)."""
+        if shrinkage_rad > 0.0 or object_mask is not None:
+            phase = np.angle(obj)
+            amp = np.abs(obj)
+            if object_mask is not None:
+                shrinkage_rad += phase[..., object_mask].mean()
+            phase -= shrinkage_rad
+            obj = amp * np.exp(1.0j * phase)
+        phase = np.angle(obj)
+        amplitude = 1.0 if pure_phase_object else np.minimum(np.abs(obj), 1.0)
+        return amplitude * np.exp(1.0j * phase)
+
+    def probe_com_constraint(self, probe):
+        """ptychographic_constraints.py:894-918."""
+        intensity = np.abs(probe) ** 2
+        x0, y0 = get_CoM(intensity, corner_centered=True)
+        return fft_shift(probe, -np.array([x0, y0]), self._mode)
+
+    # -- reconstruct --------------------------------------------------------
+    def reconstruct(self, num_iter=8, max_batch_size=None, seed_random=None,
+                    step_size=0.5, normalization_min=1, fix_probe=False,
+                    fix_probe_com=True, fix_potential_baseline=True,
+                    shrinkage_rad=0.0, pure_phase_object=False,
+                    store_iterations=False, reset=None, **kwargs):
+        """phase/singleslice_ptychography.py:606-996 for GD."""
+        # ptychographic_methods.py:252-294
+        if store_iterations and (not hasattr(self, "object_iterations") or reset):
+            self.object_iterations = []
+            self.probe_iterations = []
+        if reset is True:
+            self.error_iterations = []
+            self._object = self._object_initial.copy()
+            self._probe = self._probe_initial.copy()
+            self._positions_px = self._positions_px_initial.copy()
+        elif reset is None:
+            if hasattr(self, "error"):
+                warnings.warn("Continuing reconstruction from previous result. "
+                              "Use reset=True for a fresh start.", UserWarning)
+            else:
+                self.error_iterations = []
+        else:
+            if not hasattr(self, "error"):
+                self.error_iterations = []
+
+        N = self._num_diffraction_patterns
+        shuffled_indices = np.arange(N)
+        if max_batch_size is not None:
+            np.random.seed(seed_random)
+        else:
+            max_batch_size = N
+        object_mask = (self._object_fov_mask_inverse
+                       if fix_potential_baseline and self._object_fov_mask_inverse.sum() > 0
+                       else None)
+        error = 0.0
+        for _ in range(num_iter):
+            error = 0.0
+            np.random.shuffle(shuffled_indices)
+            for start, end in generate_batches(N, max_batch=max_batch_size):
+                idx = shuffled_indices[start:end]
+                pos = self._positions_px[idx]
+                sp, op, ew, err = self.forward(self._object, self._probe, pos,
+                                               self._amplitudes[idx])
+                self._object, self._probe = self.adjoint(
+                    self._object, self._probe, op, sp, pos, ew, step_size,
+                    normalization_min, fix_probe)
+                error += err
+            error /= self._mean_diffraction_intensity * N
+            self._object = self.object_constraints(self._object, object_mask,
+                                                   shrinkage_rad, pure_phase_object)
+            if fix_probe_com and not fix_probe:
+                self._probe = self.probe_com_constraint(self._probe)
+            self.error_iterations.append(float(error))
+            if store_iterations:
+                self.object_iterations.append(np.asarray(self._object).copy())
+                self.probe_iterations.append(np.fft.fftshift(self._probe, axes=(-2, -1)))
+        self.object = np.asarray(self._object)
+        self.probe = np.fft.fftshift(self._probe, axes=(-2, -1))
+        self.error = float(error)
+        return self

@@ -1,0 +1,547 @@
+// Shared host/device core of the ptychography kernels.
+//
+// Every kernel body is written as a sequence of "phases": per-thread
+// functions separated by work-group barriers.  The HIP kernels in
+// pty_hip.hip call the phases with threadIdx.x and __syncthreads(); the CPU
+// emulator used by the tests calls the same functions for every thread id in
+// turn, with an LDS view that flags cross-thread races inside a phase and
+// out-of-range indices.  No phase may read a word another thread writes in
+// the same phase, or write a word another thread reads or writes in it.
+//
+// FFT scheme (frames of S x S complex64, S = N1 * N2):
+//   forward  = pass A (N1-point DFTs at stride N2, then twiddles)
+//              + pass B (N2-point DFTs of contiguous runs)
+//            natural order in -> "permuted" order out: X[k1 + N1*k2] is
+//            stored at position k2 + N2*k1.
+//   inverse  = pass B' + pass A' (the transposed algorithm):
+//            permuted order in -> natural order out.
+// Pointwise work between a forward and an inverse transform (the far-field
+// amplitude projection, the probe phase ramp) therefore never needs a
+// reordering pass; the measured amplitudes and the probe spectrum are simply
+// stored in the permuted order once.
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define PTY_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#include <cstdint>
+#define PTY_HD inline
+#endif
+
+#ifndef PTY_CHECK
+#define PTY_CHECK(cond, what) ((void)0)
+#endif
+
+namespace pty {
+
+struct __attribute__((aligned(8))) cf {
+  float x, y;
+};
+
+PTY_HD cf mkcf(float x, float y) { cf r; r.x = x; r.y = y; return r; }
+PTY_HD cf cadd(cf a, cf b) { return mkcf(a.x + b.x, a.y + b.y); }
+PTY_HD cf csub(cf a, cf b) { return mkcf(a.x - b.x, a.y - b.y); }
+PTY_HD cf cmul(cf a, cf b) { return mkcf(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// conj(a) * b
+PTY_HD cf cconjmul(cf a, cf b) { return mkcf(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
+PTY_HD cf cscale(cf a, float s) { return mkcf(a.x * s, a.y * s); }
+PTY_HD cf cconj(cf a) { return mkcf(a.x, -a.y); }
+PTY_HD float cabs2(cf a) { return a.x * a.x + a.y * a.y; }
+
+// reciprocal square root (hardware v_rsq_f32 on the GPU)
+PTY_HD float frsq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / std::sqrt(x);
+#endif
+}
+
+// multiply by the quarter-turn of the transform direction: -i (forward), +i (inverse)
+template <bool INV> PTY_HD cf rot90(cf a) { return INV ? mkcf(-a.y, a.x) : mkcf(a.y, -a.x); }
+
+// ---------------------------------------------------------------------------
+// in-register DFTs, natural order in and out; INV selects exp(+2 pi i nk/N)
+// ---------------------------------------------------------------------------
+template <bool INV> PTY_HD void dft4(cf& x0, cf& x1, cf& x2, cf& x3) {
+  cf t0 = cadd(x0, x2), t1 = csub(x0, x2);
+  cf t2 = cadd(x1, x3), t3 = rot90<INV>(csub(x1, x3));
+  x0 = cadd(t0, t2);
+  x2 = csub(t0, t2);
+  x1 = cadd(t1, t3);
+  x3 = csub(t1, t3);
+}
+
+template <bool INV> PTY_HD void dft8(cf* x) {
+  const float c = 0.70710678118654752f;
+  cf e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+  cf o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  // twiddles W8^k (forward: exp(-i pi k / 4))
+  cf w1 = INV ? mkcf(c * (o1.x - o1.y), c * (o1.x + o1.y)) : mkcf(c * (o1.x + o1.y), c * (o1.y - o1.x));
+  cf w2 = rot90<INV>(o2);
+  cf w3 = INV ? mkcf(-c * (o3.x + o3.y), c * (o3.x - o3.y)) : mkcf(c * (o3.y - o3.x), -c * (o3.x + o3.y));
+  x[0] = cadd(e0, o0); x[4] = csub(e0, o0);
+  x[1] = cadd(e1, w1); x[5] = csub(e1, w1);
+  x[2] = cadd(e2, w2); x[6] = csub(e2, w2);
+  x[3] = cadd(e3, w3); x[7] = csub(e3, w3);
+}
+
+// 16-point DFT as 4 x 4: inner DFT4 over n1 (stride 4), twiddle W16^(n2*k1),
+// outer DFT4 over n2.
+template <bool INV> PTY_HD void dft16(cf* x) {
+  const float c1 = 0.92387953251128676f;  // cos(pi/8)
+  const float s1 = 0.38268343236508977f;  // sin(pi/8)
+  const float c2 = 0.70710678118654752f;
+  const float sg = INV ? 1.f : -1.f;      // sign of the twiddle angle
+  cf y[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    cf a = x[n2], b = x[n2 + 4], cc = x[n2 + 8], d = x[n2 + 12];
+    dft4<INV>(a, b, cc, d);
+    y[n2 * 4 + 0] = a; y[n2 * 4 + 1] = b; y[n2 * 4 + 2] = cc; y[n2 * 4 + 3] = d;
+  }
+  // W16^m for m = n2*k1 in {1,2,3,2,4,6,3,6,9}
+  const float wc[10] = {1.f, c1, c2, s1, 0.f, -s1, -c2, -c1, -1.f, -c1};
+  const float ws[10] = {0.f, s1, c2, c1, 1.f, c1, c2, s1, 0.f, -s1};
+#pragma unroll
+  for (int n2 = 1; n2 < 4; ++n2) {
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) {
+      const int m = n2 * k1;
+      cf w = mkcf(wc[m], sg * ws[m]);
+      y[n2 * 4 + k1] = cmul(y[n2 * 4 + k1], w);
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cf a = y[k1], b = y[4 + k1], cc = y[8 + k1], d = y[12 + k1];
+    dft4<INV>(a, b, cc, d);
+    x[k1] = a; x[k1 + 4] = b; x[k1 + 8] = cc; x[k1 + 12] = d;
+  }
+}
+
+template <int N, bool INV> PTY_HD void dft(cf* x);
+template <> PTY_HD void dft<8, false>(cf* x) { dft8<false>(x); }
+template <> PTY_HD void dft<8, true>(cf* x) { dft8<true>(x); }
+template <> PTY_HD void dft<16, false>(cf* x) { dft16<false>(x); }
+template <> PTY_HD void dft<16, true>(cf* x) { dft16<true>(x); }
+
+// ---------------------------------------------------------------------------
+// frame geometry
+// ---------------------------------------------------------------------------
+template <int S> struct Geo;
+template <> struct Geo<64> { static constexpr int N1 = 8, N2 = 8; };
+template <> struct Geo<128> { static constexpr int N1 = 16, N2 = 8; };
+
+template <int S> struct Frame {
+  static constexpr int N1 = Geo<S>::N1;
+  static constexpr int N2 = Geo<S>::N2;
+  static constexpr int R = N1 / N2;   // pass-B groups per thread
+  static constexpr int T = S * N2;    // threads per frame
+  static constexpr int PITCH = S + S / N1; // LDS row pitch in complex words
+  static constexpr int LDS_WORDS = S * PITCH;
+  // One pad word after every N1-word block of a row.  Pass A (stride N2)
+  // and pass B (contiguous runs of N1 starting at N1*t) then hit distinct
+  // banks, the row pitch S + S/N1 = S + 8 words staggers neighbouring rows,
+  // and every per-thread access is base + compile-time offset.
+  static PTY_HD int phys(int p) { return p + p / N1; }
+  static PTY_HD int at(int row, int col) { return row * PITCH + phys(col); }
+  // natural frequency index held at permuted position p
+  static PTY_HD int kidx(int p) { return p / N2 + N1 * (p % N2); }
+  // fftfreq integer of natural index k
+  static PTY_HD int fidx(int k) { return k < S / 2 ? k : k - S; }
+  // thread -> (line, q) for row passes and column passes
+  static PTY_HD int row_line(int tid) { return tid / N2; }
+  static PTY_HD int row_q(int tid) { return tid % N2; }
+  static PTY_HD int col_line(int tid) { return tid % S; }
+  static PTY_HD int col_q(int tid) { return tid / S; }
+};
+
+PTY_HD int wrapi(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// Passes on one line (a row or a column) of a frame.  v[] is the thread's
+// register block; for A passes v[m] sits at line position n2 + N2*m, for B
+// passes v[j] sits at position N1*t + j.
+template <int S> struct Pass {
+  using F = Frame<S>;
+  static constexpr int N1 = F::N1, N2 = F::N2, R = F::R;
+
+  // tw[m] = exp(-2 pi i m / S)
+  template <bool INV> static PTY_HD void A(int n2, cf* v, const cf* tw) {
+    if (!INV) {
+      dft<N1, false>(v);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+    } else {
+      dft<N1, true>(v);
+    }
+  }
+  template <bool INV> static PTY_HD void B(int t, cf* v, const cf* tw) {
+#pragma unroll
+    for (int g = 0; g < R; ++g) {
+      cf* u = v + g * N2;
+      dft<N2, INV>(u);
+      if (INV) {
+        const int k1 = R * t + g;
+#pragma unroll
+        for (int n2 = 1; n2 < N2; ++n2) u[n2] = cmul(u[n2], cconj(tw[n2 * k1]));
+      }
+    }
+  }
+  // twiddles for the inverse A' pass are applied in B'; the forward A pass
+  // applies its own.  (Inverse: B' multiplies V[k1][n2] by conj W^(n2 k1).)
+};
+
+// LDS access policy used by the GPU kernels (the emulator supplies a checking one)
+struct LdsPtr {
+  cf* p;
+  PTY_HD cf ld(int i) const { return p[i]; }
+  PTY_HD void st(int i, cf v) const { p[i] = v; }
+};
+
+// ---------------------------------------------------------------------------
+// Fused per-position kernel: shifted probe, exit wave, far-field projection
+// and the adjoint terms, one work group per scan position.
+// ---------------------------------------------------------------------------
+struct FusedArgs {
+  const cf* probe_hat;     // [S][S] fft2(probe), permuted layout
+  const cf* object;        // [Px][Py]
+  const float* positions;  // [N][2] float32 pixel positions
+  const float* amps;       // [N][S][S] measured amplitudes, permuted layout
+  const int* order;        // [B] position index of each batch slot
+  int N, B, Px, Py;
+  int b0;                  // first batch slot handled by this launch
+  cf* ps_out;              // [B][S][S] shifted probe P_shift      (natural)
+  cf* chi_out;             // [B][S][S] chi = ifft2(A e^{i angle F} - F) (natural)
+  double* err;             // [B] sum (A - |F|)^2 per position
+  cf* fwd_out;             // optional [B][S][S] far field F (permuted), or null
+};
+
+template <int S> struct Fused {
+  using F = Frame<S>;
+  using P = Pass<S>;
+  static constexpr int N1 = F::N1, N2 = F::N2, R = F::R, T = F::T;
+
+  struct Regs {
+    cf v[N1];
+    double err;
+    int x0, y0;   // patch anchor, wrapped into [0, Px) x [0, Py)
+    int idx;      // position index of this batch slot
+  };
+
+  // extra LDS words after the frame: exp(-2 pi i k f / S) ramp factors for
+  // rows (k in [0,S)) then columns
+  static constexpr int RAMP = F::LDS_WORDS;
+  static constexpr int LDS_TOTAL = F::LDS_WORDS + 2 * S;
+
+  // twiddle table into LDS (tid < S)
+  template <class L> static PTY_HD void twiddles(int tid, L tw) {
+    if (tid < S) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      double sn, cs;
+      sincospi(-2.0 * tid / S, &sn, &cs);
+#else
+      double cs = std::cos(-2.0 * M_PI * tid / S), sn = std::sin(-2.0 * M_PI * tid / S);
+#endif
+      tw.st(tid, mkcf((float)cs, (float)sn));
+    }
+  }
+
+  // ramp factor exp(-2 pi i f k / S) evaluated in float32 the way the
+  // reference's fourier_translation_operator does: theta = (-2 pi kf) * frac
+  static PTY_HD cf ramp1(int k, float frac) {
+    const float kf = (float)F::fidx(k) * (1.0f / S);
+    const float th = (-6.28318530717958648f * kf) * frac;
+    float s, c;
+#if defined(__HIP_DEVICE_COMPILE__)
+    sincosf(th, &s, &c);
+#else
+    s = std::sin(th); c = std::cos(th);
+#endif
+    return mkcf(c, s);
+  }
+
+  // P0: anchor of slot b and the ramp table (threads tid < 2S)
+  template <class L> static PTY_HD void p_setup(int tid, const FusedArgs& a, int b, Regs& r, L lds) {
+    int idx = a.order[b];
+    PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
+    idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;  // memory-safe on bad input
+    const float px = a.positions[2 * idx], py = a.positions[2 * idx + 1];
+    const float rx = rintf(px), ry = rintf(py);  // round half to even, like np.round
+    r.idx = idx;
+    r.x0 = wrapi((int)rx, a.Px);
+    r.y0 = wrapi((int)ry, a.Py);
+    r.err = 0.0;
+    if (tid < 2 * S) {
+      const int k = tid < S ? tid : tid - S;
+      const float frac = tid < S ? px - rx : py - ry;
+      lds.st(RAMP + tid, ramp1(k, frac));
+    }
+  }
+
+  // P1: row pass B' of ifft2(probe_hat * ramp), input from global memory
+  template <class L> static PTY_HD void p_probe_rowBinv(int tid, const FusedArgs& a, Regs& r,
+                                                        L lds, const cf* tw) {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+    const cf ex = lds.ld(RAMP + F::kidx(rr));
+    const cf* ph = a.probe_hat + rr * S + N1 * t;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      const cf e = cmul(ex, lds.ld(RAMP + S + F::kidx(N1 * t + j)));
+      r.v[j] = cmul(ph[j], e);
+    }
+    P::template B<true>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
+  }
+
+  // generic passes reading and writing LDS
+  template <bool INV, class L> static PTY_HD void p_rowA(int tid, Regs& r, L lds, const cf* tw) {
+    const int rr = F::row_line(tid), n2 = F::row_q(tid);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) r.v[m] = lds.ld(F::at(rr, n2 + N2 * m));
+    P::template A<INV>(n2, r.v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) lds.st(F::at(rr, n2 + N2 * m), r.v[m]);
+  }
+  template <bool INV, class L> static PTY_HD void p_rowB(int tid, Regs& r, L lds, const cf* tw) {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
+    P::template B<INV>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
+  }
+  template <bool INV, class L> static PTY_HD void p_colB(int tid, Regs& r, L lds, const cf* tw) {
+    const int cc = F::col_line(tid), t = F::col_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(N1 * t + j, cc));
+    P::template B<INV>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) lds.st(F::at(N1 * t + j, cc), r.v[j]);
+  }
+  // column pass A' leaving the natural-order result in registers
+  template <class L> static PTY_HD void p_colAinv_to_regs(int tid, Regs& r, L lds, const cf* tw) {
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) r.v[m] = lds.ld(F::at(n2 + N2 * m, cc));
+    P::template A<true>(n2, r.v, tw);
+    const float s = 1.0f / (float)(S * S);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) r.v[m] = cscale(r.v[m], s);
+  }
+
+  // periodic object index of window element (i, j); x0/y0 already wrapped
+  // and Px, Py >= S, so one conditional correction per axis suffices
+  static PTY_HD unsigned object_offset(const FusedArgs& a, const Regs& r, int i, int j) {
+    int row = r.x0 + F::fidx(i);
+    row += row < 0 ? a.Px : 0;
+    row -= row >= a.Px ? a.Px : 0;
+    int col = r.y0 + F::fidx(j);
+    col += col < 0 ? a.Py : 0;
+    col -= col >= a.Py ? a.Py : 0;
+    PTY_CHECK(row >= 0 && row < a.Px && col >= 0 && col < a.Py, "object index out of range");
+    return (unsigned)row * (unsigned)a.Py + (unsigned)col;
+  }
+
+  // P4+P5: shifted probe out, exit wave psi = P * O_patch, then the forward
+  // column pass A on psi straight from registers
+  template <class L> static PTY_HD void p_exitwave_colA(int tid, const FusedArgs& a, int b, Regs& r,
+                                                        L lds, const cf* tw) {
+    p_colAinv_to_regs(tid, r, lds, tw);
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+    cf* ps_b = a.ps_out + (size_t)b * S * S;
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int i = n2 + N2 * m;
+      ps_b[i * S + cc] = r.v[m];
+      r.v[m] = cmul(r.v[m], a.object[object_offset(a, r, i, cc)]);
+    }
+    P::template A<false>(n2, r.v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) lds.st(F::at(n2 + N2 * m, cc), r.v[m]);
+  }
+
+  // P8: row pass B (completing fft2), amplitude projection, row pass B'
+  template <class L> static PTY_HD void p_project(int tid, const FusedArgs& a, int b, Regs& r,
+                                                  L lds, const cf* tw) {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
+    P::template B<false>(t, r.v, tw);
+    const float* A = a.amps + (size_t)r.idx * S * S + rr * S + N1 * t;
+    cf* far = a.fwd_out ? a.fwd_out + (size_t)b * S * S + rr * S + N1 * t : nullptr;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      const cf f = r.v[j];
+      if (far) far[j] = f;
+      const float amp = A[j];
+      const float m2 = cabs2(f);
+      cf g;
+      float mag;
+      if (m2 > 0.f) {
+        const float inv = frsq(m2);
+        mag = m2 * inv;
+        g = mkcf(amp * (f.x * inv) - f.x, amp * (f.y * inv) - f.y);
+      } else {
+        mag = 0.f;
+        g = mkcf(amp - f.x, -f.y);
+      }
+      const float d = amp - mag;
+      acc += (double)(d * d);
+      r.v[j] = g;
+    }
+    r.err += acc;
+    P::template B<true>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
+  }
+
+  // P11: chi (natural order) out
+  template <class L> static PTY_HD void p_outputs(int tid, const FusedArgs& a, int b, Regs& r,
+                                                  L lds, const cf* tw) {
+    p_colAinv_to_regs(tid, r, lds, tw);
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+    cf* chi_b = a.chi_out + (size_t)b * S * S;
+#pragma unroll
+    for (int m = 0; m < N1; ++m) chi_b[(n2 + N2 * m) * S + cc] = r.v[m];
+  }
+};
+
+// The phase sequence for one scan position.  ``run`` executes a phase for
+// every thread (GPU: this thread; emulator: all thread ids in turn) and
+// ``run.barrier()`` separates phases.  Eleven barriers per position.
+template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
+  using K = Fused<S>;
+  using Regs = typename K::Regs;
+  // ifft2(probe_hat * ramp): B', A' along rows, then B', A' along columns
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+  run.barrier();
+  // shifted probe in registers; exit wave; fft2 starts with columns
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_exitwave_colA(tid, a, b, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
+  run.barrier();
+  // far field -> projection -> start of ifft2
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_project(tid, a, b, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_outputs(tid, a, b, r, lds, tw); });
+  run.reduce_err(b);  // ends with a barrier
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic scatter-add by object pixel ("sort by pixel"): one work group
+// per TILE x TILE object tile walks the batch in order and sums, in float64,
+// conj(P_shift) * chi and |P_shift|^2 of the staged patches that cover each
+// of its pixels.  Per pixel the sum
+// order is the batch order, as in the reference's np.bincount.
+// ---------------------------------------------------------------------------
+constexpr int TILE = 32;
+constexpr int GATHER_THREADS = 256;
+constexpr int GATHER_PX = TILE * TILE / GATHER_THREADS;  // pixels per thread
+
+struct GatherArgs {
+  const cf* ps;            // [B][S][S] shifted probes
+  const cf* chi;           // [B][S][S] exit-wave updates
+  const float* positions;  // [N][2]
+  const int* order;        // [B]
+  int N, B, S, Px, Py, tiles_y;
+  cf* grad;                // [Px][Py] float32 of the float64 sums
+  float* spsum;            // [Px][Py]
+  float* tilemax;          // [num_tiles] max of spsum over the tile
+};
+
+// window-relative index of object coordinate r for a window anchored at x0,
+// or -1 if outside; Px >= S is required by the host.
+PTY_HD int window_index(int r, int x0, int S, int Px) {
+  const int d = wrapi(r - x0 + S / 2, Px);
+  return d < S ? (d + S / 2) % S : -1;
+}
+
+struct GatherAcc {
+  double re[GATHER_PX], im[GATHER_PX], s[GATHER_PX];
+};
+
+PTY_HD void gather_tile_origin(const GatherArgs& a, int tile, int& R0, int& C0) {
+  R0 = (tile / a.tiles_y) * TILE;
+  C0 = (tile % a.tiles_y) * TILE;
+}
+
+// one thread's accumulation over the whole batch (uniform skip per position)
+PTY_HD void gather_accumulate(const GatherArgs& a, int tile, int tid, GatherAcc& acc) {
+  int R0, C0;
+  gather_tile_origin(a, tile, R0, C0);
+  const int tr = (a.Px - R0) < TILE ? (a.Px - R0) : TILE;
+  const int tc = (a.Py - C0) < TILE ? (a.Py - C0) : TILE;
+  const int ty = tid / TILE, tx = tid % TILE;
+  for (int k = 0; k < GATHER_PX; ++k) { acc.re[k] = 0.0; acc.im[k] = 0.0; acc.s[k] = 0.0; }
+  const int S = a.S;
+  for (int b = 0; b < a.B; ++b) {
+    int idx = a.order[b];
+    idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
+    const int x0 = (int)rintf(a.positions[2 * idx]);
+    const int y0 = (int)rintf(a.positions[2 * idx + 1]);
+    const int dr = wrapi(R0 - x0 + S / 2, a.Px);
+    const int dc = wrapi(C0 - y0 + S / 2, a.Py);
+    const bool hit_r = dr < S || dr + tr > a.Px;
+    const bool hit_c = dc < S || dc + tc > a.Py;
+    if (!(hit_r && hit_c)) continue;  // uniform across the work group
+    const int j = (tx < tc) ? window_index(C0 + tx, y0, S, a.Py) : -1;
+    if (j < 0) continue;
+    for (int k = 0; k < GATHER_PX; ++k) {
+      const int rl = ty + k * (GATHER_THREADS / TILE);
+      if (rl >= tr) break;
+      const int i = window_index(R0 + rl, x0, S, a.Px);
+      if (i < 0) continue;
+      const size_t o = ((size_t)b * S + i) * S + j;
+      const cf p = a.ps[o];
+      const cf g = cconjmul(p, a.chi[o]);  // float32 product, as the reference forms it
+      acc.re[k] += (double)g.x;
+      acc.im[k] += (double)g.y;
+      acc.s[k] += (double)cabs2(p);
+    }
+  }
+}
+
+// writes the tile's pixels; returns the thread's max of spsum
+PTY_HD float gather_store(const GatherArgs& a, int tile, int tid, const GatherAcc& acc) {
+  int R0, C0;
+  gather_tile_origin(a, tile, R0, C0);
+  const int tr = (a.Px - R0) < TILE ? (a.Px - R0) : TILE;
+  const int tc = (a.Py - C0) < TILE ? (a.Py - C0) : TILE;
+  const int ty = tid / TILE, tx = tid % TILE;
+  float m = 0.f;
+  if (tx >= tc) return m;
+  for (int k = 0; k < GATHER_PX; ++k) {
+    const int rl = ty + k * (GATHER_THREADS / TILE);
+    if (rl >= tr) break;
+    const size_t o = (size_t)(R0 + rl) * a.Py + (C0 + tx);
+    a.grad[o] = mkcf((float)acc.re[k], (float)acc.im[k]);
+    const float s = (float)acc.s[k];
+    a.spsum[o] = s;
+    m = s > m ? s : m;
+  }
+  return m;
+}
+
+}  // namespace pty

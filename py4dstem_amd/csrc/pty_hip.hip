@@ -1,0 +1,802 @@
+// HIP kernels and C-ABI for single-slice GD ptychography on MI355X (gfx950).
+//
+// Work decomposition (one mini-batch of B positions, frames S x S):
+//   k_fused          one work group per position: ifft2(probe_hat * ramp),
+//                    periodic object gather, exit wave, fft2, amplitude
+//                    projection, ifft2, adjoint products.  The frame lives in
+//                    LDS (S=128: 136 KiB of the 160 KiB per CU).
+//   k_gather         one work group per 32 x 32 object tile: float64 sums of
+//                    the staged patches in batch order (deterministic).
+//   k_object_update  normalised object step.
+//   k_probe_partial / k_probe_update   probe step from staged terms.
+//   k_constraint_*   per-iteration object constraints; k_probe_com probe
+//                    recentring; k_error_reduce per-iteration error.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pty_core.h"
+#include "../../include/pty_hip.h"
+
+using namespace pty;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PTY_E_LAUNCH, "%s: %s", what, hipGetErrorString(e));
+  return PTY_OK;
+}
+
+bool aligned8(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 7u) == 0; }
+
+inline hipStream_t st(pty_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------
+// fused per-position kernel
+// ---------------------------------------------------------------------------
+template <int S> struct GpuRunner {
+  using K = Fused<S>;
+  static constexpr int T = Frame<S>::T;
+  typename K::Regs& r;
+  LdsPtr lds;
+  const cf* tw;
+  int tid;
+  double* red;
+  const FusedArgs& a;
+  // The empty asm makes the thread id opaque at every phase boundary, so the
+  // per-phase LDS/global address arithmetic is recomputed instead of being
+  // shared (and kept live) across phases.
+  template <class Fn> __device__ __forceinline__ void phase(Fn fn) {
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    fn(t, r, lds, tw);
+  }
+  __device__ __forceinline__ void barrier() { __syncthreads(); }
+  __device__ __forceinline__ void reduce_err(int b) {
+    double v = r.err;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < T / 64; ++w) s += red[w];
+      a.err[b] = s;
+    }
+  }
+};
+
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
+  __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
+  __shared__ cf tw[S];
+  __shared__ double red[Frame<S>::T / 64];
+  const int tid = threadIdx.x;
+  Fused<S>::twiddles(tid, LdsPtr{tw});
+  __syncthreads();
+  typename Fused<S>::Regs r;
+  GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
+  fused_position<S>(run, a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
+}
+
+// ---------------------------------------------------------------------------
+// standalone 2-D FFT of one frame per work group
+//   forward: natural in -> natural (or kernel-layout) out
+//   inverse: natural (or kernel-layout) in -> natural out, scaled 1/S^2
+// ---------------------------------------------------------------------------
+template <int S, bool INV>
+__global__ __launch_bounds__(Frame<S>::T) void k_fft2(const cf* __restrict__ in, cf* __restrict__ out,
+                                                      int natural_io) {
+  using F = Frame<S>;
+  using K = Fused<S>;
+  __shared__ cf lds_mem[F::LDS_WORDS];
+  __shared__ cf tw[S];
+  const int tid = threadIdx.x;
+  K::twiddles(tid, LdsPtr{tw});
+  __syncthreads();
+  const cf* src = in + (size_t)blockIdx.x * S * S;
+  cf* dst = out + (size_t)blockIdx.x * S * S;
+  LdsPtr lds{lds_mem};
+  typename K::Regs r;
+  if (!INV) {
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+#pragma unroll
+    for (int m = 0; m < F::N1; ++m) r.v[m] = src[(n2 + F::N2 * m) * S + cc];
+    Pass<S>::template A<false>(n2, r.v, tw);
+#pragma unroll
+    for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
+    __syncthreads();
+    K::template p_colB<false>(tid, r, lds, tw);
+    __syncthreads();
+    K::template p_rowA<false>(tid, r, lds, tw);
+    __syncthreads();
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
+    Pass<S>::template B<false>(t, r.v, tw);
+    if (natural_io) {
+      const int kr = F::kidx(rr);
+#pragma unroll
+      for (int j = 0; j < F::N1; ++j) dst[kr * S + F::kidx(F::N1 * t + j)] = r.v[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < F::N1; ++j) dst[rr * S + F::N1 * t + j] = r.v[j];
+    }
+  } else {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+    if (natural_io) {
+      const int kr = F::kidx(rr);
+#pragma unroll
+      for (int j = 0; j < F::N1; ++j) r.v[j] = src[kr * S + F::kidx(F::N1 * t + j)];
+    } else {
+#pragma unroll
+      for (int j = 0; j < F::N1; ++j) r.v[j] = src[rr * S + F::N1 * t + j];
+    }
+    Pass<S>::template B<true>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
+    __syncthreads();
+    K::template p_rowA<true>(tid, r, lds, tw);
+    __syncthreads();
+    K::template p_colB<true>(tid, r, lds, tw);
+    __syncthreads();
+    K::p_colAinv_to_regs(tid, r, lds, tw);
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+#pragma unroll
+    for (int m = 0; m < F::N1; ++m) dst[(n2 + F::N2 * m) * S + cc] = r.v[m];
+  }
+}
+
+template <int S>
+__global__ void k_permute(const float* __restrict__ in, float* __restrict__ out, long long total,
+                          int to_kernel) {
+  using F = Frame<S>;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long f = e / (S * S);
+    const int rem = (int)(e - f * S * S);
+    const int p = rem / S, q = rem % S;
+    const long long nat = f * S * S + (long long)F::kidx(p) * S + F::kidx(q);
+    if (to_kernel) out[e] = in[nat];
+    else out[nat] = in[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scatter-add by tile
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GATHER_THREADS) void k_gather(GatherArgs a) {
+  __shared__ float smax[GATHER_THREADS];
+  const int tile = blockIdx.x, tid = threadIdx.x;
+  GatherAcc acc;
+  gather_accumulate(a, tile, tid, acc);
+  smax[tid] = gather_store(a, tile, tid, acc);
+  __syncthreads();
+  for (int s = GATHER_THREADS / 2; s > 0; s >>= 1) {
+    if (tid < s) smax[tid] = fmaxf(smax[tid], smax[tid + s]);
+    __syncthreads();
+  }
+  if (tid == 0) a.tilemax[tile] = smax[0];
+}
+
+__device__ float block_max(float v, float* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  float m = sh[0];
+  __syncthreads();
+  return m;
+}
+
+// 1 / sqrt(1e-16 + (c1 s)^2 + (c2 m)^2) in float32, without contraction
+__device__ __forceinline__ float overlap_norm(float s, float m, float c1, float c2) {
+  const float a = __fmul_rn(c1, s);
+  float t = __fadd_rn(1e-16f, __fmul_rn(a, a));
+  const float b = __fmul_rn(c2, m);
+  t = __fadd_rn(t, __fmul_rn(b, b));
+  return 1.0f / sqrtf(t);
+}
+
+__device__ __forceinline__ cf step_add(cf o, cf g, float nrm, float step) {
+  const float ux = __fmul_rn(step, __fmul_rn(g.x, nrm));
+  const float uy = __fmul_rn(step, __fmul_rn(g.y, nrm));
+  return mkcf(__fadd_rn(o.x, ux), __fadd_rn(o.y, uy));
+}
+
+__global__ __launch_bounds__(256) void k_object_update(cf* __restrict__ obj, const cf* __restrict__ grad,
+                                                       const float* __restrict__ spsum,
+                                                       const float* __restrict__ tilemax, int ntiles,
+                                                       long long npix, float step, float c1, float c2) {
+  __shared__ float sh[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) m = fmaxf(m, tilemax[i]);
+  m = block_max(m, sh);
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    obj[p] = step_add(obj[p], grad[p], overlap_norm(spsum[p], m, c1, c2), step);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// probe update: partial sums over chunks of the batch, then one work group
+// ---------------------------------------------------------------------------
+constexpr int PROBE_CHUNK = 64;
+constexpr int PROBE_UPDATE_THREADS = 1024;
+
+struct ProbePartialArgs {
+  const cf* chi;
+  const cf* object;
+  const float* positions;
+  const int* order;
+  int N, B, S, Px, Py;
+  double* part;  // [nchunk][S*S][3] (re, im, |O|^2)
+};
+
+__global__ __launch_bounds__(256) void k_probe_partial(ProbePartialArgs a) {
+  const int S = a.S;
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunk = blockIdx.y;
+  if (pix >= S * S) return;
+  const int i = pix / S, j = pix % S;
+  const int fi = i < S / 2 ? i : i - S;
+  const int fj = j < S / 2 ? j : j - S;
+  const int b1 = min(a.B, (chunk + 1) * PROBE_CHUNK);
+  double re = 0.0, im = 0.0, so = 0.0;
+  for (int b = chunk * PROBE_CHUNK; b < b1; ++b) {
+    int idx = a.order[b];
+    idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
+    const int x0 = (int)rintf(a.positions[2 * idx]);
+    const int y0 = (int)rintf(a.positions[2 * idx + 1]);
+    const cf o = a.object[(size_t)wrapi(x0 + fi, a.Px) * a.Py + wrapi(y0 + fj, a.Py)];
+    const float ao = hypotf(o.x, o.y);
+    so += (double)(ao * ao);
+    const cf g = cconjmul(o, a.chi[((size_t)b * S + i) * S + j]);
+    re += (double)g.x;
+    im += (double)g.y;
+  }
+  double* p = a.part + ((size_t)chunk * S * S + pix) * 3;
+  p[0] = re;
+  p[1] = im;
+  p[2] = so;
+}
+
+template <int S>
+__global__ __launch_bounds__(PROBE_UPDATE_THREADS) void k_probe_update(cf* __restrict__ probe,
+                                                                       const double* __restrict__ part,
+                                                                       int nchunk, float step, float c1,
+                                                                       float c2) {
+  constexpr int PER = S * S / PROBE_UPDATE_THREADS;
+  __shared__ float sh[PROBE_UPDATE_THREADS];
+  cf g[PER];
+  float so[PER];
+  float m = 0.f;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int pix = threadIdx.x + k * PROBE_UPDATE_THREADS;
+    double re = 0.0, im = 0.0, s = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+      const double* p = part + ((size_t)c * S * S + pix) * 3;
+      re += p[0];
+      im += p[1];
+      s += p[2];
+    }
+    g[k] = mkcf((float)re, (float)im);
+    so[k] = (float)s;
+    m = fmaxf(m, so[k]);
+  }
+  m = block_max(m, sh);
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int pix = threadIdx.x + k * PROBE_UPDATE_THREADS;
+    probe[pix] = step_add(probe[pix], g[k], overlap_norm(so[k], m, c1, c2), step);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// constraints
+// ---------------------------------------------------------------------------
+constexpr int CONSTRAINT_BLOCKS = 256;
+
+__global__ __launch_bounds__(256) void k_constraint_partial(const cf* __restrict__ obj,
+                                                            const unsigned char* __restrict__ mask,
+                                                            long long npix, double* part) {
+  __shared__ double ssum[256];
+  __shared__ double scnt[256];
+  double s = 0.0, c = 0.0;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    if (mask[p]) {
+      const cf o = obj[p];
+      s += (double)atan2f(o.y, o.x);
+      c += 1.0;
+    }
+  }
+  ssum[threadIdx.x] = s;
+  scnt[threadIdx.x] = c;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      ssum[threadIdx.x] += ssum[threadIdx.x + k];
+      scnt[threadIdx.x] += scnt[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ssum[0];
+    part[2 * blockIdx.x + 1] = scnt[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_constraint_apply(cf* __restrict__ obj, long long npix,
+                                                          const double* __restrict__ part, int nparts,
+                                                          int use_mask, float shrinkage_rad,
+                                                          int pure_phase) {
+  __shared__ float shrink_sh;
+  if (threadIdx.x == 0) {
+    float shrink = shrinkage_rad;
+    if (use_mask) {
+      double s = 0.0, c = 0.0;
+      for (int k = 0; k < nparts; ++k) {
+        s += part[2 * k];
+        c += part[2 * k + 1];
+      }
+      shrink = shrink + (float)(s / c);
+    }
+    shrink_sh = shrink;
+  }
+  __syncthreads();
+  const float shrink = shrink_sh;
+  const bool do_shrink = use_mask || shrinkage_rad > 0.f;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    cf o = obj[p];
+    if (do_shrink) {
+      const float ph = atan2f(o.y, o.x) - shrink;
+      const float am = hypotf(o.x, o.y);
+      float sn, cs;
+      sincosf(ph, &sn, &cs);
+      o = mkcf(am * cs, am * sn);
+    }
+    const float ph = atan2f(o.y, o.x);
+    const float am = pure_phase ? 1.0f : fminf(hypotf(o.x, o.y), 1.0f);
+    float sn, cs;
+    sincosf(ph, &sn, &cs);
+    obj[p] = mkcf(am * cs, am * sn);
+  }
+}
+
+// probe centre of mass (corner-centred coordinates) and Fourier recentring
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_probe_com(cf* __restrict__ probe) {
+  using F = Frame<S>;
+  using K = Fused<S>;
+  constexpr int T = F::T;
+  __shared__ cf lds_mem[F::LDS_WORDS];
+  __shared__ cf tw[S];
+  __shared__ double red[3][T / 64];
+  __shared__ float shift[2];
+  const int tid = threadIdx.x;
+  K::twiddles(tid, LdsPtr{tw});
+  LdsPtr lds{lds_mem};
+  typename K::Regs r;
+  const int cc = F::col_line(tid), n2 = F::col_q(tid);
+  double sI = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) {
+    const int i = n2 + F::N2 * m;
+    r.v[m] = probe[i * S + cc];
+    const float a = hypotf(r.v[m].x, r.v[m].y);
+    const double I = (double)(a * a);
+    sI += I;
+    sx += (double)F::fidx(i) * I;
+    sy += (double)F::fidx(cc) * I;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sI += __shfl_xor(sI, off);
+    sx += __shfl_xor(sx, off);
+    sy += __shfl_xor(sy, off);
+  }
+  if ((tid & 63) == 0) {
+    red[0][tid >> 6] = sI;
+    red[1][tid >> 6] = sx;
+    red[2][tid >> 6] = sy;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int w = 0; w < T / 64; ++w) { a += red[0][w]; b += red[1][w]; c += red[2][w]; }
+    shift[0] = (float)(-(b / a));
+    shift[1] = (float)(-(c / a));
+  }
+  // forward fft2 (columns then rows) with the ramp applied in kernel layout
+  Pass<S>::template A<false>(n2, r.v, tw);
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
+  __syncthreads();
+  K::template p_colB<false>(tid, r, lds, tw);
+  __syncthreads();
+  K::template p_rowA<false>(tid, r, lds, tw);
+  __syncthreads();
+  {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
+    Pass<S>::template B<false>(t, r.v, tw);
+    const cf ex = K::ramp1(F::kidx(rr), shift[0]);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) {
+      const cf e = cmul(ex, K::ramp1(F::kidx(F::N1 * t + j), shift[1]));
+      r.v[j] = cmul(r.v[j], e);
+    }
+    Pass<S>::template B<true>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
+  }
+  __syncthreads();
+  K::template p_rowA<true>(tid, r, lds, tw);
+  __syncthreads();
+  K::template p_colB<true>(tid, r, lds, tw);
+  __syncthreads();
+  K::p_colAinv_to_regs(tid, r, lds, tw);
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) probe[(n2 + F::N2 * m) * S + cc] = r.v[m];
+}
+
+// per-iteration error: fixed-order float64 sum, normalised
+__global__ __launch_bounds__(1024) void k_error_reduce(const double* __restrict__ err, int n,
+                                                       double norm, double* __restrict__ out) {
+  __shared__ double sh[1024];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += err[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0] / norm;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+bool frame_supported(int S) { return S == 64 || S == 128; }
+
+int launch_fused(const FusedArgs& a, int S, hipStream_t stream) {
+  const int grid = a.B - a.b0;
+  if (grid <= 0) return PTY_OK;
+  if (S == 64) hipLaunchKernelGGL(k_fused<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+  else hipLaunchKernelGGL(k_fused<128>, dim3(grid), dim3(Frame<128>::T), 0, stream, a);
+  return check_launch("k_fused");
+}
+
+int launch_fft(const cf* in, cf* out, int batch, int S, bool inv, int natural, hipStream_t stream) {
+  if (batch <= 0) return PTY_OK;
+  if (S == 64) {
+    if (inv) hipLaunchKernelGGL((k_fft2<64, true>), dim3(batch), dim3(Frame<64>::T), 0, stream, in, out, natural);
+    else hipLaunchKernelGGL((k_fft2<64, false>), dim3(batch), dim3(Frame<64>::T), 0, stream, in, out, natural);
+  } else {
+    if (inv) hipLaunchKernelGGL((k_fft2<128, true>), dim3(batch), dim3(Frame<128>::T), 0, stream, in, out, natural);
+    else hipLaunchKernelGGL((k_fft2<128, false>), dim3(batch), dim3(Frame<128>::T), 0, stream, in, out, natural);
+  }
+  return check_launch("k_fft2");
+}
+
+int num_tiles(int Px, int Py) { return ((Px + TILE - 1) / TILE) * ((Py + TILE - 1) / TILE); }
+
+int launch_gather(const GatherArgs& a, int Px, int Py, hipStream_t stream) {
+  hipLaunchKernelGGL(k_gather, dim3(num_tiles(Px, Py)), dim3(GATHER_THREADS), 0, stream, a);
+  return check_launch("k_gather");
+}
+
+int grid_for(long long n, int threads, int cap = 4096) {
+  long long g = (n + threads - 1) / threads;
+  return (int)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+int launch_object_update(cf* obj, const cf* grad, const float* spsum, const float* tilemax, int ntiles,
+                         int Px, int Py, float step, float nmin, hipStream_t stream) {
+  const long long npix = (long long)Px * Py;
+  const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
+  hipLaunchKernelGGL(k_object_update, dim3(grid_for(npix, 256)), dim3(256), 0, stream, obj, grad, spsum,
+                     tilemax, ntiles, npix, step, c1, c2);
+  return check_launch("k_object_update");
+}
+
+int probe_nchunk(int B) { return (B + PROBE_CHUNK - 1) / PROBE_CHUNK; }
+
+int launch_probe_update(cf* probe, const cf* chi, const cf* obj, int Px, int Py, const float* pos, int N,
+                        const int* order, int B, int S, float step, float nmin, double* part,
+                        hipStream_t stream) {
+  ProbePartialArgs pa{chi, obj, pos, order, N, B, S, Px, Py, part};
+  const int nchunk = probe_nchunk(B);
+  hipLaunchKernelGGL(k_probe_partial, dim3((S * S + 255) / 256, nchunk), dim3(256), 0, stream, pa);
+  int rc = check_launch("k_probe_partial");
+  if (rc) return rc;
+  const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
+  if (S == 64)
+    hipLaunchKernelGGL(k_probe_update<64>, dim3(1), dim3(PROBE_UPDATE_THREADS), 0, stream, probe, part,
+                       nchunk, step, c1, c2);
+  else
+    hipLaunchKernelGGL(k_probe_update<128>, dim3(1), dim3(PROBE_UPDATE_THREADS), 0, stream, probe, part,
+                       nchunk, step, c1, c2);
+  return check_launch("k_probe_update");
+}
+
+int launch_constraints(cf* obj, int Px, int Py, const unsigned char* mask, float shrink, int pure,
+                       double* part, hipStream_t stream) {
+  const long long npix = (long long)Px * Py;
+  if (mask) {
+    hipLaunchKernelGGL(k_constraint_partial, dim3(CONSTRAINT_BLOCKS), dim3(256), 0, stream, obj, mask, npix,
+                       part);
+    int rc = check_launch("k_constraint_partial");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_constraint_apply, dim3(grid_for(npix, 256)), dim3(256), 0, stream, obj, npix, part,
+                     CONSTRAINT_BLOCKS, mask ? 1 : 0, shrink, pure);
+  return check_launch("k_constraint_apply");
+}
+
+int launch_probe_com(cf* probe, int S, hipStream_t stream) {
+  if (S == 64) hipLaunchKernelGGL(k_probe_com<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe);
+  else hipLaunchKernelGGL(k_probe_com<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe);
+  return check_launch("k_probe_com");
+}
+
+// near-equal batch sizes, larger ones first (phase/utils.py:539-579)
+std::vector<int> subdivide(int n, int max_batch) {
+  const int nb = (n + ((max_batch - n % max_batch) % max_batch)) / max_batch;
+  std::vector<int> v(nb, n / nb);
+  const int extra = n % nb;
+  for (int i = 0; i < extra; ++i) v[i] += 1;
+  return v;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct WsLayout {
+  size_t phat, ps, chi, err, grad, spsum, tilemax, ppart, cpart, total;
+};
+
+WsLayout layout(const pty_problem& p, const pty_gd_params& q) {
+  WsLayout L{};
+  const size_t S2 = (size_t)p.S * p.S;
+  const size_t B = (size_t)q.max_batch;
+  const size_t npix = (size_t)p.Px * p.Py;
+  size_t o = 0;
+  L.phat = o; o = align256(o + S2 * 8);
+  L.ps = o; o = align256(o + B * S2 * 8);
+  L.chi = o; o = align256(o + B * S2 * 8);
+  L.err = o; o = align256(o + (size_t)p.N * 8);
+  L.grad = o; o = align256(o + npix * 8);
+  L.spsum = o; o = align256(o + npix * 4);
+  L.tilemax = o; o = align256(o + (size_t)num_tiles(p.Px, p.Py) * 4);
+  L.ppart = o; o = align256(o + (q.fix_probe ? 0 : (size_t)probe_nchunk(q.max_batch) * S2 * 24));
+  L.cpart = o; o = align256(o + (size_t)CONSTRAINT_BLOCKS * 16);
+  L.total = o;
+  return L;
+}
+
+int validate_problem(const pty_problem* p) {
+  if (!p) return fail(PTY_E_ARG, "problem is NULL");
+  if (!frame_supported(p->S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported (64 or 128)", p->S);
+  if (p->Px < p->S || p->Py < p->S)
+    return fail(PTY_E_ARG, "object %dx%d smaller than the %dx%d frame", p->Px, p->Py, p->S, p->S);
+  if (p->N <= 0) return fail(PTY_E_ARG, "no scan positions");
+  if (!p->object || !p->probe || !p->positions || !p->amplitudes)
+    return fail(PTY_E_ARG, "null device buffer in problem");
+  if (!aligned8(p->object) || !aligned8(p->probe)) return fail(PTY_E_ARG, "complex buffers must be 8-byte aligned");
+  return PTY_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int pty_version(void) { return 10000; }
+
+const char* pty_last_error(void) { return g_err.c_str(); }
+
+int pty_fft2_c64(const void* in, void* out, int batch, int S, int inverse, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (batch < 0 || !in || !out) return fail(PTY_E_ARG, "bad fft2 arguments");
+  if (in == out) return fail(PTY_E_ARG, "fft2 must not run in place");
+  return launch_fft((const cf*)in, (cf*)out, batch, S, inverse != 0, 1, st(stream));
+}
+
+int pty_permute_frames(const float* in, float* out, int count, int S, int direction, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (count < 0 || !in || !out || in == out) return fail(PTY_E_ARG, "bad permute arguments");
+  const long long total = (long long)count * S * S;
+  if (total == 0) return PTY_OK;
+  const int to_kernel = direction == 0;
+  if (S == 64) hipLaunchKernelGGL(k_permute<64>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
+  else hipLaunchKernelGGL(k_permute<128>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
+  return check_launch("k_permute");
+}
+
+int pty_probe_spectrum(const void* probe, void* probe_hat, int S, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!probe || !probe_hat || probe == probe_hat) return fail(PTY_E_ARG, "bad probe spectrum arguments");
+  return launch_fft((const cf*)probe, (cf*)probe_hat, 1, S, false, 0, st(stream));
+}
+
+int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px, int Py, const float* positions,
+                              int N, const float* amplitudes, const int* order, int B, int S, void* ps, void* chi,
+                              double* err, void* far, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (Px < S || Py < S) return fail(PTY_E_ARG, "object %dx%d smaller than frame %d", Px, Py, S);
+  if (N <= 0 || B < 0) return fail(PTY_E_ARG, "bad N=%d or B=%d", N, B);
+  if (!probe_hat || !object || !positions || !amplitudes || !order || !ps || !chi || !err)
+    return fail(PTY_E_ARG, "null buffer in forward_adjoint_batch");
+  FusedArgs a{};
+  a.probe_hat = (const cf*)probe_hat;
+  a.object = (const cf*)object;
+  a.positions = positions;
+  a.amps = amplitudes;
+  a.order = order;
+  a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0;
+  a.ps_out = (cf*)ps; a.chi_out = (cf*)chi; a.err = err; a.fwd_out = (cf*)far;
+  return launch_fused(a, S, st(stream));
+}
+
+int pty_scatter_patches(const void* ps, const void* chi, const float* positions, int N, const int* order, int B,
+                        int S, int Px, int Py, void* grad, float* spsum, float* tilemax, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (Px < S || Py < S || N <= 0 || B < 0) return fail(PTY_E_ARG, "bad scatter sizes");
+  if (!ps || !chi || !positions || !order || !grad || !spsum || !tilemax) return fail(PTY_E_ARG, "null buffer in scatter");
+  GatherArgs a{};
+  a.ps = (const cf*)ps; a.chi = (const cf*)chi; a.positions = positions; a.order = order;
+  a.N = N; a.B = B; a.S = S; a.Px = Px; a.Py = Py; a.tiles_y = (Py + TILE - 1) / TILE;
+  a.grad = (cf*)grad; a.spsum = spsum; a.tilemax = tilemax;
+  return launch_gather(a, Px, Py, st(stream));
+}
+
+int pty_object_update(void* object, const void* grad, const float* spsum, const float* tilemax, int ntiles, int Px,
+                      int Py, float step_size, float normalization_min, pty_stream_t stream) {
+  if (!object || !grad || !spsum || !tilemax || ntiles <= 0 || Px <= 0 || Py <= 0)
+    return fail(PTY_E_ARG, "bad object update arguments");
+  return launch_object_update((cf*)object, (const cf*)grad, spsum, tilemax, ntiles, Px, Py, step_size,
+                              normalization_min, st(stream));
+}
+
+size_t pty_probe_partials_bytes(int S, int B) { return (size_t)probe_nchunk(B) * S * S * 24; }
+
+int pty_probe_update(void* probe, const void* chi, const void* object, int Px, int Py, const float* positions, int N,
+                     const int* order, int B, int S, float step_size, float normalization_min, void* partials,
+                     pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!probe || !chi || !object || !positions || !order || !partials || B <= 0 || N <= 0)
+    return fail(PTY_E_ARG, "bad probe update arguments");
+  return launch_probe_update((cf*)probe, (const cf*)chi, (const cf*)object, Px, Py, positions, N, order, B, S,
+                             step_size, normalization_min, (double*)partials, st(stream));
+}
+
+size_t pty_constraint_partials_bytes(void) { return (size_t)CONSTRAINT_BLOCKS * 16; }
+
+int pty_object_constraints(void* object, int Px, int Py, const unsigned char* mask, float shrinkage_rad,
+                           int pure_phase_object, void* partials, pty_stream_t stream) {
+  if (!object || Px <= 0 || Py <= 0 || (mask && !partials)) return fail(PTY_E_ARG, "bad constraint arguments");
+  return launch_constraints((cf*)object, Px, Py, mask, shrinkage_rad, pure_phase_object, (double*)partials,
+                            st(stream));
+}
+
+int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!probe) return fail(PTY_E_ARG, "probe is NULL");
+  return launch_probe_com((cf*)probe, S, st(stream));
+}
+
+size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params) {
+  if (!problem || !params || params->max_batch <= 0) return 0;
+  return layout(*problem, *params).total;
+}
+
+int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const int* orders, double* errors,
+               void* workspace, size_t workspace_bytes, pty_stream_t stream_) {
+  int rc = validate_problem(problem);
+  if (rc) return rc;
+  if (!params) return fail(PTY_E_ARG, "params is NULL");
+  const pty_problem& p = *problem;
+  const pty_gd_params& q = *params;
+  if (q.num_iter < 0 || q.max_batch <= 0 || q.max_batch > p.N)
+    return fail(PTY_E_ARG, "bad num_iter=%d / max_batch=%d for N=%d", q.num_iter, q.max_batch, p.N);
+  if (!orders || !errors || !workspace) return fail(PTY_E_ARG, "null orders/errors/workspace");
+  const WsLayout L = layout(p, q);
+  if (workspace_bytes < L.total)
+    return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.total);
+  hipStream_t stream = st(stream_);
+  char* ws = (char*)workspace;
+  cf* phat = (cf*)(ws + L.phat);
+  cf* ps = (cf*)(ws + L.ps);
+  cf* chi = (cf*)(ws + L.chi);
+  double* err = (double*)(ws + L.err);
+  cf* grad = (cf*)(ws + L.grad);
+  float* spsum = (float*)(ws + L.spsum);
+  float* tilemax = (float*)(ws + L.tilemax);
+  double* ppart = (double*)(ws + L.ppart);
+  double* cpart = (double*)(ws + L.cpart);
+  cf* obj = (cf*)p.object;
+  cf* probe = (cf*)p.probe;
+  const std::vector<int> batches = subdivide(p.N, q.max_batch);
+  const int ntiles = num_tiles(p.Px, p.Py);
+  const double norm = (double)(float)(p.mean_intensity * (float)p.N);
+  bool phat_valid = false;
+  for (int it = 0; it < q.num_iter; ++it) {
+    const int* order_it = orders + (size_t)it * p.N;
+    int start = 0;
+    for (int nb : batches) {
+      const int* ob = order_it + start;
+      if (!phat_valid) {
+        if ((rc = launch_fft(probe, phat, 1, p.S, false, 0, stream))) return rc;
+        phat_valid = q.fix_probe != 0;
+      }
+      FusedArgs a{};
+      a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes; a.order = ob;
+      a.N = p.N; a.B = nb; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
+      a.ps_out = ps; a.chi_out = chi; a.err = err + start; a.fwd_out = nullptr;
+      if ((rc = launch_fused(a, p.S, stream))) return rc;
+      if (!q.fix_probe) {
+        ProbePartialArgs pa{chi, obj, p.positions, ob, p.N, nb, p.S, p.Px, p.Py, ppart};
+        hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(nb)), dim3(256), 0, stream,
+                           pa);
+        if ((rc = check_launch("k_probe_partial"))) return rc;
+      }
+      GatherArgs g{};
+      g.ps = ps; g.chi = chi; g.positions = p.positions; g.order = ob; g.N = p.N; g.B = nb; g.S = p.S;
+      g.Px = p.Px; g.Py = p.Py; g.tiles_y = (p.Py + TILE - 1) / TILE; g.grad = grad; g.spsum = spsum;
+      g.tilemax = tilemax;
+      if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+      if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
+                                     q.normalization_min, stream)))
+        return rc;
+      if (!q.fix_probe) {
+        const float c1 = (float)(1.0 - (double)q.normalization_min), c2 = q.normalization_min;
+        if (p.S == 64)
+          hipLaunchKernelGGL(k_probe_update<64>, dim3(1), dim3(PROBE_UPDATE_THREADS), 0, stream, probe, ppart,
+                             probe_nchunk(nb), q.step_size, c1, c2);
+        else
+          hipLaunchKernelGGL(k_probe_update<128>, dim3(1), dim3(PROBE_UPDATE_THREADS), 0, stream, probe, ppart,
+                             probe_nchunk(nb), q.step_size, c1, c2);
+        if ((rc = check_launch("k_probe_update"))) return rc;
+      }
+      start += nb;
+    }
+    hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
+    if ((rc = check_launch("k_error_reduce"))) return rc;
+    if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
+                                 stream)))
+      return rc;
+    if (!q.fix_probe && q.fix_probe_com) {
+      if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
+    }
+  }
+  return PTY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+"""Tensor-level wrappers over libpty_hip.so.
+
+Every function takes and returns torch tensors on the current ROCm device and
+launches on the current torch stream; shapes and dtypes are checked here so
+that the C-ABI only ever sees consistent pointers.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import PtyGdParams, PtyProblem, c_void_p, ptr, stream_handle
+
+SUPPORTED_FRAMES = (64, 128)
+TILE = 32
+
+
+def _req(t, dtype, shape=None, name="tensor"):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a CUDA/ROCm tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+def check_frame(S):
+    if S not in SUPPORTED_FRAMES:
+        raise NotImplementedError(f"frame size {S} not supported (compiled for {SUPPORTED_FRAMES})")
+
+
+def kernel_index(S):
+    """Natural frequency index held at each kernel-layout position."""
+    check_frame(S)
+    N1, N2 = (16, 8) if S == 128 else (8, 8)
+    p = np.arange(S)
+    return p // N2 + N1 * (p % N2)
+
+
+def fft2(x, inverse=False):
+    """Batched 2-D FFT of (..., S, S) complex64 frames (numpy semantics)."""
+    S = x.shape[-1]
+    check_frame(S)
+    _req(x, torch.complex64, name="x")
+    if x.shape[-2] != S:
+        raise ValueError("frames must be square")
+    out = torch.empty_like(x)
+    batch = x.numel() // (S * S)
+    _lib.call("pty_fft2_c64", ptr(x), ptr(out), batch, S, int(bool(inverse)), stream_handle())
+    return out
+
+
+def to_kernel_layout(frames, inverse=False):
+    """Reorder real (..., S, S) float32 frames to (or from) kernel layout."""
+    S = frames.shape[-1]
+    check_frame(S)
+    _req(frames, torch.float32, name="frames")
+    out = torch.empty_like(frames)
+    _lib.call("pty_permute_frames", ptr(frames), ptr(out), frames.numel() // (S * S), S,
+              1 if inverse else 0, stream_handle())
+    return out
+
+
+def probe_spectrum(probe):
+    S = probe.shape[-1]
+    check_frame(S)
+    _req(probe, torch.complex64, (S, S), "probe")
+    out = torch.empty_like(probe)
+    _lib.call("pty_probe_spectrum", ptr(probe), ptr(out), S, stream_handle())
+    return out
+
+
+def forward_adjoint(probe_hat, obj, positions, amplitudes_k, order, with_far=False):
+    """Fused forward pass for the slots of ``order``; returns
+    (P_shift, chi, err[, far]) with P_shift/chi natural (B, S, S)."""
+    S = probe_hat.shape[-1]
+    check_frame(S)
+    _req(probe_hat, torch.complex64, (S, S), "probe_hat")
+    _req(obj, torch.complex64, name="object")
+    N = positions.shape[0]
+    _req(positions, torch.float32, (N, 2), "positions")
+    _req(amplitudes_k, torch.float32, (N, S, S), "amplitudes")
+    _req(order, torch.int32, name="order")
+    B = order.numel()
+    Px, Py = obj.shape
+    ps = torch.empty((B, S, S), dtype=torch.complex64, device=obj.device)
+    chi = torch.empty_like(ps)
+    err = torch.empty(B, dtype=torch.float64, device=obj.device)
+    far = torch.empty_like(ps) if with_far else None
+    _lib.call("pty_forward_adjoint_batch", ptr(probe_hat), ptr(obj), Px, Py, ptr(positions), N,
+              ptr(amplitudes_k), ptr(order), B, S, ptr(ps), ptr(chi), ptr(err), ptr(far),
+              stream_handle())
+    return (ps, chi, err, far) if with_far else (ps, chi, err)
+
+
+def num_tiles(Px, Py):
+    return ((Px + TILE - 1) // TILE) * ((Py + TILE - 1) // TILE)
+
+
+def scatter(ps, chi, positions, order, object_shape):
+    """(scatter(conj(ps) chi), scatter(|ps|^2), per-tile max)."""
+    B, S, _ = ps.shape
+    Px, Py = object_shape
+    dev = ps.device
+    grad = torch.empty((Px, Py), dtype=torch.complex64, device=dev)
+    spsum = torch.empty((Px, Py), dtype=torch.float32, device=dev)
+    tmax = torch.empty(num_tiles(Px, Py), dtype=torch.float32, device=dev)
+    _lib.call("pty_scatter_patches", ptr(ps), ptr(chi), ptr(positions), positions.shape[0],
+              ptr(order), B, S, Px, Py, ptr(grad), ptr(spsum), ptr(tmax), stream_handle())
+    return grad, spsum, tmax
+
+
+def object_update(obj, grad, spsum, tilemax, step_size, normalization_min):
+    Px, Py = obj.shape
+    _lib.call("pty_object_update", ptr(obj), ptr(grad), ptr(spsum), ptr(tilemax), tilemax.numel(),
+              Px, Py, float(step_size), float(normalization_min), stream_handle())
+    return obj
+
+
+def probe_update(probe, chi, obj, positions, order, step_size, normalization_min):
+    S = probe.shape[-1]
+    B = order.numel()
+    nbytes = _lib.load().pty_probe_partials_bytes(S, B)
+    part = torch.empty(nbytes, dtype=torch.uint8, device=probe.device)
+    Px, Py = obj.shape
+    _lib.call("pty_probe_update", ptr(probe), ptr(chi), ptr(obj), Px, Py, ptr(positions),
+              positions.shape[0], ptr(order), B, S, float(step_size), float(normalization_min),
+              ptr(part), stream_handle())
+    return probe
+
+
+def object_constraints(obj, mask_inverse=None, shrinkage_rad=0.0, pure_phase_object=False):
+    Px, Py = obj.shape
+    part = torch.empty(_lib.load().pty_constraint_partials_bytes(), dtype=torch.uint8,
+                       device=obj.device)
+    if mask_inverse is not None:
+        _req(mask_inverse, torch.uint8, (Px, Py), "mask")
+    _lib.call("pty_object_constraints", ptr(obj), Px, Py, ptr(mask_inverse), float(shrinkage_rad),
+              int(bool(pure_phase_object)), ptr(part), stream_handle())
+    return obj
+
+
+def probe_com_constraint(probe):
+    _lib.call("pty_probe_com_constraint", ptr(probe), probe.shape[-1], stream_handle())
+    return probe
+
+
+def make_problem(obj, probe, positions, amplitudes_k, mask_inverse, mean_intensity):
+    S = probe.shape[-1]
+    Px, Py = obj.shape
+    return PtyProblem(S, Px, Py, positions.shape[0], c_void_p(obj.data_ptr()),
+                      c_void_p(probe.data_ptr()), c_void_p(positions.data_ptr()),
+                      c_void_p(amplitudes_k.data_ptr()),
+                      c_void_p(mask_inverse.data_ptr()) if mask_inverse is not None else None,
+                      float(mean_intensity))
+
+
+def gd_workspace_bytes(problem, params):
+    import ctypes
+    return int(_lib.load().pty_gd_workspace_bytes(ctypes.byref(problem), ctypes.byref(params)))
+
+
+def gd_run(problem, params, orders, errors, workspace):
+    import ctypes
+    _lib.call("pty_gd_run", ctypes.byref(problem), ctypes.byref(params), ptr(orders), ptr(errors),
+              ptr(workspace), workspace.numel(), stream_handle())
+
+
+__all__ = ["fft2", "to_kernel_layout", "probe_spectrum", "forward_adjoint", "scatter",
+           "object_update", "probe_update", "object_constraints", "probe_com_constraint",
+           "make_problem", "gd_workspace_bytes", "gd_run", "PtyProblem", "PtyGdParams",
+           "kernel_index", "num_tiles"]

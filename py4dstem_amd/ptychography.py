@@ -1,0 +1,609 @@
+"""Drop-in ``SingleslicePtychography`` running its GD loop on MI355X.
+
+Keeps the constructor / ``preprocess()`` / ``reconstruct()`` signatures,
+return-self chaining, error behaviour and result attributes of
+py4DSTEM's class (py4DSTEM/process/phase/singleslice_ptychography.py:42-996),
+so callers such as ``PtychographyOptimizer``
+(phase/parameter_optimize.py:580-660) can switch by import.
+
+The one-time ``preprocess()`` set-up runs on the host (NumPy/SciPy) plus one
+GPU pass for the probe-overlap field; every ``reconstruct()`` iteration runs
+in libpty_hip.so (one native call per run).  Options of the reference that
+this build does not implement raise ``NotImplementedError`` rather than being
+silently ignored.
+"""
+
+from __future__ import annotations
+
+import sys
+import warnings
+
+import numpy as np
+import scipy.ndimage
+import torch
+
+from . import kernels as K
+from ._lib import PtyGdParams, require_gpu
+from .preprocess import (
+    center_positions,
+    extract_calibrations,
+    fit_origin_plane,
+    initial_object_shape,
+    intensities_center_of_mass,
+    real_space_sampling,
+    scan_positions_in_pixels,
+    solve_com_rotation,
+)
+from .probe import POLAR_ALIASES, POLAR_SYMBOLS, ComplexProbe, polar_parameters_from
+
+warnings.simplefilter("always", UserWarning)
+
+_GD_NAMES = ("GD", "gradient-descent")
+_PROJECTION_NAMES = ("generalized-projections", "DM_AP", "difference-map_alternating-projections",
+                     "RAAR", "relaxed-averaged-alternating-reflections", "RRR",
+                     "relax-reflect-reflect", "SUPERFLIP", "charge-flipping")
+
+
+def _warn(msg):
+    print(msg, file=sys.stderr)
+
+
+def subdivide_into_batches(num_items, max_batch):
+    """Near-equal batch sizes, larger first (phase/utils.py:539-579)."""
+    num_batches = (num_items + (-num_items % max_batch)) // max_batch
+    if num_items < num_batches:
+        raise RuntimeError("num_batches may not be larger than num_items")
+    base, extra = divmod(num_items, num_batches)
+    return [base + 1] * extra + [base] * (num_batches - extra)
+
+
+def bilinear_shift_periodic(frames, xshift, yshift):
+    """Per-frame bilinear periodic shift (py4DSTEM/preprocess/utils.py:59-121,
+    bilinear=True): out = sum of four rolled copies with float32 weights."""
+    n, qx, qy = frames.shape
+    xF = np.floor(xshift).astype(int)
+    yF = np.floor(yshift).astype(int)
+    wx = (xshift - xF).astype(np.float32)
+    wy = (yshift - yF).astype(np.float32)
+    ix = np.arange(qx)
+    iy = np.arange(qy)
+    b = np.arange(n)[:, None, None]
+
+    def rolled(dx, dy):
+        rows = (ix[None, :] - (xF + dx)[:, None]) % qx
+        cols = (iy[None, :] - (yF + dy)[:, None]) % qy
+        return frames[b, rows[:, :, None], cols[:, None, :]]
+
+    one = np.float32(1)
+    w00 = ((one - wx) * (one - wy))[:, None, None]
+    w10 = (wx * (one - wy))[:, None, None]
+    w01 = ((one - wx) * wy)[:, None, None]
+    w11 = (wx * wy)[:, None, None]
+    return rolled(0, 0) * w00 + rolled(1, 0) * w10 + rolled(0, 1) * w01 + rolled(1, 1) * w11
+
+
+class SingleslicePtychography:
+    """Single-slice ptychographic reconstruction (GD) on MI355X.
+
+    Parameters mirror py4DSTEM's ``SingleslicePtychography``
+    (singleslice_ptychography.py:53-116).  ``device`` / ``storage`` accept
+    'cpu' or 'gpu' for call compatibility; the iteration always runs in the
+    HIP library on the current ROCm device.
+    """
+
+    _class_specific_metadata = ()
+
+    def __init__(self, energy, datacube=None, semiangle_cutoff=None, semiangle_cutoff_pixels=None,
+                 rolloff=2.0, vacuum_probe_intensity=None, polar_parameters=None,
+                 initial_object_guess=None, initial_probe_guess=None, initial_scan_positions=None,
+                 object_fov_ang=None, positions_offset_ang=None, object_padding_px=None,
+                 object_type="complex", positions_mask=None, verbose=True, device="cpu",
+                 storage=None, clear_fft_cache=True, name="ptychographic_reconstruction",
+                 **kwargs):
+        if storage is None:
+            storage = device
+        self.set_device(device, clear_fft_cache)
+        self.set_storage(storage)
+        for key in kwargs:
+            if key not in POLAR_SYMBOLS and key not in POLAR_ALIASES:
+                raise ValueError("{} not a recognized parameter".format(key))
+        merged = dict(polar_parameters or {})
+        merged.update(kwargs)
+        self._polar_parameters = polar_parameters_from(merged)
+        if object_type != "potential" and object_type != "complex":
+            raise ValueError(
+                f"object_type must be either 'potential' or 'complex', not {object_type}")
+        self.name = name
+        self.set_save_defaults()
+        self._datacube = datacube
+        self._object = initial_object_guess
+        self._probe = initial_probe_guess
+        self._vacuum_probe_intensity = vacuum_probe_intensity
+        self._scan_positions = initial_scan_positions
+        self._positions_offset_ang = positions_offset_ang
+        self._energy = energy
+        self._semiangle_cutoff = semiangle_cutoff
+        self._semiangle_cutoff_pixels = semiangle_cutoff_pixels
+        self._rolloff = rolloff
+        self._object_type = object_type
+        self._object_padding_px = object_padding_px
+        self._object_fov_ang = object_fov_ang
+        self._positions_mask = positions_mask
+        self._verbose = verbose
+        self._preprocessed = False
+
+    # -- device / bookkeeping (phase_base_class.py:46-221) -------------------
+    def set_device(self, device, clear_fft_cache=None):
+        if clear_fft_cache is not None:
+            self._clear_fft_cache = clear_fft_cache
+        if device is None:
+            return self
+        if device not in ("cpu", "gpu"):
+            raise ValueError(f"device must be either 'cpu' or 'gpu', not {device}")
+        self._device = device
+        return self
+
+    def set_storage(self, storage):
+        if storage not in ("cpu", "gpu"):
+            raise ValueError(f"storage must be either 'cpu' or 'gpu', not {storage}")
+        if storage == "gpu" and getattr(self, "_device", "cpu") == "cpu":
+            raise ValueError("storage='gpu' and device='cpu' is not supported")
+        self._storage = storage
+        return self
+
+    def set_save_defaults(self, save_datacube=False, save_exit_waves=False, save_iterations=True,
+                          save_iterations_frequency=1):
+        self._save_datacube = save_datacube
+        self._save_exit_waves = save_exit_waves
+        self._save_iterations = save_iterations
+        self._save_iterations_frequency = save_iterations_frequency
+        return self
+
+    def attach_datacube(self, datacube):
+        self._datacube = datacube
+        return self
+
+    # -- properties (phase_base_class.py:2197-2227; ptychographic_methods.py) --
+    @property
+    def angular_sampling(self):
+        return getattr(self, "_angular_sampling", None)
+
+    @property
+    def sampling(self):
+        if self.angular_sampling is None:
+            return None
+        return real_space_sampling(self._energy, self._angular_sampling, self._amplitudes_shape)
+
+    @property
+    def positions(self):
+        if self.angular_sampling is None:
+            return None
+        pos = np.array(self._positions_px, dtype=np.float32, copy=True)
+        pos[:, 0] *= self.sampling[0]
+        pos[:, 1] *= self.sampling[1]
+        return pos
+
+    @property
+    def probe_centered(self):
+        if getattr(self, "_probe", None) is None:
+            return None
+        return np.fft.fftshift(np.asarray(self._probe, dtype=np.complex64), axes=(-2, -1))
+
+    @property
+    def probe_fourier(self):
+        if getattr(self, "_probe", None) is None:
+            return None
+        return np.fft.fftshift(np.fft.fft2(np.asarray(self._probe)), axes=(-2, -1))
+
+    @property
+    def object_cropped(self):
+        """Object rotated into the scan frame and cropped to the scanned FOV
+        (ptychographic_methods.py:85-124)."""
+        angle = self._rotation_best_rad if self._rotation_best_transpose else -self._rotation_best_rad
+        pos = np.asarray(self._positions_px, dtype=np.float64)
+        c, s = np.cos(angle), np.sin(angle)
+        rot = np.array([[c, -s], [s, c]], dtype=np.float32)
+        origin = pos.mean(0).astype(np.float32)
+        pts = (pos - origin) @ rot + origin
+        min_x, min_y = np.floor(np.amin(pts, axis=0)).astype(int)
+        min_x, min_y = max(min_x, 0), max(min_y, 0)
+        max_x, max_y = np.ceil(np.amax(pts, axis=0)).astype(int)
+        obj = np.asarray(self.object)
+        rotated = scipy.ndimage.rotate(obj, np.rad2deg(-angle), order=1, reshape=False,
+                                       axes=(-2, -1))[..., min_x:max_x, min_y:max_y]
+        if self._rotation_best_transpose:
+            rotated = rotated.swapaxes(-2, -1)
+        return rotated
+
+    # -- preprocess -----------------------------------------------------------
+    def preprocess(self, diffraction_intensities_shape=None, reshaping_method="bilinear",
+                   padded_diffraction_intensities_shape=None, region_of_interest_shape=None,
+                   dp_mask=None, in_place_datacube_modification=False, fit_function="plane",
+                   plot_center_of_mass="default", plot_rotation=True, maximize_divergence=False,
+                   rotation_angles_deg=None, plot_probe_overlaps=True, force_com_rotation=None,
+                   force_com_transpose=None, force_com_shifts=None, force_com_measured=None,
+                   vectorized_com_calculation=True, force_scan_sampling=None,
+                   force_angular_sampling=None, force_reciprocal_sampling=None,
+                   object_fov_mask=None, crop_patterns=False, center_positions_in_fov=True,
+                   store_initial_arrays=True, device=None, clear_fft_cache=None,
+                   max_batch_size=None, **kwargs):
+        """singleslice_ptychography.py:191-604.  Returns self."""
+        self.set_device(device, clear_fft_cache)
+        if self._datacube is None:
+            raise ValueError("The preprocess() method requires a DataCube. "
+                             "Please run ptycho.attach_datacube(DataCube) first.")
+        for opt, val in (("diffraction_intensities_shape", diffraction_intensities_shape),
+                         ("padded_diffraction_intensities_shape", padded_diffraction_intensities_shape),
+                         ("region_of_interest_shape", region_of_interest_shape)):
+            if val is not None:
+                raise NotImplementedError(f"{opt} (resampling/padding) is not supported")
+        if crop_patterns:
+            raise NotImplementedError("crop_patterns is not supported")
+        if self._object_type != "complex":
+            raise NotImplementedError("only object_type='complex' is supported")
+        if fit_function not in (None, "plane"):
+            raise NotImplementedError(f"fit_function={fit_function!r} is not supported (plane only)")
+        if self._vacuum_probe_intensity is not None:
+            raise NotImplementedError("vacuum_probe_intensity is not supported")
+
+        dc = self._datacube
+        Rshape = tuple(dc.data.shape[:2])
+        if force_com_shifts is not None and np.isscalar(force_com_shifts[0]):
+            force_com_shifts = (np.ones(Rshape) * force_com_shifts[0],
+                                np.ones(Rshape) * force_com_shifts[1])
+        if force_com_measured is not None and np.isscalar(force_com_measured[0]):
+            force_com_measured = (np.ones(Rshape) * force_com_measured[0],
+                                  np.ones(Rshape) * force_com_measured[1])
+        if self._positions_mask is not None:
+            self._positions_mask = np.asarray(self._positions_mask, dtype=bool)
+
+        intensities = np.asarray(dc.data, dtype=np.float32)
+        self._grid_scan_shape = intensities.shape[:2]
+        (self._scan_sampling, self._angular_sampling,
+         self._reciprocal_sampling) = extract_calibrations(
+            dc, self._energy, True, force_scan_sampling, force_angular_sampling,
+            force_reciprocal_sampling, self._verbose)
+        if intensities.shape[-1] != intensities.shape[-2]:
+            raise NotImplementedError("only square diffraction patterns are supported")
+        S = intensities.shape[-1]
+        K.check_frame(S)
+        if self._semiangle_cutoff_pixels:
+            self._semiangle_cutoff = self._semiangle_cutoff_pixels * self._angular_sampling[0]
+
+        # centre of mass and origin fit (phase_base_class.py:613-767)
+        if force_com_measured:
+            com_mx = np.asarray(force_com_measured[0], dtype=np.float32)
+            com_my = np.asarray(force_com_measured[1], dtype=np.float32)
+        else:
+            if dp_mask is not None and dp_mask.shape != intensities.shape[-2:]:
+                raise ValueError(f"Mask shape should be (Qx,Qy):{intensities.shape[-2:]}, "
+                                 f"not {dp_mask.shape}")
+            com_mx, com_my = intensities_center_of_mass(intensities, dp_mask)
+        if force_com_shifts is None:
+            if fit_function is not None:
+                finite = np.isfinite(com_mx)
+                com_fx = np.asarray(fit_origin_plane(com_mx, finite), dtype=np.float32)
+                com_fy = np.asarray(fit_origin_plane(com_my, finite), dtype=np.float32)
+            else:
+                com_fx = np.asarray(com_mx, dtype=np.float32)
+                com_fy = np.asarray(com_my, dtype=np.float32)
+        else:
+            com_fx = np.asarray(force_com_shifts[0], dtype=np.float32)
+            com_fy = np.asarray(force_com_shifts[1], dtype=np.float32)
+        self._com_measured_x, self._com_measured_y = com_mx, com_my
+        self._com_fitted_x, self._com_fitted_y = com_fx, com_fy
+        self._com_normalized_x = np.nan_to_num(com_mx - com_fx) * self._reciprocal_sampling[0]
+        self._com_normalized_y = np.nan_to_num(com_my - com_fy) * self._reciprocal_sampling[1]
+
+        (self._rotation_best_rad, self._rotation_best_transpose, self._com_x,
+         self._com_y) = solve_com_rotation(
+            self._com_normalized_x, self._com_normalized_y, rotation_angles_deg,
+            maximize_divergence, force_com_rotation, force_com_transpose, self._verbose)
+
+        # corner-centred amplitudes (phase_base_class.py:1347-1460)
+        self._amplitudes, self._mean_diffraction_intensity = self._normalize_intensities(
+            intensities, com_fx, com_fy)
+        self._crop_mask, self._crop_mask_shape = None, intensities.shape[-2:]
+        del intensities
+        self._num_diffraction_patterns = self._amplitudes.shape[0]
+        self._amplitudes_shape = np.array(self._amplitudes.shape[-2:])
+        self._resample_exit_waves = False
+        self._region_of_interest_shape = np.array(self._amplitudes.shape[-2:])
+
+        # positions, object, probe
+        pos, self._object_padding_px = scan_positions_in_pixels(
+            self._scan_positions, self._grid_scan_shape, self._scan_sampling,
+            self._rotation_best_rad, self._rotation_best_transpose, self.sampling,
+            self._region_of_interest_shape, self._object_padding_px, self._positions_mask,
+            self._positions_offset_ang)
+        if self._object is None:
+            shape = initial_object_shape(pos, self._object_padding_px, self._region_of_interest_shape,
+                                         self._object_fov_ang, self.sampling)
+            self._object = np.ones(shape, dtype=np.complex64)
+        else:
+            self._object = np.array(self._object, dtype=np.complex64, copy=True)
+        if self._object.shape[0] < S or self._object.shape[1] < S:
+            raise ValueError(f"object {self._object.shape} is smaller than the {S}x{S} probe window")
+        if store_initial_arrays:
+            self._object_initial = self._object.copy()
+            self._object_type_initial = self._object_type
+        self._object_shape = self._object.shape
+        self._positions_px = np.asarray(pos, dtype=np.float32)
+        if center_positions_in_fov:
+            center_positions(self._positions_px, self._object_shape)
+        self._positions_px_initial_com = self._positions_px.mean(0)
+        self._positions_px_initial = self._positions_px.copy()
+        self._positions_initial = self._positions_px_initial.copy()
+        self._positions_initial[:, 0] *= self.sampling[0]
+        self._positions_initial[:, 1] *= self.sampling[1]
+
+        self._probe = self._initialize_probe(self._probe)
+        self._known_aberrations_array = ComplexProbe(
+            energy=self._energy, gpts=self._region_of_interest_shape, sampling=self.sampling,
+            parameters=self._polar_parameters)._evaluate_ctf()
+        if store_initial_arrays:
+            self._probe_initial = self._probe.copy()
+            self._probe_initial_aperture = np.abs(np.fft.fft2(self._probe))
+        else:
+            self._probe_initial_aperture = None
+
+        # device-resident data for the loop
+        dev = require_gpu()
+        amps = torch.from_numpy(np.ascontiguousarray(self._amplitudes)).to(dev)
+        self._amplitudes_dev = K.to_kernel_layout(amps)
+        del amps
+        self._positions_dev = torch.from_numpy(self._positions_px.copy()).to(dev)
+
+        # probe-overlap field and FOV mask (singleslice_ptychography.py:504-537)
+        if object_fov_mask is None or plot_probe_overlaps:
+            self._probe_overlap = self._probe_overlap_field(max_batch_size)
+        if object_fov_mask is None:
+            blurred = scipy.ndimage.gaussian_filter(self._probe_overlap, 1.0)
+            self._object_fov_mask = blurred > 0.25 * blurred.max()
+        elif object_fov_mask is True:
+            self._object_fov_mask = np.full(self._object_shape, True)
+        else:
+            self._object_fov_mask = np.asarray(object_fov_mask)
+        self._object_fov_mask_inverse = np.invert(self._object_fov_mask)
+        self._preprocessed = True
+        for attr in ("_object_dev", "_probe_dev"):
+            if hasattr(self, attr):
+                delattr(self, attr)
+        return self
+
+    def _normalize_intensities(self, intensities, com_fx, com_fy, chunk=1024):
+        """Shift each pattern by -CoM (bilinear, periodic), accumulate the
+        float32 mean intensity pattern by pattern, take sqrt(max(I, 0))."""
+        rx, ry, qx, qy = intensities.shape
+        flat = intensities.reshape(-1, qx, qy)
+        fx = (-np.asarray(com_fx, dtype=np.float32)).reshape(-1)
+        fy = (-np.asarray(com_fy, dtype=np.float32)).reshape(-1)
+        keep = (np.ones(rx * ry, dtype=bool) if self._positions_mask is None
+                else self._positions_mask.reshape(-1))
+        out = np.empty((int(keep.sum()), qx, qy), dtype=np.float32)
+        mean_intensity = np.float32(0)
+        k = 0
+        for s in range(0, rx * ry, chunk):
+            sel = np.nonzero(keep[s:s + chunk])[0] + s
+            if sel.size == 0:
+                continue
+            shifted = bilinear_shift_periodic(flat[sel], fx[sel], fy[sel])
+            for v in np.sum(shifted, axis=(-2, -1)):
+                mean_intensity += v
+            out[k:k + sel.size] = np.sqrt(np.maximum(shifted, 0))
+            k += sel.size
+        mean_intensity /= out.shape[0]
+        return out, mean_intensity
+
+    def _initialize_probe(self, initial_probe):
+        """ptychographic_methods.py:1011-1102."""
+        if initial_probe is None:
+            probe = ComplexProbe(energy=self._energy, gpts=self._region_of_interest_shape,
+                                 sampling=self.sampling, semiangle_cutoff=self._semiangle_cutoff,
+                                 rolloff=self._rolloff,
+                                 parameters=self._polar_parameters).build()._array
+            intensity = np.sum(np.abs(np.fft.fft2(probe)) ** 2)
+            probe *= np.sqrt(self._mean_diffraction_intensity / intensity)
+            return probe.astype(np.complex64)
+        if isinstance(initial_probe, ComplexProbe):
+            if tuple(initial_probe._gpts) != tuple(self._region_of_interest_shape):
+                raise ValueError()
+            probe = (initial_probe._array if hasattr(initial_probe, "_array")
+                     else initial_probe.build()._array)
+            intensity = np.sum(np.abs(np.fft.fft2(probe)) ** 2)
+            probe = probe * np.sqrt(self._mean_diffraction_intensity / intensity)
+            return probe.astype(np.complex64)
+        return np.array(initial_probe, dtype=np.complex64, copy=True)
+
+    def _probe_overlap_field(self, max_batch_size=None):
+        """sum over batches of scatter(|fft_shift(P, frac)|^2), float32."""
+        dev = self._amplitudes_dev.device
+        N = self._num_diffraction_patterns
+        B = N if max_batch_size is None else int(max_batch_size)
+        probe = torch.from_numpy(np.ascontiguousarray(self._probe)).to(dev)
+        phat = K.probe_spectrum(probe)
+        ones = torch.ones(self._object_shape, dtype=torch.complex64, device=dev)
+        overlap = torch.zeros(self._object_shape, dtype=torch.float32, device=dev)
+        start = 0
+        for nb in subdivide_into_batches(N, B):
+            order = torch.arange(start, start + nb, dtype=torch.int32, device=dev)
+            ps, chi, _ = K.forward_adjoint(phat, ones, self._positions_dev, self._amplitudes_dev, order)
+            _, spsum, _ = K.scatter(ps, chi, self._positions_dev, order, self._object_shape)
+            overlap += spsum
+            start += nb
+            del ps, chi
+        return overlap.cpu().numpy()
+
+    # -- reconstruct ------------------------------------------------------------
+    def _reset_reconstruction(self, store_iterations, reset):
+        """ptychographic_methods.py:252-294."""
+        if store_iterations and (not hasattr(self, "object_iterations") or reset):
+            self.object_iterations = []
+            self.probe_iterations = []
+        if reset is True:
+            self.error_iterations = []
+            self._object = self._object_initial.copy()
+            self._probe = self._probe_initial.copy()
+            self._positions_px = self._positions_px_initial.copy()
+            self._object_type = self._object_type_initial
+            self._exit_waves = None
+            for attr in ("_object_dev", "_probe_dev"):
+                if hasattr(self, attr):
+                    delattr(self, attr)
+        elif reset is None:
+            if hasattr(self, "error"):
+                warnings.warn("Continuing reconstruction from previous result. "
+                              "Use reset=True for a fresh start.", UserWarning)
+            else:
+                self.error_iterations = []
+                self._exit_waves = None
+        else:
+            if not hasattr(self, "error"):
+                self.error_iterations = []
+                self._exit_waves = None
+
+    def reconstruct(self, num_iter=8, reconstruction_method="gradient-descent",
+                    reconstruction_parameter=1.0, reconstruction_parameter_a=None,
+                    reconstruction_parameter_b=None, reconstruction_parameter_c=None,
+                    max_batch_size=None, seed_random=None, step_size=0.5, normalization_min=1,
+                    positions_step_size=0.5, pure_phase_object=False, fix_probe_com=True,
+                    fix_probe=False, fix_probe_aperture=False, constrain_probe_amplitude=False,
+                    constrain_probe_amplitude_relative_radius=0.5,
+                    constrain_probe_amplitude_relative_width=0.05,
+                    constrain_probe_fourier_amplitude=False,
+                    constrain_probe_fourier_amplitude_max_width_pixels=3.0,
+                    constrain_probe_fourier_amplitude_constant_intensity=False, fix_positions=True,
+                    fix_positions_com=True, max_position_update_distance=None,
+                    max_position_total_distance=None, global_affine_transformation=False,
+                    gaussian_filter_sigma=None, gaussian_filter=True, fit_probe_aberrations=False,
+                    fit_probe_aberrations_max_angular_order=4,
+                    fit_probe_aberrations_max_radial_order=4,
+                    fit_probe_aberrations_remove_initial=False,
+                    fit_probe_aberrations_using_scikit_image=True, butterworth_filter=True,
+                    q_lowpass=None, q_highpass=None, butterworth_order=2, tv_denoise=True,
+                    tv_denoise_weight=None, tv_denoise_inner_iter=40, object_positivity=True,
+                    shrinkage_rad=0.0, fix_potential_baseline=True, detector_fourier_mask=None,
+                    store_iterations=False, progress_bar=True, reset=None, device=None,
+                    clear_fft_cache=None, object_type=None):
+        """singleslice_ptychography.py:606-996 (GD).  Returns self."""
+        self.set_device(device, clear_fft_cache)
+        if not getattr(self, "_preprocessed", False):
+            raise ValueError("preprocess() must run before reconstruct()")
+        self._reset_reconstruction(store_iterations, reset)
+        if object_type is not None and object_type != self._object_type:
+            raise NotImplementedError("switching object_type is not supported")
+
+        # method parameters (phase_base_class.py:2005-2108)
+        if reconstruction_method in _PROJECTION_NAMES:
+            if reconstruction_method == "generalized-projections" and (
+                    reconstruction_parameter_a is None or reconstruction_parameter_b is None
+                    or reconstruction_parameter_c is None):
+                raise ValueError("reconstruction_parameter_a/b/c must all be specified "
+                                 "when using reconstruction_method='generalized-projections'.")
+            if max_batch_size is not None:
+                raise ValueError(
+                    "Stochastic object/probe updating is inconsistent with 'DM_AP', 'RAAR', "
+                    "'RRR', and 'SUPERFLIP'. Use reconstruction_method='GD' or set "
+                    "max_batch_size=None.")
+            raise NotImplementedError(f"reconstruction_method={reconstruction_method!r}: only "
+                                      "gradient descent is implemented on MI355X")
+        if reconstruction_method not in _GD_NAMES:
+            raise ValueError(
+                "reconstruction_method must be one of 'generalized-projections', "
+                "'DM_AP' (or 'difference-map_alternating-projections'), "
+                "'RAAR' (or 'relaxed-averaged-alternating-reflections'), "
+                "'RRR' (or 'relax-reflect-reflect'), "
+                "'SUPERFLIP' (or 'charge-flipping'), "
+                f"or 'GD' (or 'gradient-descent'), not  {reconstruction_method}.")
+        unsupported = {
+            "fix_positions=False": not fix_positions,
+            "detector_fourier_mask": detector_fourier_mask is not None,
+            "gaussian_filter_sigma": gaussian_filter and gaussian_filter_sigma is not None,
+            "q_lowpass/q_highpass": butterworth_filter and (q_lowpass is not None
+                                                            or q_highpass is not None),
+            "tv_denoise_weight": tv_denoise and tv_denoise_weight is not None,
+            "fit_probe_aberrations": fit_probe_aberrations and not fix_probe,
+            "fix_probe_aperture": fix_probe_aperture and not fix_probe,
+            "constrain_probe_amplitude": constrain_probe_amplitude and not fix_probe,
+            "constrain_probe_fourier_amplitude": constrain_probe_fourier_amplitude and not fix_probe,
+        }
+        bad = [k for k, v in unsupported.items() if v]
+        if bad:
+            raise NotImplementedError("not supported on this path: " + ", ".join(bad))
+
+        N = self._num_diffraction_patterns
+        if self._verbose:
+            first = (f"Performing {num_iter} iterations using a {self._object_type} object type, "
+                     f"with the {reconstruction_method} algorithm, with normalization_min: "
+                     f"{normalization_min} and step _size: {step_size}")
+            warnings.warn(first + (f", in batches of max {max_batch_size} measurements."
+                                   if max_batch_size is not None else "."), UserWarning)
+
+        # batch order: identical RNG use to the reference loop (:835-857)
+        shuffled = np.arange(N)
+        if max_batch_size is not None:
+            np.random.seed(seed_random)
+            B = int(max_batch_size)
+        else:
+            B = N
+        B = max(1, min(B, N))
+        orders = np.empty((max(num_iter, 1), N), dtype=np.int32)
+        for it in range(num_iter):
+            np.random.shuffle(shuffled)
+            orders[it] = shuffled
+
+        dev = require_gpu()
+        if not hasattr(self, "_object_dev"):
+            self._object_dev = torch.from_numpy(np.ascontiguousarray(self._object,
+                                                                     dtype=np.complex64)).to(dev)
+            self._probe_dev = torch.from_numpy(np.ascontiguousarray(self._probe,
+                                                                    dtype=np.complex64)).to(dev)
+        use_mask = fix_potential_baseline and self._object_fov_mask_inverse.sum() > 0
+        mask_dev = (torch.from_numpy(self._object_fov_mask_inverse.astype(np.uint8)).to(dev)
+                    if use_mask else None)
+        problem = K.make_problem(self._object_dev, self._probe_dev, self._positions_dev,
+                                 self._amplitudes_dev, mask_dev, self._mean_diffraction_intensity)
+        per_call = 1 if (store_iterations or progress_bar) else max(num_iter, 1)
+        params = PtyGdParams(per_call, B, int(bool(fix_probe)), int(bool(fix_probe_com)),
+                             int(bool(pure_phase_object)), float(step_size),
+                             float(normalization_min), float(shrinkage_rad))
+        ws = torch.empty(K.gd_workspace_bytes(problem, params), dtype=torch.uint8, device=dev)
+        orders_dev = torch.from_numpy(orders).to(dev)
+        errors_dev = torch.zeros(max(num_iter, 1), dtype=torch.float64, device=dev)
+        self._last_run = dict(problem=problem, params=params, workspace=ws, orders=orders_dev,
+                              errors=errors_dev, mask=mask_dev)
+
+        it = 0
+        bar = None
+        if progress_bar and num_iter > 0:
+            try:
+                from tqdm import tqdm
+                bar = tqdm(total=num_iter, desc="Reconstructing object and probe", unit=" iter")
+            except ImportError:
+                bar = None
+        while it < num_iter:
+            n = min(per_call, num_iter - it)
+            params.num_iter = n
+            K.gd_run(problem, params, orders_dev[it:], errors_dev[it:], ws)
+            it += n
+            if store_iterations:
+                self.object_iterations.append(self._object_dev.cpu().numpy().copy())
+                self.probe_iterations.append(
+                    np.fft.fftshift(self._probe_dev.cpu().numpy(), axes=(-2, -1)))
+            if bar is not None:
+                bar.update(n)
+        if bar is not None:
+            bar.close()
+
+        errors = errors_dev[:num_iter].cpu().numpy()
+        self.error_iterations.extend(float(e) for e in errors)
+        self._object = self._object_dev.cpu().numpy()
+        self._probe = self._probe_dev.cpu().numpy()
+        self.object = self._object.copy()
+        self.probe = self.probe_centered
+        if num_iter > 0:
+            self.error = float(errors[-1])
+        elif not hasattr(self, "error"):
+            self.error = 0.0
+        self._exit_waves = None
+        return self

@@ -1,0 +1,203 @@
+// CPU emulator of the HIP kernels' phase sequences (test infrastructure).
+//
+// Runs the exact phase functions of py4dstem_amd/csrc/pty_core.h for every
+// thread id in turn.  Between barriers it records, per LDS word, which thread
+// wrote and which read it, and counts a violation for any cross-thread
+// read-after-write, write-after-read or write-after-write inside one phase,
+// and for any out-of-range LDS or global index it can see.  Built with g++
+// by tests/conftest.py; the product library never links it.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+static long g_violations = 0;
+static char g_first[256] = {0};
+static void note_violation(const char* what) {
+  if (g_violations++ == 0) std::snprintf(g_first, sizeof g_first, "%s", what);
+}
+#define PTY_CHECK(cond, what) ((cond) ? (void)0 : note_violation(what))
+
+#include "../../py4dstem_amd/csrc/pty_core.h"
+
+using namespace pty;
+
+struct LdsCheck {
+  cf* mem;
+  int n;
+  int* wt;
+  int* rd;
+  const int* cur;
+  cf ld(int i) const {
+    if (i < 0 || i >= n) { note_violation("lds load out of range"); return mkcf(0, 0); }
+    if (wt[i] >= 0 && wt[i] != *cur) note_violation("lds read of a word another thread wrote");
+    if (rd[i] == -1) rd[i] = *cur; else if (rd[i] != *cur) rd[i] = -2;
+    return mem[i];
+  }
+  void st(int i, cf v) const {
+    if (i < 0 || i >= n) { note_violation("lds store out of range"); return; }
+    if (wt[i] >= 0 && wt[i] != *cur) note_violation("lds write-write race");
+    if (rd[i] == -2 || (rd[i] >= 0 && rd[i] != *cur)) note_violation("lds write of a word another thread read");
+    wt[i] = *cur;
+    mem[i] = v;
+  }
+};
+
+template <int S> struct EmuRunner {
+  using K = Fused<S>;
+  static constexpr int T = Frame<S>::T;
+  std::vector<cf> mem;
+  std::vector<int> wt, rd;
+  std::vector<typename K::Regs> regs;
+  std::vector<cf> tw;
+  int cur = -1;
+  const FusedArgs* a;
+  EmuRunner(const FusedArgs* args)
+      : mem(K::LDS_TOTAL, mkcf(0, 0)), wt(K::LDS_TOTAL, -1), rd(K::LDS_TOTAL, -1),
+        regs(T), tw(S), a(args) {
+    for (int t = 0; t < S; ++t) {
+      struct TwSt { cf* p; void st(int i, cf v) const { p[i] = v; } } view{tw.data()};
+      K::twiddles(t, view);
+    }
+  }
+  template <class Fn> void phase(Fn fn) {
+    LdsCheck view{mem.data(), (int)mem.size(), wt.data(), rd.data(), &cur};
+    for (int t = 0; t < T; ++t) {
+      cur = t;
+      fn(t, regs[t], view, tw.data());
+    }
+    cur = -1;
+  }
+  void barrier() {
+    std::fill(wt.begin(), wt.end(), -1);
+    std::fill(rd.begin(), rd.end(), -1);
+  }
+  void reduce_err(int b) {
+    double s = 0.0;
+    for (int t = 0; t < T; ++t) s += regs[t].err;
+    a->err[b] = s;
+    barrier();
+  }
+};
+
+template <int S> static void run_fused(const FusedArgs& a) {
+  EmuRunner<S> run(&a);
+  for (int b = a.b0; b < a.B; ++b) fused_position<S>(run, a, b);
+}
+
+extern "C" {
+
+long emu_violations(void) { return g_violations; }
+const char* emu_first_violation(void) { return g_first; }
+void emu_reset(void) { g_violations = 0; g_first[0] = 0; }
+
+// Same argument meaning as pty_forward_adjoint_batch (complex arrays are
+// interleaved float32 pairs).
+int emu_fused(int S, const float* probe_hat, const float* object, int Px, int Py,
+              const float* positions, int N, const float* amps, const int* order, int B,
+              float* ps, float* chi, double* err, float* fwd_out) {
+  FusedArgs a;
+  a.probe_hat = (const cf*)probe_hat;
+  a.object = (const cf*)object;
+  a.positions = positions;
+  a.amps = amps;
+  a.order = order;
+  a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0;
+  a.ps_out = (cf*)ps; a.chi_out = (cf*)chi; a.err = err; a.fwd_out = (cf*)fwd_out;
+  if (S == 64) run_fused<64>(a);
+  else if (S == 128) run_fused<128>(a);
+  else return -1;
+  return 0;
+}
+
+}  // extern "C"
+
+// forward 2-D FFT natural -> permuted and inverse permuted -> natural, with
+// the same passes (and phase/barrier structure) the kernels use
+template <int S> static void fft_fwd(const cf* in, cf* out) {
+  using K = Fused<S>;
+  FusedArgs dummy{};
+  EmuRunner<S> run(&dummy);
+  using Regs = typename K::Regs;
+  using F = Frame<S>;
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+    for (int m = 0; m < F::N1; ++m) r.v[m] = in[(n2 + F::N2 * m) * S + cc];
+    Pass<S>::template A<false>(n2, r.v, tw);
+    for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
+  });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
+    Pass<S>::template B<false>(t, r.v, tw);
+    for (int j = 0; j < F::N1; ++j) out[rr * S + F::N1 * t + j] = r.v[j];
+  });
+}
+
+template <int S> static void fft_inv(const cf* in, cf* out) {
+  using K = Fused<S>;
+  FusedArgs dummy{};
+  EmuRunner<S> run(&dummy);
+  using Regs = typename K::Regs;
+  using F = Frame<S>;
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+    for (int j = 0; j < F::N1; ++j) r.v[j] = in[rr * S + F::N1 * t + j];
+    Pass<S>::template B<true>(t, r.v, tw);
+    for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
+  });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::p_colAinv_to_regs(tid, r, lds, tw);
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+    for (int m = 0; m < F::N1; ++m) out[(n2 + F::N2 * m) * S + cc] = r.v[m];
+  });
+}
+
+extern "C" {
+
+int emu_fft2_perm(int S, int inverse, const float* in, float* out) {
+  const cf* i = (const cf*)in;
+  cf* o = (cf*)out;
+  if (S == 64) { if (inverse) fft_inv<64>(i, o); else fft_fwd<64>(i, o); }
+  else if (S == 128) { if (inverse) fft_inv<128>(i, o); else fft_fwd<128>(i, o); }
+  else return -1;
+  return 0;
+}
+
+int emu_kidx(int S, int p) {
+  if (S == 64) return Frame<64>::kidx(p);
+  if (S == 128) return Frame<128>::kidx(p);
+  return -1;
+}
+
+int emu_gather(int S, const float* ps, const float* chi, const float* positions, int N,
+               const int* order, int B, int Px, int Py, float* grad, float* spsum, float* tilemax) {
+  GatherArgs a;
+  a.ps = (const cf*)ps; a.chi = (const cf*)chi; a.positions = positions; a.order = order;
+  a.N = N; a.B = B; a.S = S; a.Px = Px; a.Py = Py;
+  a.tiles_y = (Py + TILE - 1) / TILE;
+  const int tiles_x = (Px + TILE - 1) / TILE;
+  a.grad = (cf*)grad; a.spsum = spsum; a.tilemax = tilemax;
+  for (int tile = 0; tile < tiles_x * a.tiles_y; ++tile) {
+    float m = 0.f;
+    for (int tid = 0; tid < GATHER_THREADS; ++tid) {
+      GatherAcc acc;
+      gather_accumulate(a, tile, tid, acc);
+      float t = gather_store(a, tile, tid, acc);
+      m = t > m ? t : m;
+    }
+    tilemax[tile] = m;
+  }
+  return 0;
+}
+
+}  // extern "C"

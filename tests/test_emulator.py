@@ -1,0 +1,113 @@
+"""Kernel phase logic on the CPU: the exact phase functions the HIP kernels
+run, executed by the emulator (tests/emu/pty_emu.cpp) with LDS race and
+bounds checking, compared with the oracle."""
+
+import numpy as np
+import pytest
+
+import oracle.ptycho_oracle as po
+from tests.conftest import cptr, rel
+
+
+def kidx(emu, S):
+    return np.array([emu.emu_kidx(S, p) for p in range(S)])
+
+
+@pytest.mark.parametrize("S", [64, 128])
+def test_fft_passes_match_numpy_and_are_race_free(emu, S):
+    emu.emu_reset()
+    rng = np.random.default_rng(S)
+    x = (rng.standard_normal((S, S)) + 1j * rng.standard_normal((S, S))).astype(np.complex64)
+    k = kidx(emu, S)
+    assert sorted(k) == list(range(S))
+    out = np.zeros_like(x)
+    emu.emu_fft2_perm(S, 0, cptr(x), cptr(out))
+    nat = np.empty_like(out)
+    nat[np.ix_(k, k)] = out
+    assert rel(nat, np.fft.fft2(x.astype(np.complex128))) < 5e-7
+    back = np.zeros_like(x)
+    emu.emu_fft2_perm(S, 1, cptr(out.copy()), cptr(back))
+    assert rel(back, x) < 5e-7
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+
+
+def test_fused_position_matches_oracle_forward(emu, small_case):
+    w, o = small_case
+    S = 64
+    rng = np.random.default_rng(4)
+    obj = (o._object * np.exp(1j * 0.1 * rng.standard_normal(o._object.shape))).astype(np.complex64)
+    probe = o._probe.astype(np.complex64)
+    order = rng.permutation(o._num_diffraction_patterns)[:12].astype(np.int32)
+    pos = o._positions_px
+    k = kidx(emu, S)
+    phat = np.ascontiguousarray(np.fft.fft2(probe)[np.ix_(k, k)].astype(np.complex64))
+    amps = np.ascontiguousarray(o._amplitudes[:, k][:, :, k])
+    B = len(order)
+    ps = np.zeros((B, S, S), np.complex64)
+    chi = np.zeros_like(ps)
+    far = np.zeros_like(ps)
+    err = np.zeros(B)
+    emu.emu_reset()
+    emu.emu_fused(S, cptr(phat), cptr(obj), obj.shape[0], obj.shape[1], cptr(pos), len(pos),
+                  cptr(amps), cptr(order), B, cptr(ps), cptr(chi), cptr(err), cptr(far))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    sp, op, ew, e = o.forward(obj, probe, pos[order], o._amplitudes[order])
+    assert rel(ps, sp) < 1e-6
+    assert rel(chi, ew) < 2e-5
+    assert abs(err.sum() / e - 1) < 1e-6
+    F = np.fft.fft2(sp * op)
+    farn = np.empty_like(far)
+    farn[:, k[:, None], k[None, :]] = far
+    assert rel(farn, F) < 1e-6
+
+
+def test_fused_position_edge_wrap_and_half_pixels(emu):
+    """Positions on the object edges (periodic wrap) and exact .5 fractions."""
+    S = 64
+    rng = np.random.default_rng(8)
+    Px, Py = 64, 70  # object as small as the frame
+    obj = np.exp(1j * rng.uniform(0, 0.5, (Px, Py))).astype(np.complex64)
+    probe = po.ComplexProbeOracle(300e3, (S, S), (0.2, 0.2), 20.0, 2.0).build()._array.astype(
+        np.complex64) * 20
+    pos = np.array([[0.5, 69.5], [63.49, -0.5], [31.5, 35.5], [-7.2, 80.1]], np.float32)
+    order = np.arange(4, dtype=np.int32)
+    amps_nat = rng.uniform(0, 3, (4, S, S)).astype(np.float32)
+    k = kidx(emu, S)
+    amps = np.ascontiguousarray(amps_nat[:, k][:, :, k])
+    phat = np.ascontiguousarray(np.fft.fft2(probe)[np.ix_(k, k)].astype(np.complex64))
+    ps = np.zeros((4, S, S), np.complex64)
+    chi = np.zeros_like(ps)
+    err = np.zeros(4)
+    emu.emu_reset()
+    emu.emu_fused(S, cptr(phat), cptr(obj), Px, Py, cptr(pos), 4, cptr(amps), cptr(order), 4,
+                  cptr(ps), cptr(chi), cptr(err), None)
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    o = po.OracleSingleslicePtychography(energy=300e3)
+    o._region_of_interest_shape = np.array([S, S])
+    sp, _, ew, e = o.forward(obj, probe, pos, amps_nat)
+    assert rel(ps, sp) < 1e-6 and rel(chi, ew) < 2e-5
+
+
+def test_scatter_is_bincount_in_batch_order(emu, small_case):
+    _, o = small_case
+    S = 64
+    rng = np.random.default_rng(6)
+    B = 40
+    order = rng.permutation(o._num_diffraction_patterns)[:B].astype(np.int32)
+    pos = o._positions_px
+    ps = (rng.standard_normal((B, S, S)) + 1j * rng.standard_normal((B, S, S))).astype(np.complex64)
+    ones = np.ones_like(ps)
+    shape = o._object.shape
+    ntiles = ((shape[0] + 31) // 32) * ((shape[1] + 31) // 32)
+    grad = np.zeros(shape, np.complex64)
+    spsum = np.zeros(shape, np.float32)
+    tmax = np.zeros(ntiles, np.float32)
+    emu.emu_gather(S, cptr(ps), cptr(ones), cptr(pos), len(pos), cptr(order), B, shape[0],
+                   shape[1], cptr(grad), cptr(spsum), cptr(tmax))
+    # chi = 1 makes the float32 product exact: the float64 sums must then be
+    # bit-identical to np.bincount in batch order
+    ref = po.scatter_bincount(np.conj(ps), pos[order], (S, S), shape)
+    np.testing.assert_array_equal(grad, ref)
+    sq = (ps.real * ps.real + ps.imag * ps.imag).astype(np.float32)
+    np.testing.assert_array_equal(spsum, po.scatter_bincount(sq, pos[order], (S, S), shape))
+    assert tmax.max() == spsum.max()

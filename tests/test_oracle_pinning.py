@@ -1,0 +1,283 @@
+"""Pins the CPU oracle to what mathematics and hand-derived values fix.
+
+The reference ships no tests, fixtures or golden vectors for this path, and
+it cannot be imported here (emdfile, pylops and others are absent), so each
+oracle function is checked against closed forms, invariants, hand-derived
+vectors or brute-force loops, chosen so that a dropped term, a wrong sign or
+index, or a transposed operand fails.
+"""
+
+import numpy as np
+import pytest
+import scipy.ndimage
+
+import oracle.ptycho_oracle as po
+from tests.conftest import rel
+
+
+def test_wavelength_textbook_values():
+    # relativistic electron wavelengths (textbook values, 4-5 significant digits)
+    assert po.electron_wavelength_angstrom(300e3) == pytest.approx(0.019687, rel=1e-4)
+    assert po.electron_wavelength_angstrom(200e3) == pytest.approx(0.025079, rel=1e-4)
+    assert po.electron_wavelength_angstrom(80e3) == pytest.approx(0.041757, rel=1e-4)
+    # the non-relativistic value at 300 kV would be 0.02239 A: must differ
+    assert abs(po.electron_wavelength_angstrom(300e3) - 0.02239) > 1e-3
+
+
+def _probe(**kw):
+    args = dict(energy=300e3, gpts=(64, 64), sampling=(0.2, 0.2), semiangle_cutoff=20.0,
+                rolloff=2.0)
+    args.update(kw)
+    return po.ComplexProbeOracle(**args)
+
+
+def test_aperture_closed_form():
+    p = _probe()
+    alpha = np.array([0.0, 0.010, 0.0179, 0.019, 0.0201, 0.030], dtype=np.float32)
+    a = p.evaluate_aperture(alpha.copy())
+    assert a[0] == 1 and a[1] == 1 and a[2] == 1      # below cutoff - rolloff
+    assert a[4] == 0 and a[5] == 0                    # above cutoff
+    assert a[3] == pytest.approx(0.5, abs=1e-5)       # middle of the cosine edge
+    hard = _probe(rolloff=0.0).evaluate_aperture(alpha.copy())
+    np.testing.assert_array_equal(hard, [1, 1, 1, 1, 0, 0])
+
+
+def test_chi_defocus_astigmatism_cs_closed_forms():
+    lam = po.electron_wavelength_angstrom(300e3)
+    alpha = np.linspace(0, 0.025, 7).astype(np.float32)[:, None] * np.ones((1, 5), np.float32)
+    phi = np.linspace(-3, 3, 5).astype(np.float32)[None, :] * np.ones((7, 1), np.float32)
+    chi = _probe(parameters={"defocus": 200.0}).evaluate_chi(alpha, phi)
+    # defocus alias flips sign: C10 = -200
+    np.testing.assert_allclose(chi, np.pi * (-200.0) * alpha.astype(np.float64) ** 2 / lam,
+                               rtol=1e-5, atol=1e-5)
+    chi = _probe(parameters={"C12": 50.0, "phi12": 0.3}).evaluate_chi(alpha, phi)
+    np.testing.assert_allclose(chi, np.pi / lam * alpha.astype(np.float64) ** 2 * 50.0
+                               * np.cos(2 * (phi - 0.3)), rtol=1e-5, atol=1e-5)
+    chi = _probe(parameters={"Cs": 1e4}).evaluate_chi(alpha, phi)
+    np.testing.assert_allclose(chi, 2 * np.pi / lam * 0.25 * alpha.astype(np.float64) ** 4 * 1e4,
+                               rtol=1e-5, atol=1e-5)
+
+
+def test_probe_build_unit_norm_aperture_and_centre():
+    p = _probe().build()._array
+    assert np.sum(np.abs(p) ** 2) == pytest.approx(1.0, rel=1e-5)
+    # no aberrations: the probe peaks at the corner pixel (corner-centred)
+    assert np.unravel_index(np.argmax(np.abs(p)), p.shape) == (0, 0)
+    alpha, _ = _probe().get_scattering_angles()
+    ap = _probe().evaluate_aperture(alpha.copy())
+    spectrum = np.abs(np.fft.fft2(p))
+    np.testing.assert_allclose(spectrum / spectrum.max(), ap / ap.max(), atol=1e-5)
+
+
+def test_fft_shift_integer_is_roll_and_composes():
+    rng = np.random.default_rng(0)
+    a = (rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))).astype(np.complex64)
+    np.testing.assert_allclose(po.fft_shift(a, np.array([3.0, -5.0])),
+                               np.roll(a, (3, -5), axis=(0, 1)), atol=2e-5)
+    s1 = po.fft_shift(po.fft_shift(a, np.array([0.3, 0.7])), np.array([0.4, -0.2]))
+    s2 = po.fft_shift(a, np.array([0.7, 0.5]))
+    np.testing.assert_allclose(s1, s2, atol=5e-5)
+    # +1 moves a delta from index 0 to index 1 (np.roll direction)
+    d = np.zeros((8, 8), np.complex64)
+    d[0, 0] = 1
+    out = po.fft_shift(d, np.array([1.0, 0.0]))
+    assert np.argmax(np.abs(out)) == 8
+    # batched operator: one frame per position
+    op = po.fourier_translation_operator(np.array([[1.0, 0.0], [0.0, 2.0]]), (8, 8))
+    assert op.shape == (2, 8, 8)
+
+
+def test_subdivide_into_batches_hand_derived():
+    assert po.subdivide_into_batches(10, max_batch=3) == [3, 3, 2, 2]
+    assert po.subdivide_into_batches(16384, max_batch=1024) == [1024] * 16
+    assert po.subdivide_into_batches(1000, max_batch=1024) == [1000]
+    assert po.subdivide_into_batches(1025, max_batch=1024) == [513, 512]
+    assert list(po.generate_batches(7, max_batch=3)) == [(0, 3), (3, 5), (5, 7)]
+    with pytest.raises(RuntimeError):
+        po.subdivide_into_batches(5, num_batches=2, max_batch=2)
+
+
+def _brute_indices(positions, S, shape):
+    B = len(positions)
+    rows = np.zeros((B, S), int)
+    cols = np.zeros((B, S), int)
+    for b, (x, y) in enumerate(positions):
+        x0, y0 = int(np.round(x)), int(np.round(y))  # numpy rounds half to even
+        for i in range(S):
+            f = i if i < S // 2 else i - S
+            rows[b, i] = (x0 + f) % shape[0]
+            cols[b, i] = (y0 + f) % shape[1]
+    return rows, cols
+
+
+def test_patch_indices_brute_force_round_half_even():
+    pos = np.array([[2.5, 3.5], [0.2, -1.6], [9.5, 10.49], [-3.5, 12.5]], dtype=np.float32)
+    S, shape = 8, (11, 13)
+    r, c = po.patch_indices(pos, (S, S), shape)
+    br, bc = _brute_indices(pos, S, shape)
+    np.testing.assert_array_equal(r[:, :, 0], br)
+    np.testing.assert_array_equal(c[:, 0, :], bc)
+    assert r[0, 0, 0] == 2 and c[0, 0, 0] == 4  # 2.5 -> 2, 3.5 -> 4
+
+
+def test_scatter_bincount_brute_force_float64_order():
+    rng = np.random.default_rng(1)
+    S, shape = 8, (12, 10)
+    pos = rng.uniform(-2, 14, (6, 2)).astype(np.float32)
+    patches = (rng.standard_normal((6, S, S)) + 1j * rng.standard_normal((6, S, S))).astype(
+        np.complex64)
+    out = po.scatter_bincount(patches, pos, (S, S), shape)
+    br, bc = _brute_indices(pos, S, shape)
+    re = np.zeros(shape)
+    im = np.zeros(shape)
+    for b in range(6):
+        for i in range(S):
+            for j in range(S):
+                re[br[b, i], bc[b, j]] += float(patches[b, i, j].real)
+                im[br[b, i], bc[b, j]] += float(patches[b, i, j].imag)
+    np.testing.assert_array_equal(out.real, re.astype(np.float32))
+    np.testing.assert_array_equal(out.imag, im.astype(np.float32))
+
+
+def test_get_shifted_ar_integer_and_linear():
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((16, 16)).astype(np.float32)
+    np.testing.assert_array_equal(po.get_shifted_ar(a, 3.0, -2.0), np.roll(a, (3, -2), (0, 1)))
+    ramp = np.tile(np.arange(16, dtype=np.float32)[:, None], (1, 16))
+    out = po.get_shifted_ar(ramp, 0.5, 0.0)
+    # interior rows: (1-w) f(i) + w f(i-1) = i - 0.5
+    np.testing.assert_allclose(out[1:, 0], np.arange(1, 16) - 0.5, atol=1e-6)
+
+
+def test_get_com_closed_form():
+    a = np.zeros((9, 7))
+    a[3, 5] = 2.0
+    assert po.get_CoM(a) == (3.0, 5.0)
+    b = np.zeros((8, 8))
+    b[7, 1] = 1.0  # corner-centred coordinate of index 7 is -1
+    assert po.get_CoM(b, corner_centered=True) == (-1.0, 1.0)
+
+
+def test_fit_plane_recovers_noiseless_plane():
+    x, y = np.linspace(0, 1, 12), np.linspace(0, 1, 9)
+    ry, rx = np.meshgrid(y, x)
+    plane = 3.0 * rx - 2.0 * ry + 1.5
+    np.testing.assert_allclose(po.fit_plane(plane), plane, atol=1e-10)
+    mask = np.ones_like(plane, bool)
+    mask[::3] = False
+    corrupted = plane.copy()
+    corrupted[~mask] = 99.0
+    np.testing.assert_allclose(po.fit_plane(corrupted, mask), plane, atol=1e-10)
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    """A small self-consistent problem: 16 px frames on a 40^2 object."""
+    rng = np.random.default_rng(5)
+    o = po.OracleSingleslicePtychography(energy=300e3)
+    S = 16
+    o._region_of_interest_shape = np.array([S, S])
+    probe = po.ComplexProbeOracle(300e3, (S, S), (0.4, 0.4), 20.0, 2.0,
+                                  parameters={"defocus": 50.0}).build()._array * 30
+    obj = np.exp(1j * 0.3 * scipy.ndimage.gaussian_filter(rng.standard_normal((40, 40)), 2)
+                 / 0.1).astype(np.complex64)
+    pos = rng.uniform(8, 32, (12, 2)).astype(np.float32)
+    return o, probe.astype(np.complex64), obj, pos
+
+
+def test_forward_parseval_and_fixed_point(tiny):
+    o, probe, obj, pos = tiny
+    frac = pos - np.round(pos)
+    rows, cols = po.patch_indices(pos, (16, 16), obj.shape)
+    psi = po.fft_shift(probe, frac) * obj[rows, cols]
+    F = np.fft.fft2(psi.astype(np.complex128))
+    np.testing.assert_allclose(np.sum(np.abs(F) ** 2), 256 * np.sum(np.abs(psi) ** 2), rtol=1e-5)
+    # measured amplitudes equal to the model: chi = 0 and err = 0
+    amps = np.abs(F).astype(np.float32)
+    _, _, chi, err = o.forward(obj, probe, pos, amps)
+    assert np.max(np.abs(chi)) < 1e-4 * np.max(np.abs(psi))
+    assert err < 1e-6 * np.sum(amps ** 2)
+    # zero amplitudes: chi = ifft2(-F) = -psi
+    _, _, chi0, err0 = o.forward(obj, probe, pos, np.zeros_like(amps))
+    np.testing.assert_allclose(chi0, -psi, atol=1e-5)
+    assert err0 == pytest.approx(np.sum(np.abs(F) ** 2), rel=1e-4)
+    # zero far field: angle(0) = 0 so chi = ifft2(A)
+    _, _, chiz, _ = o.forward(np.zeros_like(obj), probe, pos, amps)
+    np.testing.assert_allclose(chiz, np.fft.ifft2(amps), atol=1e-5)
+
+
+def test_gradient_is_scaled_adjoint_term_fp64_finite_differences(tiny):
+    """dL/dRe O + i dL/dIm O = -2 S^2 scatter(conj(P_shift) chi) for
+    L = sum (|F| - A)^2, checked by float64 central differences."""
+    o, probe, obj, pos = tiny
+    o = po.OracleSingleslicePtychography(energy=300e3, mode="c128")
+    o._region_of_interest_shape = np.array([16, 16])
+    rng = np.random.default_rng(9)
+    obj64 = obj.astype(np.complex128)
+    probe64 = probe.astype(np.complex128)
+    amps = np.abs(np.fft.fft2(po.fft_shift(probe64, pos - np.round(pos), "c128")
+                              * 1.05)).astype(np.float64) * rng.uniform(0.8, 1.2, (12, 16, 16))
+
+    def loss(objv):
+        frac = pos - np.round(pos)
+        rows, cols = po.patch_indices(pos, (16, 16), objv.shape)
+        F = np.fft.fft2(po.fft_shift(probe64, frac, "c128") * objv[rows, cols])
+        return np.sum((np.abs(F) - amps) ** 2)
+
+    sp, _, chi, _ = o.forward(obj64, probe64, pos, amps)
+    x0 = np.round(pos[:, 0]).astype(int)
+    y0 = np.round(pos[:, 1]).astype(int)
+    idx = np.fft.fftfreq(16, 1 / 16).astype(int)
+    grad = np.zeros(obj.shape, np.complex128)
+    for b in range(len(pos)):
+        grad[np.ix_((x0[b] + idx) % 40, (y0[b] + idx) % 40)] += np.conj(sp[b]) * chi[b]
+    grad *= -2 * 256
+    h = 1e-6
+    for (i, j) in [(20, 20), (15, 27), (25, 12)]:
+        for unit, comp in ((1.0, grad[i, j].real), (1j, grad[i, j].imag)):
+            e = np.zeros_like(obj64)
+            e[i, j] = unit * h
+            fd = (loss(obj64 + e) - loss(obj64 - e)) / (2 * h)
+            assert fd == pytest.approx(comp, rel=1e-5, abs=1e-6 * np.abs(grad).max())
+
+
+def test_noise_free_truth_is_fixed_point_of_update(tiny):
+    o, probe, obj, pos = tiny
+    pos = np.round(pos).astype(np.float32)
+    o._region_of_interest_shape = np.array([16, 16])
+    rows, cols = po.patch_indices(pos, (16, 16), obj.shape)
+    amps = np.abs(np.fft.fft2((probe[None] * obj[rows, cols]).astype(np.complex128))).astype(
+        np.float32)
+    obj2, probe2 = obj.copy(), probe.copy()
+    sp, op, chi, err = o.forward(obj2, probe2, pos, amps)
+    o.adjoint(obj2, probe2, op, sp, pos, chi, 0.5, 1, False)
+    assert rel(obj2, obj) < 1e-5 and rel(probe2, probe) < 1e-5
+
+
+def test_object_constraints_invariants():
+    rng = np.random.default_rng(3)
+    obj = (1.3 * np.exp(1j * (0.4 + 0.1 * rng.standard_normal((20, 20))))).astype(np.complex64)
+    obj[:5] *= 0.5
+    mask = np.zeros((20, 20), bool)
+    mask[:, :4] = True
+    out = po.OracleSingleslicePtychography.object_constraints(obj.copy(), mask)
+    assert abs(np.angle(out)[mask].mean()) < 1e-6
+    assert np.abs(out).max() <= 1.0 + 1e-6
+    np.testing.assert_allclose(np.abs(out[:5]), np.minimum(np.abs(obj[:5]), 1), rtol=1e-6)
+    pure = po.OracleSingleslicePtychography.object_constraints(obj.copy(), None,
+                                                               pure_phase_object=True)
+    np.testing.assert_allclose(np.abs(pure), 1.0, rtol=1e-6)
+
+
+def test_probe_com_constraint_recentres():
+    o = po.OracleSingleslicePtychography(energy=300e3)
+    p = po.ComplexProbeOracle(300e3, (32, 32), (0.3, 0.3), 25.0, 2.0).build()._array
+    p = po.fft_shift(p.astype(np.complex64), np.array([1.3, -0.7]))
+    x0, y0 = po.get_CoM(np.abs(p) ** 2, corner_centered=True)
+    assert abs(x0 - 1.3) < 0.05 and abs(y0 + 0.7) < 0.05
+    q = o.probe_com_constraint(p)
+    x1, y1 = po.get_CoM(np.abs(q) ** 2, corner_centered=True)
+    # the corner-centred CoM of a periodic frame is not exactly shift
+    # equivariant (tails wrap), so one recentring leaves a small residual
+    assert abs(x1) < 0.05 and abs(y1) < 0.05
+    assert np.hypot(x1, y1) < np.hypot(x0, y0) / 20
