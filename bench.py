@@ -145,8 +145,11 @@ def main():
     achieved = per_update * B / k_avg / 1e9
     log(f"k_fused avg {k_avg * 1e6:.1f} us per launch of {B} positions")
 
-    cpu_rate, sample = cpu_baseline(p, B, args.cpu_seconds)
-    log(f"cpu baseline {cpu_rate:,.1f} pos-updates/s")
+    if args.cpu_seconds > 0:
+        cpu_rate, sample = cpu_baseline(p, B, args.cpu_seconds)
+        log(f"cpu baseline {cpu_rate:,.1f} pos-updates/s")
+    else:
+        cpu_rate, sample = None, "skipped (--cpu-seconds 0)"
 
     out = {
         "metric": "position-updates/s",
@@ -179,7 +182,7 @@ def main():
             "bytes_per_update": per_update,
         },
         "cpu_baseline": {
-            "value": round(cpu_rate, 2),
+            "value": round(cpu_rate, 2) if cpu_rate is not None else None,
             "unit": "position-updates/s",
             "cores": 1,
             "kind": "port",

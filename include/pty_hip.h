@@ -75,7 +75,9 @@ int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px,
  * (phase_base_class.py:1905-1968, _sum_overlapping_patches_bincounts, as
  * called at ptychographic_methods.py:1859-1886):
  *   grad = scatter(conj(ps) * chi), spsum = scatter(|ps|^2);
- *   tilemax[t] = max spsum over 32 x 32 tile t (ceil(Px/32)*ceil(Py/32)). */
+ *   tilemax[t] = max spsum over object tile t, pty_scatter_num_tiles(Px, Py)
+ *   entries. */
+int pty_scatter_num_tiles(int Px, int Py);
 int pty_scatter_patches(const void* ps, const void* chi, const float* positions, int N,
                         const int* order, int B, int S, int Px, int Py, void* grad, float* spsum,
                         float* tilemax, pty_stream_t stream);
@@ -88,7 +90,7 @@ int pty_object_update(void* object, const void* grad, const float* spsum, const 
 
 /* Probe step sum_b conj(O_patch) * chi normalised by sum_b |O_patch|^2
  * (ptychographic_methods.py:1888-1906); reads the pre-update object.
- * `partials` needs pty_probe_partials_bytes(S, B) bytes. */
+ * `partials` is scratch of pty_probe_partials_bytes(S, B) bytes. */
 size_t pty_probe_partials_bytes(int S, int B);
 int pty_probe_update(void* probe, const void* chi, const void* object, int Px, int Py,
                      const float* positions, int N, const int* order, int B, int S,

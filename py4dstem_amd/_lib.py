@@ -43,6 +43,7 @@ _SIGNATURES = {
     "pty_forward_adjoint_batch": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
                                           c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p]),
+    "pty_scatter_num_tiles": (c_int, [c_int, c_int]),
     "pty_scatter_patches": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int,
                                     c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pty_object_update": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,

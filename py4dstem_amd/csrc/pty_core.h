@@ -450,15 +450,20 @@ template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs
 }
 
 // ---------------------------------------------------------------------------
-// Deterministic scatter-add by object pixel ("sort by pixel"): one work group
-// per TILE x TILE object tile walks the batch in order and sums, in float64,
-// conj(P_shift) * chi and |P_shift|^2 of the staged patches that cover each
-// of its pixels.  Per pixel the sum
-// order is the batch order, as in the reference's np.bincount.
+// Deterministic scatter-add by object pixel ("sort by pixel").  One work group
+// owns a TR x TC object tile, one pixel per thread.  For each segment of the
+// batch it (1) tests which positions' windows touch the tile, (2) compacts
+// the hits into an in-order list in LDS with a parallel prefix scan, and
+// (3) every thread sums, in float64 and in list (= batch) order,
+// conj(P_shift) * chi and |P_shift|^2 of the staged patch element covering its
+// pixel.  Per pixel the sum order is the batch order, as in the reference's
+// np.bincount; the float32 products are formed exactly as the reference does.
 // ---------------------------------------------------------------------------
-constexpr int TILE = 32;
-constexpr int GATHER_THREADS = 256;
-constexpr int GATHER_PX = TILE * TILE / GATHER_THREADS;  // pixels per thread
+constexpr int GT_R = 8;                  // tile rows
+constexpr int GT_C = 32;                 // tile columns
+constexpr int GATHER_THREADS = GT_R * GT_C;
+constexpr int GATHER_CAND = 4;           // candidates tested per thread per segment
+constexpr int GATHER_SEG = GATHER_THREADS * GATHER_CAND;
 
 struct GatherArgs {
   const cf* ps;            // [B][S][S] shifted probes
@@ -471,77 +476,206 @@ struct GatherArgs {
   float* tilemax;          // [num_tiles] max of spsum over the tile
 };
 
-// window-relative index of object coordinate r for a window anchored at x0,
-// or -1 if outside; Px >= S is required by the host.
-PTY_HD int window_index(int r, int x0, int S, int Px) {
-  const int d = wrapi(r - x0 + S / 2, Px);
-  return d < S ? (d + S / 2) % S : -1;
-}
-
-struct GatherAcc {
-  double re[GATHER_PX], im[GATHER_PX], s[GATHER_PX];
+struct GatherHit {
+  int b, x0, y0;
 };
 
-PTY_HD void gather_tile_origin(const GatherArgs& a, int tile, int& R0, int& C0) {
-  R0 = (tile / a.tiles_y) * TILE;
-  C0 = (tile % a.tiles_y) * TILE;
-}
+// LDS of one gather work group, and the view the GPU runner hands to phases
+struct GatherShared {
+  int scan[2][GATHER_THREADS];
+  GatherHit list[GATHER_SEG];
+  float red[GATHER_THREADS];
+};
 
-// one thread's accumulation over the whole batch (uniform skip per position)
-PTY_HD void gather_accumulate(const GatherArgs& a, int tile, int tid, GatherAcc& acc) {
-  int R0, C0;
-  gather_tile_origin(a, tile, R0, C0);
-  const int tr = (a.Px - R0) < TILE ? (a.Px - R0) : TILE;
-  const int tc = (a.Py - C0) < TILE ? (a.Py - C0) : TILE;
-  const int ty = tid / TILE, tx = tid % TILE;
-  for (int k = 0; k < GATHER_PX; ++k) { acc.re[k] = 0.0; acc.im[k] = 0.0; acc.s[k] = 0.0; }
-  const int S = a.S;
-  for (int b = 0; b < a.B; ++b) {
+template <class T> struct LdsArr {
+  T* p;
+  PTY_HD T ld(int i) const { return p[i]; }
+  PTY_HD void st(int i, T v) const { p[i] = v; }
+};
+
+struct GatherSharedView {
+  GatherShared* s;
+  PTY_HD LdsArr<int> scan(int i) const { return LdsArr<int>{s->scan[i]}; }
+  PTY_HD LdsArr<GatherHit> list() const { return LdsArr<GatherHit>{s->list}; }
+  PTY_HD LdsArr<float> red() const { return LdsArr<float>{s->red}; }
+};
+
+PTY_HD int gather_tiles_y(int Py) { return (Py + GT_C - 1) / GT_C; }
+PTY_HD int gather_num_tiles(int Px, int Py) { return ((Px + GT_R - 1) / GT_R) * gather_tiles_y(Py); }
+
+struct Gather {
+  struct Regs {
+    int cnt;
+    int total;
+    double re, im, s;
+  };
+
+  static PTY_HD void origin(const GatherArgs& a, int tile, int& R0, int& C0) {
+    R0 = (tile / a.tiles_y) * GT_R;
+    C0 = (tile % a.tiles_y) * GT_C;
+  }
+
+  // anchor of slot b, wrapped into the object; hit test against the tile
+  static PTY_HD bool test(const GatherArgs& a, int R0, int C0, int b, int& x0, int& y0) {
     int idx = a.order[b];
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
-    const int x0 = (int)rintf(a.positions[2 * idx]);
-    const int y0 = (int)rintf(a.positions[2 * idx + 1]);
+    x0 = wrapi((int)rintf(a.positions[2 * idx]), a.Px);
+    y0 = wrapi((int)rintf(a.positions[2 * idx + 1]), a.Py);
+    const int S = a.S;
+    const int tr = (a.Px - R0) < GT_R ? (a.Px - R0) : GT_R;
+    const int tc = (a.Py - C0) < GT_C ? (a.Py - C0) : GT_C;
     const int dr = wrapi(R0 - x0 + S / 2, a.Px);
     const int dc = wrapi(C0 - y0 + S / 2, a.Py);
-    const bool hit_r = dr < S || dr + tr > a.Px;
-    const bool hit_c = dc < S || dc + tc > a.Py;
-    if (!(hit_r && hit_c)) continue;  // uniform across the work group
-    const int j = (tx < tc) ? window_index(C0 + tx, y0, S, a.Py) : -1;
-    if (j < 0) continue;
-    for (int k = 0; k < GATHER_PX; ++k) {
-      const int rl = ty + k * (GATHER_THREADS / TILE);
-      if (rl >= tr) break;
-      const int i = window_index(R0 + rl, x0, S, a.Px);
-      if (i < 0) continue;
-      const size_t o = ((size_t)b * S + i) * S + j;
-      const cf p = a.ps[o];
-      const cf g = cconjmul(p, a.chi[o]);  // float32 product, as the reference forms it
-      acc.re[k] += (double)g.x;
-      acc.im[k] += (double)g.y;
-      acc.s[k] += (double)cabs2(p);
+    return (dr < S || dr + tr > a.Px) && (dc < S || dc + tc > a.Py);
+  }
+
+  // phase 1: count hits among this thread's candidates of the segment
+  template <class L> static PTY_HD void p_count(int tid, const GatherArgs& a, int tile, int seg0,
+                                                Regs& r, L scan0) {
+    int R0, C0, x0, y0;
+    origin(a, tile, R0, C0);
+    int c = 0;
+    for (int k = 0; k < GATHER_CAND; ++k) {
+      const int b = seg0 + tid * GATHER_CAND + k;
+      if (b < a.B && test(a, R0, C0, b, x0, y0)) ++c;
+    }
+    r.cnt = c;
+    scan0.st(tid, c);
+  }
+
+  // Hillis-Steele inclusive scan step over the per-thread counts
+  template <class L> static PTY_HD void p_scan(int tid, int d, L src, L dst) {
+    int v = src.ld(tid);
+    if (tid >= d) v += src.ld(tid - d);
+    dst.st(tid, v);
+  }
+
+  // phase 3: write this thread's hits at its offset of the in-order list
+  template <class L, class H> static PTY_HD void p_fill(int tid, const GatherArgs& a, int tile,
+                                                       int seg0, Regs& r, L incl, H list) {
+    int R0, C0, x0, y0;
+    origin(a, tile, R0, C0);
+    int o = incl.ld(tid) - r.cnt;
+    r.total = incl.ld(GATHER_THREADS - 1);
+    for (int k = 0; k < GATHER_CAND; ++k) {
+      const int b = seg0 + tid * GATHER_CAND + k;
+      if (b < a.B && test(a, R0, C0, b, x0, y0)) {
+        GatherHit h;
+        h.b = b; h.x0 = x0; h.y0 = y0;
+        list.st(o++, h);
+      }
     }
   }
-}
 
-// writes the tile's pixels; returns the thread's max of spsum
-PTY_HD float gather_store(const GatherArgs& a, int tile, int tid, const GatherAcc& acc) {
-  int R0, C0;
-  gather_tile_origin(a, tile, R0, C0);
-  const int tr = (a.Px - R0) < TILE ? (a.Px - R0) : TILE;
-  const int tc = (a.Py - C0) < TILE ? (a.Py - C0) : TILE;
-  const int ty = tid / TILE, tx = tid % TILE;
-  float m = 0.f;
-  if (tx >= tc) return m;
-  for (int k = 0; k < GATHER_PX; ++k) {
-    const int rl = ty + k * (GATHER_THREADS / TILE);
-    if (rl >= tr) break;
-    const size_t o = (size_t)(R0 + rl) * a.Py + (C0 + tx);
-    a.grad[o] = mkcf((float)acc.re[k], (float)acc.im[k]);
-    const float s = (float)acc.s[k];
-    a.spsum[o] = s;
-    m = s > m ? s : m;
+  // window element of pixel (row, col) for anchor (x0, y0), or -1 outside
+  static PTY_HD int welem(int row, int col, const GatherHit& h, int S, int Px, int Py) {
+    int dr = row - h.x0 + S / 2;
+    dr += dr < 0 ? Px : 0;
+    dr -= dr >= Px ? Px : 0;
+    int dc = col - h.y0 + S / 2;
+    dc += dc < 0 ? Py : 0;
+    dc -= dc >= Py ? Py : 0;
+    if (dr >= S || dc >= S) return -1;
+    return ((dr + S / 2) & (S - 1)) * S + ((dc + S / 2) & (S - 1));
   }
-  return m;
+
+  // phase 4: accumulate this thread's pixel over the segment's hit list
+  template <class H> static PTY_HD void p_accumulate(int tid, const GatherArgs& a, int tile,
+                                                     Regs& r, H list) {
+    int R0, C0;
+    origin(a, tile, R0, C0);
+    const int row = R0 + tid / GT_C, col = C0 + tid % GT_C;
+    if (row >= a.Px || col >= a.Py) return;
+    const int S = a.S;
+    const size_t SS = (size_t)S * S;
+    int h = 0;
+    // four hits per round: loads first, then the float64 adds in order
+    for (; h + 4 <= r.total; h += 4) {
+      cf p[4], c[4];
+      bool in[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const GatherHit e = list.ld(h + u);
+        const int w = welem(row, col, e, S, a.Px, a.Py);
+        in[u] = w >= 0;
+        const size_t o = (size_t)e.b * SS + (in[u] ? w : 0);
+        p[u] = a.ps[o];
+        c[u] = a.chi[o];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (in[u]) {
+          const cf g = cconjmul(p[u], c[u]);
+          r.re += (double)g.x;
+          r.im += (double)g.y;
+          r.s += (double)cabs2(p[u]);
+        }
+      }
+    }
+    for (; h < r.total; ++h) {
+      const GatherHit e = list.ld(h);
+      const int w = welem(row, col, e, S, a.Px, a.Py);
+      if (w < 0) continue;
+      const size_t o = (size_t)e.b * SS + w;
+      const cf p = a.ps[o];
+      const cf g = cconjmul(p, a.chi[o]);
+      r.re += (double)g.x;
+      r.im += (double)g.y;
+      r.s += (double)cabs2(p);
+    }
+  }
+
+  // phase 5: store the pixel, stage its spsum for the tile maximum
+  template <class L> static PTY_HD void p_store(int tid, const GatherArgs& a, int tile, Regs& r,
+                                                L red) {
+    int R0, C0;
+    origin(a, tile, R0, C0);
+    const int row = R0 + tid / GT_C, col = C0 + tid % GT_C;
+    float m = 0.f;
+    if (row < a.Px && col < a.Py) {
+      const size_t o = (size_t)row * a.Py + col;
+      a.grad[o] = mkcf((float)r.re, (float)r.im);
+      m = (float)r.s;
+      a.spsum[o] = m;
+    }
+    red.st(tid, m);
+  }
+};
+
+// Whole gather sequence for one tile (see fused_position for the runner).
+template <class Run> PTY_HD void gather_tile(Run& run, const GatherArgs& a, int tile) {
+  using G = Gather;
+  using Regs = G::Regs;
+  run.phase([&](int tid, Regs& r, auto& sh) { r.re = 0.0; r.im = 0.0; r.s = 0.0; (void)tid; (void)sh; });
+  for (int seg0 = 0; seg0 < a.B; seg0 += GATHER_SEG) {
+    run.phase([&](int tid, Regs& r, auto& sh) { G::p_count(tid, a, tile, seg0, r, sh.scan(0)); });
+    run.barrier();
+    int src = 0;
+    for (int d = 1; d < GATHER_THREADS; d <<= 1) {
+      run.phase([&](int tid, Regs&, auto& sh) { G::p_scan(tid, d, sh.scan(src), sh.scan(1 - src)); });
+      run.barrier();
+      src = 1 - src;
+    }
+    run.phase([&](int tid, Regs& r, auto& sh) { G::p_fill(tid, a, tile, seg0, r, sh.scan(src), sh.list()); });
+    run.barrier();
+    run.phase([&](int tid, Regs& r, auto& sh) { G::p_accumulate(tid, a, tile, r, sh.list()); });
+    run.barrier();
+  }
+  run.phase([&](int tid, Regs& r, auto& sh) { G::p_store(tid, a, tile, r, sh.red()); });
+  run.barrier();
+  for (int d = GATHER_THREADS / 2; d > 0; d >>= 1) {
+    run.phase([&](int tid, Regs&, auto& sh) {
+      if (tid < d) {
+        auto red = sh.red();
+        const float x = red.ld(tid), y = red.ld(tid + d);
+        red.st(tid, x > y ? x : y);
+      }
+    });
+    run.barrier();
+  }
+  run.phase([&](int tid, Regs&, auto& sh) {
+    if (tid == 0) a.tilemax[tile] = sh.red().ld(0);
+  });
 }
 
 }  // namespace pty

@@ -45,6 +45,8 @@ int check_launch(const char* what) {
   return PTY_OK;
 }
 
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 bool aligned8(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 7u) == 0; }
 
 inline hipStream_t st(pty_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
@@ -183,18 +185,23 @@ __global__ void k_permute(const float* __restrict__ in, float* __restrict__ out,
 // ---------------------------------------------------------------------------
 // scatter-add by tile
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(GATHER_THREADS) void k_gather(GatherArgs a) {
-  __shared__ float smax[GATHER_THREADS];
-  const int tile = blockIdx.x, tid = threadIdx.x;
-  GatherAcc acc;
-  gather_accumulate(a, tile, tid, acc);
-  smax[tid] = gather_store(a, tile, tid, acc);
-  __syncthreads();
-  for (int s = GATHER_THREADS / 2; s > 0; s >>= 1) {
-    if (tid < s) smax[tid] = fmaxf(smax[tid], smax[tid + s]);
-    __syncthreads();
+struct GatherGpuRunner {
+  Gather::Regs& r;
+  GatherSharedView sh;
+  int tid;
+  template <class Fn> __device__ __forceinline__ void phase(Fn fn) {
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    fn(t, r, sh);
   }
-  if (tid == 0) a.tilemax[tile] = smax[0];
+  __device__ __forceinline__ void barrier() { __syncthreads(); }
+};
+
+__global__ __launch_bounds__(GATHER_THREADS) void k_gather(GatherArgs a) {
+  __shared__ GatherShared sh;
+  Gather::Regs r;
+  GatherGpuRunner run{r, GatherSharedView{&sh}, (int)threadIdx.x};
+  gather_tile(run, a, (int)blockIdx.x);
 }
 
 __device__ float block_max(float v, float* sh) {
@@ -242,7 +249,6 @@ __global__ __launch_bounds__(256) void k_object_update(cf* __restrict__ obj, con
 // probe update: partial sums over chunks of the batch, then one work group
 // ---------------------------------------------------------------------------
 constexpr int PROBE_CHUNK = 64;
-constexpr int PROBE_UPDATE_THREADS = 1024;
 
 struct ProbePartialArgs {
   const cf* chi;
@@ -281,36 +287,40 @@ __global__ __launch_bounds__(256) void k_probe_partial(ProbePartialArgs a) {
   p[2] = so;
 }
 
-template <int S>
-__global__ __launch_bounds__(PROBE_UPDATE_THREADS) void k_probe_update(cf* __restrict__ probe,
-                                                                       const double* __restrict__ part,
-                                                                       int nchunk, float step, float c1,
-                                                                       float c2) {
-  constexpr int PER = S * S / PROBE_UPDATE_THREADS;
-  __shared__ float sh[PROBE_UPDATE_THREADS];
-  cf g[PER];
-  float so[PER];
-  float m = 0.f;
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int pix = threadIdx.x + k * PROBE_UPDATE_THREADS;
+// per-pixel sum of the chunk partials; per-block max of sum |O|^2
+__global__ __launch_bounds__(256) void k_probe_reduce(const double* __restrict__ part, int nchunk,
+                                                      int npix, cf* __restrict__ gsum,
+                                                      float* __restrict__ osum,
+                                                      float* __restrict__ bmax) {
+  __shared__ float sh[256];
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  float so = 0.f;
+  if (pix < npix) {
     double re = 0.0, im = 0.0, s = 0.0;
     for (int c = 0; c < nchunk; ++c) {
-      const double* p = part + ((size_t)c * S * S + pix) * 3;
+      const double* p = part + ((size_t)c * npix + pix) * 3;
       re += p[0];
       im += p[1];
       s += p[2];
     }
-    g[k] = mkcf((float)re, (float)im);
-    so[k] = (float)s;
-    m = fmaxf(m, so[k]);
+    gsum[pix] = mkcf((float)re, (float)im);
+    so = (float)s;
+    osum[pix] = so;
   }
+  const float m = block_max(so, sh);
+  if (threadIdx.x == 0) bmax[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(256) void k_probe_apply(cf* __restrict__ probe, const cf* __restrict__ gsum,
+                                                     const float* __restrict__ osum,
+                                                     const float* __restrict__ bmax, int nblk, int npix,
+                                                     float step, float c1, float c2) {
+  __shared__ float sh[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) m = fmaxf(m, bmax[i]);
   m = block_max(m, sh);
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int pix = threadIdx.x + k * PROBE_UPDATE_THREADS;
-    probe[pix] = step_add(probe[pix], g[k], overlap_norm(so[k], m, c1, c2), step);
-  }
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix < npix) probe[pix] = step_add(probe[pix], gsum[pix], overlap_norm(osum[pix], m, c1, c2), step);
 }
 
 // ---------------------------------------------------------------------------
@@ -504,7 +514,7 @@ int launch_fft(const cf* in, cf* out, int batch, int S, bool inv, int natural, h
   return check_launch("k_fft2");
 }
 
-int num_tiles(int Px, int Py) { return ((Px + TILE - 1) / TILE) * ((Py + TILE - 1) / TILE); }
+int num_tiles(int Px, int Py) { return gather_num_tiles(Px, Py); }
 
 int launch_gather(const GatherArgs& a, int Px, int Py, hipStream_t stream) {
   hipLaunchKernelGGL(k_gather, dim3(num_tiles(Px, Py)), dim3(GATHER_THREADS), 0, stream, a);
@@ -527,6 +537,31 @@ int launch_object_update(cf* obj, const cf* grad, const float* spsum, const floa
 
 int probe_nchunk(int B) { return (B + PROBE_CHUNK - 1) / PROBE_CHUNK; }
 
+// partials [nchunk][S*S][3] doubles, then gsum [S*S] cf, osum [S*S] f32, bmax
+size_t probe_ws_bytes(int S, int B) {
+  const size_t S2 = (size_t)S * S;
+  return align256((size_t)probe_nchunk(B) * S2 * 24) + align256(S2 * 8) + align256(S2 * 4) +
+         align256((S2 / 256 + 1) * 4);
+}
+
+int launch_probe_apply(cf* probe, double* ws, int S, int B, float step, float nmin, hipStream_t stream) {
+  const size_t S2 = (size_t)S * S;
+  char* base = (char*)ws;
+  double* part = ws;
+  cf* gsum = (cf*)(base + align256((size_t)probe_nchunk(B) * S2 * 24));
+  float* osum = (float*)((char*)gsum + align256(S2 * 8));
+  float* bmax = (float*)((char*)osum + align256(S2 * 4));
+  const int npix = (int)S2, nblk = (npix + 255) / 256;
+  hipLaunchKernelGGL(k_probe_reduce, dim3(nblk), dim3(256), 0, stream, part, probe_nchunk(B), npix, gsum, osum,
+                     bmax);
+  int rc = check_launch("k_probe_reduce");
+  if (rc) return rc;
+  const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
+  hipLaunchKernelGGL(k_probe_apply, dim3(nblk), dim3(256), 0, stream, probe, gsum, osum, bmax, nblk, npix, step,
+                     c1, c2);
+  return check_launch("k_probe_apply");
+}
+
 int launch_probe_update(cf* probe, const cf* chi, const cf* obj, int Px, int Py, const float* pos, int N,
                         const int* order, int B, int S, float step, float nmin, double* part,
                         hipStream_t stream) {
@@ -535,14 +570,7 @@ int launch_probe_update(cf* probe, const cf* chi, const cf* obj, int Px, int Py,
   hipLaunchKernelGGL(k_probe_partial, dim3((S * S + 255) / 256, nchunk), dim3(256), 0, stream, pa);
   int rc = check_launch("k_probe_partial");
   if (rc) return rc;
-  const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
-  if (S == 64)
-    hipLaunchKernelGGL(k_probe_update<64>, dim3(1), dim3(PROBE_UPDATE_THREADS), 0, stream, probe, part,
-                       nchunk, step, c1, c2);
-  else
-    hipLaunchKernelGGL(k_probe_update<128>, dim3(1), dim3(PROBE_UPDATE_THREADS), 0, stream, probe, part,
-                       nchunk, step, c1, c2);
-  return check_launch("k_probe_update");
+  return launch_probe_apply(probe, part, S, B, step, nmin, stream);
 }
 
 int launch_constraints(cf* obj, int Px, int Py, const unsigned char* mask, float shrink, int pure,
@@ -574,8 +602,6 @@ std::vector<int> subdivide(int n, int max_batch) {
   return v;
 }
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
 struct WsLayout {
   size_t phat, ps, chi, err, grad, spsum, tilemax, ppart, cpart, total;
 };
@@ -593,7 +619,7 @@ WsLayout layout(const pty_problem& p, const pty_gd_params& q) {
   L.grad = o; o = align256(o + npix * 8);
   L.spsum = o; o = align256(o + npix * 4);
   L.tilemax = o; o = align256(o + (size_t)num_tiles(p.Px, p.Py) * 4);
-  L.ppart = o; o = align256(o + (q.fix_probe ? 0 : (size_t)probe_nchunk(q.max_batch) * S2 * 24));
+  L.ppart = o; o = align256(o + (q.fix_probe ? 0 : probe_ws_bytes(p.S, q.max_batch)));
   L.cpart = o; o = align256(o + (size_t)CONSTRAINT_BLOCKS * 16);
   L.total = o;
   return L;
@@ -672,7 +698,7 @@ int pty_scatter_patches(const void* ps, const void* chi, const float* positions,
   if (!ps || !chi || !positions || !order || !grad || !spsum || !tilemax) return fail(PTY_E_ARG, "null buffer in scatter");
   GatherArgs a{};
   a.ps = (const cf*)ps; a.chi = (const cf*)chi; a.positions = positions; a.order = order;
-  a.N = N; a.B = B; a.S = S; a.Px = Px; a.Py = Py; a.tiles_y = (Py + TILE - 1) / TILE;
+  a.N = N; a.B = B; a.S = S; a.Px = Px; a.Py = Py; a.tiles_y = gather_tiles_y(Py);
   a.grad = (cf*)grad; a.spsum = spsum; a.tilemax = tilemax;
   return launch_gather(a, Px, Py, st(stream));
 }
@@ -685,7 +711,9 @@ int pty_object_update(void* object, const void* grad, const float* spsum, const 
                               normalization_min, st(stream));
 }
 
-size_t pty_probe_partials_bytes(int S, int B) { return (size_t)probe_nchunk(B) * S * S * 24; }
+size_t pty_probe_partials_bytes(int S, int B) { return probe_ws_bytes(S, B); }
+
+int pty_scatter_num_tiles(int Px, int Py) { return gather_num_tiles(Px, Py); }
 
 int pty_probe_update(void* probe, const void* chi, const void* object, int Px, int Py, const float* positions, int N,
                      const int* order, int B, int S, float step_size, float normalization_min, void* partials,
@@ -769,21 +797,14 @@ int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const in
       }
       GatherArgs g{};
       g.ps = ps; g.chi = chi; g.positions = p.positions; g.order = ob; g.N = p.N; g.B = nb; g.S = p.S;
-      g.Px = p.Px; g.Py = p.Py; g.tiles_y = (p.Py + TILE - 1) / TILE; g.grad = grad; g.spsum = spsum;
+      g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
       g.tilemax = tilemax;
       if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
       if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
                                      q.normalization_min, stream)))
         return rc;
       if (!q.fix_probe) {
-        const float c1 = (float)(1.0 - (double)q.normalization_min), c2 = q.normalization_min;
-        if (p.S == 64)
-          hipLaunchKernelGGL(k_probe_update<64>, dim3(1), dim3(PROBE_UPDATE_THREADS), 0, stream, probe, ppart,
-                             probe_nchunk(nb), q.step_size, c1, c2);
-        else
-          hipLaunchKernelGGL(k_probe_update<128>, dim3(1), dim3(PROBE_UPDATE_THREADS), 0, stream, probe, ppart,
-                             probe_nchunk(nb), q.step_size, c1, c2);
-        if ((rc = check_launch("k_probe_update"))) return rc;
+        if ((rc = launch_probe_apply(probe, ppart, p.S, nb, q.step_size, q.normalization_min, stream))) return rc;
       }
       start += nb;
     }

@@ -14,7 +14,6 @@ from . import _lib
 from ._lib import PtyGdParams, PtyProblem, c_void_p, ptr, stream_handle
 
 SUPPORTED_FRAMES = (64, 128)
-TILE = 32
 
 
 def _req(t, dtype, shape=None, name="tensor"):
@@ -99,7 +98,8 @@ def forward_adjoint(probe_hat, obj, positions, amplitudes_k, order, with_far=Fal
 
 
 def num_tiles(Px, Py):
-    return ((Px + TILE - 1) // TILE) * ((Py + TILE - 1) // TILE)
+    """Number of object tiles of the deterministic scatter (tilemax length)."""
+    return int(_lib.load().pty_scatter_num_tiles(int(Px), int(Py)))
 
 
 def scatter(ps, chi, positions, order, object_shape):

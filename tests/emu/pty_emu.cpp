@@ -79,6 +79,59 @@ template <int S> struct EmuRunner {
   }
 };
 
+// typed LDS array with the same race/bounds bookkeeping as LdsCheck
+template <class T> struct LdsArrCheck {
+  T* mem;
+  int n;
+  int* wt;
+  int* rd;
+  const int* cur;
+  T ld(int i) const {
+    if (i < 0 || i >= n) { note_violation("lds load out of range"); return T{}; }
+    if (wt[i] >= 0 && wt[i] != *cur) note_violation("lds read of a word another thread wrote");
+    if (rd[i] == -1) rd[i] = *cur; else if (rd[i] != *cur) rd[i] = -2;
+    return mem[i];
+  }
+  void st(int i, T v) const {
+    if (i < 0 || i >= n) { note_violation("lds store out of range"); return; }
+    if (wt[i] >= 0 && wt[i] != *cur) note_violation("lds write-write race");
+    if (rd[i] == -2 || (rd[i] >= 0 && rd[i] != *cur)) note_violation("lds write of a word another thread read");
+    wt[i] = *cur;
+    mem[i] = v;
+  }
+};
+
+template <class T> struct TrackedArr {
+  std::vector<T> mem;
+  std::vector<int> wt, rd;
+  explicit TrackedArr(int n) : mem(n), wt(n, -1), rd(n, -1) {}
+  LdsArrCheck<T> view(const int* cur) { return LdsArrCheck<T>{mem.data(), (int)mem.size(), wt.data(), rd.data(), cur}; }
+  void clear() { std::fill(wt.begin(), wt.end(), -1); std::fill(rd.begin(), rd.end(), -1); }
+};
+
+struct GatherEmu {
+  int cur = -1;
+  TrackedArr<int> scan0{GATHER_THREADS}, scan1{GATHER_THREADS};
+  TrackedArr<GatherHit> list{GATHER_SEG};
+  TrackedArr<float> red{GATHER_THREADS};
+  std::vector<Gather::Regs> regs = std::vector<Gather::Regs>(GATHER_THREADS);
+  struct View {
+    GatherEmu* e;
+    LdsArrCheck<int> scan(int i) const { return (i ? e->scan1 : e->scan0).view(&e->cur); }
+    LdsArrCheck<GatherHit> list() const { return e->list.view(&e->cur); }
+    LdsArrCheck<float> red() const { return e->red.view(&e->cur); }
+  };
+  template <class Fn> void phase(Fn fn) {
+    View v{this};
+    for (int t = 0; t < GATHER_THREADS; ++t) {
+      cur = t;
+      fn(t, regs[t], v);
+    }
+    cur = -1;
+  }
+  void barrier() { scan0.clear(); scan1.clear(); list.clear(); red.clear(); }
+};
+
 template <int S> static void run_fused(const FusedArgs& a) {
   EmuRunner<S> run(&a);
   for (int b = a.b0; b < a.B; ++b) fused_position<S>(run, a, b);
@@ -184,20 +237,15 @@ int emu_gather(int S, const float* ps, const float* chi, const float* positions,
   GatherArgs a;
   a.ps = (const cf*)ps; a.chi = (const cf*)chi; a.positions = positions; a.order = order;
   a.N = N; a.B = B; a.S = S; a.Px = Px; a.Py = Py;
-  a.tiles_y = (Py + TILE - 1) / TILE;
-  const int tiles_x = (Px + TILE - 1) / TILE;
+  a.tiles_y = gather_tiles_y(Py);
   a.grad = (cf*)grad; a.spsum = spsum; a.tilemax = tilemax;
-  for (int tile = 0; tile < tiles_x * a.tiles_y; ++tile) {
-    float m = 0.f;
-    for (int tid = 0; tid < GATHER_THREADS; ++tid) {
-      GatherAcc acc;
-      gather_accumulate(a, tile, tid, acc);
-      float t = gather_store(a, tile, tid, acc);
-      m = t > m ? t : m;
-    }
-    tilemax[tile] = m;
+  for (int tile = 0; tile < gather_num_tiles(Px, Py); ++tile) {
+    GatherEmu run;
+    gather_tile(run, a, tile);
   }
   return 0;
 }
+
+int emu_gather_num_tiles(int Px, int Py) { return gather_num_tiles(Px, Py); }
 
 }  // extern "C"

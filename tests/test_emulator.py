@@ -98,12 +98,14 @@ def test_scatter_is_bincount_in_batch_order(emu, small_case):
     ps = (rng.standard_normal((B, S, S)) + 1j * rng.standard_normal((B, S, S))).astype(np.complex64)
     ones = np.ones_like(ps)
     shape = o._object.shape
-    ntiles = ((shape[0] + 31) // 32) * ((shape[1] + 31) // 32)
+    ntiles = emu.emu_gather_num_tiles(shape[0], shape[1])
     grad = np.zeros(shape, np.complex64)
     spsum = np.zeros(shape, np.float32)
     tmax = np.zeros(ntiles, np.float32)
+    emu.emu_reset()
     emu.emu_gather(S, cptr(ps), cptr(ones), cptr(pos), len(pos), cptr(order), B, shape[0],
                    shape[1], cptr(grad), cptr(spsum), cptr(tmax))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
     # chi = 1 makes the float32 product exact: the float64 sums must then be
     # bit-identical to np.bincount in batch order
     ref = po.scatter_bincount(np.conj(ps), pos[order], (S, S), shape)
@@ -111,3 +113,24 @@ def test_scatter_is_bincount_in_batch_order(emu, small_case):
     sq = (ps.real * ps.real + ps.imag * ps.imag).astype(np.float32)
     np.testing.assert_array_equal(spsum, po.scatter_bincount(sq, pos[order], (S, S), shape))
     assert tmax.max() == spsum.max()
+
+
+def test_scatter_multi_segment_order(emu, small_case):
+    """More batch slots than one hit-list segment holds (and repeated
+    positions): per-pixel float64 order must still be the batch order."""
+    _, o = small_case
+    S = 64
+    rng = np.random.default_rng(12)
+    B = 1500
+    order = rng.integers(0, o._num_diffraction_patterns, B).astype(np.int32)
+    pos = o._positions_px
+    ps = (rng.standard_normal((B, S, S)) + 1j * rng.standard_normal((B, S, S))).astype(np.complex64)
+    shape = o._object.shape
+    grad = np.zeros(shape, np.complex64)
+    spsum = np.zeros(shape, np.float32)
+    tmax = np.zeros(emu.emu_gather_num_tiles(*shape), np.float32)
+    emu.emu_reset()
+    emu.emu_gather(S, cptr(ps), cptr(np.ones_like(ps)), cptr(pos), len(pos), cptr(order), B,
+                   shape[0], shape[1], cptr(grad), cptr(spsum), cptr(tmax))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    np.testing.assert_array_equal(grad, po.scatter_bincount(np.conj(ps), pos[order], (S, S), shape))

@@ -125,6 +125,18 @@ def test_scatter_bitexact_and_updates_match_oracle(dev, small_case):
     np.testing.assert_array_equal(grad.cpu().numpy(), ref)
     assert tmax.max().item() == spsum.max().item()
 
+    # several hit-list segments, repeated positions
+    B2 = 2500
+    order2 = rng.integers(0, o._num_diffraction_patterns, B2).astype(np.int32)
+    ps2n = (rng.standard_normal((B2, S, S)) + 1j * rng.standard_normal((B2, S, S))).astype(
+        np.complex64)
+    ps2 = torch.from_numpy(ps2n).to(dev)
+    g2, _, _ = K().scatter(ps2, torch.ones_like(ps2), pos, torch.from_numpy(order2).to(dev),
+                           o._object.shape)
+    np.testing.assert_array_equal(g2.cpu().numpy(), po.scatter_bincount(
+        np.conj(ps2n), o._positions_px[order2], (S, S), o._object.shape))
+    del ps2, g2
+
     obj = (o._object * np.exp(1j * 0.1 * rng.standard_normal(o._object.shape))).astype(np.complex64)
     probe = o._probe.astype(np.complex64)
     obj_t = torch.from_numpy(obj).to(dev)
