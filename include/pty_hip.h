@@ -109,6 +109,28 @@ int pty_object_constraints(void* object, int Px, int Py, const unsigned char* ma
 /* Probe centre-of-mass recentring (ptychographic_constraints.py:894-918). */
 int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream);
 
+/* ---- loss and gradients (backward of the forward model) ------------------ */
+
+/* L = sum_b sum (|fft2(P_shift_b * O_patch_b)| - A_b)^2 over the num_slots
+ * entries of `order` (indices into the N positions), processed `chunk` slots
+ * at a time, and its gradients
+ * as dL/dRe + i dL/dIm, each multiplied by `scale`:
+ *   grad_object = -2 S^2 scatter(conj(P_shift) chi)
+ *       (the un-normalised object term of _gradient_descent_adjoint,
+ *        ptychographic_methods.py:1878-1884);
+ *   grad_probe  = -2 S^2 ifft2(sum_b fft2(conj(O_patch_b) chi_b) conj(T_b))
+ *       (chain rule through fft_shift, phase/utils.py:508-533; equals the
+ *        reference's probe term, ptychographic_methods.py:1900-1906, only
+ *        for integer positions).
+ * `loss` is one float64 on the device; NULL grad pointers skip that part. */
+size_t pty_loss_workspace_bytes(int S, int Px, int Py, int num_slots, int chunk);
+int pty_loss_gradients(const void* object, const void* probe, int Px, int Py,
+                       const float* positions, int N, const float* amplitudes,
+                       const int* order, int num_slots, int chunk, int S, double scale,
+                       double* loss,
+                       void* grad_object, void* grad_probe, void* workspace,
+                       size_t workspace_bytes, pty_stream_t stream);
+
 /* ---- whole reconstruction loop ------------------------------------------ */
 
 typedef struct pty_problem {

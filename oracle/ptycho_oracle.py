@@ -37,6 +37,7 @@ PINNING = {
     "fit_plane": "exact recovery of a noiseless plane",
     "forward": "Parseval, projection fixed point, A=0 => chi=-psi, angle(0)=0",
     "adjoint": "noise-free ground truth is a fixed point; FP64 finite differences",
+    "loss_gradients": "FP64 central finite differences (object and probe pixels)",
     "object_constraints": "mean phase outside FOV is 0 after shrinkage; |O|<=1",
     "probe_com_constraint": "probe CoM ~ 0 after recentring",
     "reconstruct": "composition of the above; c64 vs c128 sensitivity measured",
@@ -657,6 +658,32 @@ class OracleSingleslicePtychography:
                 np.sum(np.conj(object_patches) * exit_waves, axis=0) * object_normalization
             )
         return obj, probe
+
+    def loss_gradients(self, obj, probe, positions_px, amplitudes):
+        """Gradients of L = sum_b sum (|fft2(P_shift_b * O_b)| - A_b)^2 as
+        dL/dRe + i dL/dIm.
+
+        Object: -2 S^2 scatter(conj(P_shift) chi), i.e. the un-normalised,
+        un-stepped object term of _gradient_descent_adjoint
+        (ptychographic_methods.py:1878-1884).  Probe: the chain rule through
+        the Fourier sub-pixel shift P_shift = ifft2(fft2(P) T_b)
+        (phase/utils.py:508-533), -2 S^2 sum_b ifft2(fft2(conj(O_b) chi_b)
+        conj(T_b)); it equals the reference's probe term
+        (ptychographic_methods.py:1900-1906) only when every fraction is 0.
+        Returns (loss, grad_object, grad_probe).
+        """
+        mode = self._mode
+        roi = self._region_of_interest_shape
+        S2 = int(roi[0]) * int(roi[1])
+        sp, op, chi, err = self.forward(obj, probe, positions_px, amplitudes)
+        g_obj = -2.0 * S2 * scatter_bincount(np.conj(sp) * chi, positions_px, roi, obj.shape)
+        frac = positions_px - np.round(positions_px)
+        # per-position adjoint shift ifft2(fft2(g_b) conj(T_b)); fft_shift would
+        # broadcast a 3-D array against 3-D positions as an outer product
+        T = fourier_translation_operator(frac, tuple(roi))
+        back = _ifft2(_fft2(np.conj(op) * chi, mode) * np.conj(T), mode)
+        g_probe = -2.0 * S2 * np.sum(back, axis=0)
+        return err, g_obj, g_probe
 
     # -- constraints --------------------------------------------------------
     @staticmethod

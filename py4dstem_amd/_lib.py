@@ -55,6 +55,10 @@ _SIGNATURES = {
     "pty_object_constraints": (c_int, [c_void_p, c_int, c_int, c_void_p, c_float, c_int,
                                        c_void_p, c_void_p]),
     "pty_probe_com_constraint": (c_int, [c_void_p, c_int, c_void_p]),
+    "pty_loss_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
+    "pty_loss_gradients": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p,
+                                   c_void_p, c_int, c_int, c_int, ctypes.c_double, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pty_gd_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams)]),
     "pty_gd_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams), c_void_p,
                            c_void_p, c_void_p, c_size_t, c_void_p]),

@@ -222,6 +222,8 @@ struct FusedArgs {
   cf* chi_out;             // [B][S][S] chi = ifft2(A e^{i angle F} - F) (natural)
   double* err;             // [B] sum (A - |F|)^2 per position
   cf* fwd_out;             // optional [B][S][S] far field F (permuted), or null
+  const cf* chi_in;        // probe-gradient pass: [B][S][S] chi (natural)
+  cf* pgrad_out;           // probe-gradient pass: [B][S][S] (kernel layout)
 };
 
 template <int S> struct Fused {
@@ -269,7 +271,8 @@ template <int S> struct Fused {
   }
 
   // P0: anchor of slot b and the ramp table (threads tid < 2S)
-  template <class L> static PTY_HD void p_setup(int tid, const FusedArgs& a, int b, Regs& r, L lds) {
+  template <class L> static PTY_HD void p_setup(int tid, const FusedArgs& a, int b, Regs& r, L lds,
+                                                bool conj_ramp = false) {
     int idx = a.order[b];
     PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;  // memory-safe on bad input
@@ -282,7 +285,8 @@ template <int S> struct Fused {
     if (tid < 2 * S) {
       const int k = tid < S ? tid : tid - S;
       const float frac = tid < S ? px - rx : py - ry;
-      lds.st(RAMP + tid, ramp1(k, frac));
+      // (-2 pi kf) * (-frac) is the exact negation: conj of the forward ramp
+      lds.st(RAMP + tid, ramp1(k, conj_ramp ? -frac : frac));
     }
   }
 
@@ -405,6 +409,39 @@ template <int S> struct Fused {
     for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
   }
 
+  // probe gradient, first phase: g = conj(O_patch) * chi from global memory,
+  // forward column pass A from registers
+  template <class L> static PTY_HD void p_pgrad_colA(int tid, const FusedArgs& a, int b, Regs& r,
+                                                     L lds, const cf* tw) {
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+    const cf* chi_b = a.chi_in + (size_t)b * S * S;
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int i = n2 + N2 * m;
+      r.v[m] = cconjmul(a.object[object_offset(a, r, i, cc)], chi_b[i * S + cc]);
+    }
+    P::template A<false>(n2, r.v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) lds.st(F::at(n2 + N2 * m, cc), r.v[m]);
+  }
+
+  // probe gradient, last phase: row pass B completes fft2(g); multiply by the
+  // conjugate ramp held in the LDS table; store in kernel layout
+  template <class L> static PTY_HD void p_pgrad_out(int tid, const FusedArgs& a, int b, Regs& r,
+                                                    L lds, const cf* tw) {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
+    P::template B<false>(t, r.v, tw);
+    const cf ex = lds.ld(RAMP + F::kidx(rr));
+    cf* out = a.pgrad_out + (size_t)b * S * S + rr * S + N1 * t;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      const cf e = cmul(ex, lds.ld(RAMP + S + F::kidx(N1 * t + j)));
+      out[j] = cmul(r.v[j], e);
+    }
+  }
+
   // P11: chi (natural order) out
   template <class L> static PTY_HD void p_outputs(int tid, const FusedArgs& a, int b, Regs& r,
                                                   L lds, const cf* tw) {
@@ -449,6 +486,23 @@ template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs
   run.reduce_err(b);  // ends with a barrier
 }
 
+// Backward of the probe path for one position:
+//   pgrad_out[b] = fft2(conj(O_patch) * chi) * conj(T_b)   (kernel layout)
+template <int S, class Run> PTY_HD void probe_grad_position(Run& run, const FusedArgs& a, int b) {
+  using K = Fused<S>;
+  using Regs = typename K::Regs;
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, true); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_pgrad_colA(tid, a, b, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_pgrad_out(tid, a, b, r, lds, tw); });
+  run.barrier();  // the ramp table and frame are rewritten by the next position
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic scatter-add by object pixel ("sort by pixel").  One work group
 // owns a TR x TC object tile, one pixel per thread.  For each segment of the
@@ -474,6 +528,7 @@ struct GatherArgs {
   cf* grad;                // [Px][Py] float32 of the float64 sums
   float* spsum;            // [Px][Py]
   float* tilemax;          // [num_tiles] max of spsum over the tile
+  double* acc;             // optional [Px][Py][2]: add the float64 sums here instead
 };
 
 struct GatherHit {
@@ -634,9 +689,14 @@ struct Gather {
     float m = 0.f;
     if (row < a.Px && col < a.Py) {
       const size_t o = (size_t)row * a.Py + col;
-      a.grad[o] = mkcf((float)r.re, (float)r.im);
-      m = (float)r.s;
-      a.spsum[o] = m;
+      if (a.acc) {
+        a.acc[2 * o] += r.re;
+        a.acc[2 * o + 1] += r.im;
+      } else {
+        a.grad[o] = mkcf((float)r.re, (float)r.im);
+        m = (float)r.s;
+        a.spsum[o] = m;
+      }
     }
     red.st(tid, m);
   }
@@ -674,7 +734,7 @@ template <class Run> PTY_HD void gather_tile(Run& run, const GatherArgs& a, int 
     run.barrier();
   }
   run.phase([&](int tid, Regs&, auto& sh) {
-    if (tid == 0) a.tilemax[tile] = sh.red().ld(0);
+    if (tid == 0 && a.tilemax) a.tilemax[tile] = sh.red().ld(0);
   });
 }
 

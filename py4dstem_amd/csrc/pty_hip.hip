@@ -99,6 +99,60 @@ __global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
   fused_position<S>(run, a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
 }
 
+// backward of the probe path, one work group per position
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_probe_grad(FusedArgs a) {
+  __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
+  __shared__ cf tw[S];
+  const int tid = threadIdx.x;
+  Fused<S>::twiddles(tid, LdsPtr{tw});
+  __syncthreads();
+  typename Fused<S>::Regs r;
+  GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, nullptr, a};
+  probe_grad_position<S>(run, a, a.b0 + (int)blockIdx.x);
+}
+
+// sum over b of complex frames [B][n], float64, in 64-slot chunks
+__global__ __launch_bounds__(256) void k_sum_frames_partial(const cf* __restrict__ frames, int B, int n,
+                                                            double* __restrict__ part) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunk = blockIdx.y;
+  if (pix >= n) return;
+  const int b1 = min(B, (chunk + 1) * 64);
+  double re = 0.0, im = 0.0;
+  for (int b = chunk * 64; b < b1; ++b) {
+    const cf v = frames[(size_t)b * n + pix];
+    re += (double)v.x;
+    im += (double)v.y;
+  }
+  part[((size_t)chunk * n + pix) * 2] = re;
+  part[((size_t)chunk * n + pix) * 2 + 1] = im;
+}
+
+__global__ __launch_bounds__(256) void k_sum_frames_finish(const double* __restrict__ part, int nchunk, int n,
+                                                           double* __restrict__ acc) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= n) return;
+  double re = 0.0, im = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    re += part[((size_t)c * n + pix) * 2];
+    im += part[((size_t)c * n + pix) * 2 + 1];
+  }
+  acc[2 * pix] += re;
+  acc[2 * pix + 1] += im;
+}
+
+__global__ __launch_bounds__(256) void k_fill_f64(double* __restrict__ x, long long n, double v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_acc_to_c64(const double* __restrict__ acc, long long n, double scale,
+                                                    cf* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = mkcf((float)(scale * acc[2 * i]), (float)(scale * acc[2 * i + 1]));
+}
+
 // ---------------------------------------------------------------------------
 // standalone 2-D FFT of one frame per work group
 //   forward: natural in -> natural (or kernel-layout) out
@@ -738,6 +792,117 @@ int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream) {
   if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (!probe) return fail(PTY_E_ARG, "probe is NULL");
   return launch_probe_com((cf*)probe, S, st(stream));
+}
+
+// ---- loss and its gradients (autograd backward) ----
+namespace {
+struct LossLayout {
+  size_t phat, ps, chi, pg, err, oacc, ppart, pacc, tmp, total;
+};
+LossLayout loss_layout(int S, int Px, int Py, int N, int Bc) {
+  LossLayout L{};
+  const size_t S2 = (size_t)S * S;
+  size_t o = 0;
+  L.phat = o; o = align256(o + S2 * 8);
+  L.ps = o; o = align256(o + (size_t)Bc * S2 * 8);
+  L.chi = o; o = align256(o + (size_t)Bc * S2 * 8);
+  L.pg = o; o = align256(o + (size_t)Bc * S2 * 8);
+  L.err = o; o = align256(o + (size_t)N * 8);
+  L.oacc = o; o = align256(o + (size_t)Px * Py * 16);
+  L.ppart = o; o = align256(o + (size_t)probe_nchunk(Bc) * S2 * 16);
+  L.pacc = o; o = align256(o + S2 * 16);
+  L.tmp = o; o = align256(o + S2 * 8);
+  L.total = o;
+  return L;
+}
+}  // namespace
+
+size_t pty_loss_workspace_bytes(int S, int Px, int Py, int num_slots, int chunk) {
+  if (!frame_supported(S) || num_slots <= 0 || chunk <= 0) return 0;
+  return loss_layout(S, Px, Py, num_slots, chunk < num_slots ? chunk : num_slots).total;
+}
+
+int pty_loss_gradients(const void* object, const void* probe, int Px, int Py, const float* positions, int N,
+                       const float* amplitudes, const int* order, int num_slots, int chunk, int S, double scale,
+                       double* loss,
+                       void* grad_object, void* grad_probe, void* workspace, size_t workspace_bytes,
+                       pty_stream_t stream_) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (Px < S || Py < S || N <= 0 || num_slots <= 0 || chunk <= 0)
+    return fail(PTY_E_ARG, "bad loss_gradients sizes");
+  if (!object || !probe || !positions || !amplitudes || !order || !loss || !workspace)
+    return fail(PTY_E_ARG, "null buffer in loss_gradients");
+  const int M = num_slots;
+  const int Bc = chunk < M ? chunk : M;
+  const LossLayout L = loss_layout(S, Px, Py, M, Bc);
+  if (workspace_bytes < L.total) return fail(PTY_E_WORKSPACE, "workspace %zu < %zu", workspace_bytes, L.total);
+  hipStream_t stream = st(stream_);
+  char* ws = (char*)workspace;
+  cf* phat = (cf*)(ws + L.phat);
+  cf* ps = (cf*)(ws + L.ps);
+  cf* chi = (cf*)(ws + L.chi);
+  cf* pg = (cf*)(ws + L.pg);
+  double* err = (double*)(ws + L.err);
+  double* oacc = (double*)(ws + L.oacc);
+  double* ppart = (double*)(ws + L.ppart);
+  double* pacc = (double*)(ws + L.pacc);
+  cf* tmp = (cf*)(ws + L.tmp);
+  const int S2 = S * S;
+  int rc;
+  if (grad_object) {
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)Px * Py * 2, 256)), dim3(256), 0, stream, oacc,
+                       (long long)Px * Py * 2, 0.0);
+    if ((rc = check_launch("k_fill_f64"))) return rc;
+  }
+  if (grad_probe) {
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)S2 * 2, 256)), dim3(256), 0, stream, pacc,
+                       (long long)S2 * 2, 0.0);
+    if ((rc = check_launch("k_fill_f64"))) return rc;
+  }
+  if ((rc = launch_fft((const cf*)probe, phat, 1, S, false, 0, stream))) return rc;
+  for (int start = 0; start < M; start += Bc) {
+    const int nb = (M - start) < Bc ? (M - start) : Bc;
+    FusedArgs a{};
+    a.probe_hat = phat; a.object = (const cf*)object; a.positions = positions; a.amps = amplitudes;
+    a.order = order + start; a.N = N; a.B = nb; a.Px = Px; a.Py = Py; a.b0 = 0;
+    a.ps_out = ps; a.chi_out = chi; a.err = err + start; a.fwd_out = nullptr;
+    if ((rc = launch_fused(a, S, stream))) return rc;
+    if (grad_object) {
+      GatherArgs g{};
+      g.ps = ps; g.chi = chi; g.positions = positions; g.order = order + start; g.N = N; g.B = nb; g.S = S;
+      g.Px = Px; g.Py = Py; g.tiles_y = gather_tiles_y(Py); g.grad = nullptr; g.spsum = nullptr;
+      g.tilemax = nullptr; g.acc = oacc;
+      if ((rc = launch_gather(g, Px, Py, stream))) return rc;
+    }
+    if (grad_probe) {
+      a.chi_in = chi;
+      a.pgrad_out = pg;
+      if (S == 64) hipLaunchKernelGGL(k_probe_grad<64>, dim3(nb), dim3(Frame<64>::T), 0, stream, a);
+      else hipLaunchKernelGGL(k_probe_grad<128>, dim3(nb), dim3(Frame<128>::T), 0, stream, a);
+      if ((rc = check_launch("k_probe_grad"))) return rc;
+      const int nchunk = probe_nchunk(nb);
+      hipLaunchKernelGGL(k_sum_frames_partial, dim3((S2 + 255) / 256, nchunk), dim3(256), 0, stream, pg, nb, S2,
+                         ppart);
+      if ((rc = check_launch("k_sum_frames_partial"))) return rc;
+      hipLaunchKernelGGL(k_sum_frames_finish, dim3((S2 + 255) / 256), dim3(256), 0, stream, ppart, nchunk, S2,
+                         pacc);
+      if ((rc = check_launch("k_sum_frames_finish"))) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, M, 1.0, loss);
+  if ((rc = check_launch("k_error_reduce"))) return rc;
+  const double g = -2.0 * (double)S2 * scale;
+  if (grad_object) {
+    hipLaunchKernelGGL(k_acc_to_c64, dim3(grid_for((long long)Px * Py, 256)), dim3(256), 0, stream, oacc,
+                       (long long)Px * Py, g, (cf*)grad_object);
+    if ((rc = check_launch("k_acc_to_c64"))) return rc;
+  }
+  if (grad_probe) {
+    hipLaunchKernelGGL(k_acc_to_c64, dim3(grid_for(S2, 256)), dim3(256), 0, stream, pacc, (long long)S2, g, tmp);
+    if ((rc = check_launch("k_acc_to_c64"))) return rc;
+    if ((rc = launch_fft(tmp, (cf*)grad_probe, 1, S, true, 0, stream))) return rc;
+  }
+  return PTY_OK;
 }
 
 size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params) {

@@ -148,7 +148,7 @@ void emu_reset(void) { g_violations = 0; g_first[0] = 0; }
 int emu_fused(int S, const float* probe_hat, const float* object, int Px, int Py,
               const float* positions, int N, const float* amps, const int* order, int B,
               float* ps, float* chi, double* err, float* fwd_out) {
-  FusedArgs a;
+  FusedArgs a{};
   a.probe_hat = (const cf*)probe_hat;
   a.object = (const cf*)object;
   a.positions = positions;
@@ -234,7 +234,7 @@ int emu_kidx(int S, int p) {
 
 int emu_gather(int S, const float* ps, const float* chi, const float* positions, int N,
                const int* order, int B, int Px, int Py, float* grad, float* spsum, float* tilemax) {
-  GatherArgs a;
+  GatherArgs a{};
   a.ps = (const cf*)ps; a.chi = (const cf*)chi; a.positions = positions; a.order = order;
   a.N = N; a.B = B; a.S = S; a.Px = Px; a.Py = Py;
   a.tiles_y = gather_tiles_y(Py);
@@ -247,5 +247,18 @@ int emu_gather(int S, const float* ps, const float* chi, const float* positions,
 }
 
 int emu_gather_num_tiles(int Px, int Py) { return gather_num_tiles(Px, Py); }
+
+// probe-gradient pass: out[b] = fft2(conj(O_b) chi_b) conj(T_b), kernel layout
+int emu_probe_grad(int S, const float* chi, const float* object, int Px, int Py, const float* positions,
+                   int N, const int* order, int B, float* out) {
+  FusedArgs a{};
+  a.object = (const cf*)object; a.positions = positions; a.order = order;
+  a.N = N; a.B = B; a.Px = Px; a.Py = Py;
+  a.chi_in = (const cf*)chi; a.pgrad_out = (cf*)out;
+  if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) probe_grad_position<64>(run, a, b); }
+  else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < B; ++b) probe_grad_position<128>(run, a, b); }
+  else return -1;
+  return 0;
+}
 
 }  // extern "C"

@@ -134,3 +134,29 @@ def test_scatter_multi_segment_order(emu, small_case):
                    shape[0], shape[1], cptr(grad), cptr(spsum), cptr(tmax))
     assert emu.emu_violations() == 0, emu.emu_first_violation()
     np.testing.assert_array_equal(grad, po.scatter_bincount(np.conj(ps), pos[order], (S, S), shape))
+
+
+def test_probe_gradient_pass_matches_oracle(emu, small_case):
+    """Backward probe pass: fft2(conj(O_b) chi_b) conj(T_b) per position;
+    summed and inverse transformed it is the oracle's true probe adjoint."""
+    _, o = small_case
+    S = 64
+    rng = np.random.default_rng(13)
+    obj = (o._object * np.exp(1j * 0.1 * rng.standard_normal(o._object.shape))).astype(np.complex64)
+    B = 10
+    order = rng.permutation(o._num_diffraction_patterns)[:B].astype(np.int32)
+    pos = o._positions_px
+    chi = (rng.standard_normal((B, S, S)) + 1j * rng.standard_normal((B, S, S))).astype(np.complex64)
+    out = np.zeros_like(chi)
+    emu.emu_reset()
+    emu.emu_probe_grad(S, cptr(chi), cptr(obj), obj.shape[0], obj.shape[1], cptr(pos), len(pos),
+                       cptr(order), B, cptr(out))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    k = kidx(emu, S)
+    nat = np.empty_like(out)
+    nat[:, k[:, None], k[None, :]] = out
+    rows, cols = po.patch_indices(pos[order], (S, S), obj.shape)
+    frac = pos[order] - np.round(pos[order])
+    T = po.fourier_translation_operator(frac, (S, S))
+    ref = np.fft.fft2(np.conj(obj[rows, cols]) * chi) * np.conj(T)
+    assert rel(nat, ref) < 1e-6

@@ -281,3 +281,36 @@ def test_probe_com_constraint_recentres():
     # equivariant (tails wrap), so one recentring leaves a small residual
     assert abs(x1) < 0.05 and abs(y1) < 0.05
     assert np.hypot(x1, y1) < np.hypot(x0, y0) / 20
+
+
+def test_probe_gradient_true_adjoint_fp64_finite_differences(tiny):
+    """The probe gradient including the adjoint of the sub-pixel shift,
+    checked by float64 central differences; and it reduces to the
+    reference's sum_b conj(O_b) chi_b term for integer positions."""
+    _, probe, obj, pos = tiny
+    o = po.OracleSingleslicePtychography(energy=300e3, mode="c128")
+    o._region_of_interest_shape = np.array([16, 16])
+    rng = np.random.default_rng(10)
+    probe64 = probe.astype(np.complex128)
+    obj64 = obj.astype(np.complex128)
+    amps = rng.uniform(0.5, 2.0, (12, 16, 16)) * np.abs(np.fft.fft2(probe64)).max() / 4
+    _, g_obj, g_probe = o.loss_gradients(obj64, probe64, pos, amps)
+
+    def loss(pv):
+        frac = pos - np.round(pos)
+        rows, cols = po.patch_indices(pos, (16, 16), obj.shape)
+        F = np.fft.fft2(po.fft_shift(pv, frac, "c128") * obj64[rows, cols])
+        return np.sum((np.abs(F) - amps) ** 2)
+
+    h = 1e-6
+    for (i, j) in [(0, 0), (1, 15), (5, 3)]:
+        for unit, comp in ((1.0, g_probe[i, j].real), (1j, g_probe[i, j].imag)):
+            e = np.zeros_like(probe64)
+            e[i, j] = unit * h
+            fd = (loss(probe64 + e) - loss(probe64 - e)) / (2 * h)
+            assert fd == pytest.approx(comp, rel=1e-5, abs=1e-6 * np.abs(g_probe).max())
+    ipos = np.round(pos).astype(np.float32)
+    _, _, g_int = o.loss_gradients(obj64, probe64, ipos, amps)
+    sp, op, chi, _ = o.forward(obj64, probe64, ipos, amps)
+    np.testing.assert_allclose(g_int, -2 * 256 * np.sum(np.conj(op) * chi, axis=0), rtol=1e-10,
+                               atol=1e-10 * np.abs(g_int).max())
