@@ -131,6 +131,14 @@ int pty_loss_gradients(const void* object, const void* probe, int Px, int Py,
                        void* grad_object, void* grad_probe, void* workspace,
                        size_t workspace_bytes, pty_stream_t stream);
 
+/* Noise-free intensities |fft2(P_shift_b * O_patch_b)|^2 of positions 0..N-1
+ * (float32 [N][S][S], natural corner-centred order) for synthetic data; the
+ * forward model of _overlap_projection + fft2 (ptychographic_methods.py:
+ * 1482-1527, 1597-1600). */
+int pty_simulate_intensities(const void* probe_hat, const void* object, int Px, int Py,
+                             const float* positions, int N, int S, float* out,
+                             pty_stream_t stream);
+
 /* ---- whole reconstruction loop ------------------------------------------ */
 
 typedef struct pty_problem {
@@ -152,6 +160,8 @@ typedef struct pty_gd_params {
   float step_size;
   float normalization_min;
   float shrinkage_rad;
+  int stage;            /* positions per fused launch (0 = whole mini-batch); a
+                           performance knob only: results are bitwise identical */
 } pty_gd_params;
 
 size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);
@@ -162,6 +172,26 @@ size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* p
  * per-iteration error (float64, device, num_iter entries). */
 int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const int* orders,
                double* errors, void* workspace, size_t workspace_bytes, pty_stream_t stream);
+
+/* ---- many reconstructions together --------------------------------------- */
+
+/* One entry per independent reconstruction; params.num_iter is ignored. */
+typedef struct pty_multi_problem {
+  pty_problem problem;
+  pty_gd_params params;
+  const int* orders;   /* int32 [num_iter][N], device */
+  double* errors;      /* float64 [num_iter], device */
+} pty_multi_problem;
+
+/* Advances `count` reconstructions (frames 64 and/or 128, any sizes) by
+ * num_iter GD iterations.  Every stage of an iteration is one launch for all
+ * problems; with use_graph the iteration's launch sequence is captured once
+ * into a hipGraph and replayed num_iter times (returns after completion).
+ * Each problem's result is bitwise identical to its own pty_gd_run.  No
+ * reference counterpart: the reference runs reconstructions one at a time. */
+size_t pty_multi_workspace_bytes(const pty_multi_problem* problems, int count);
+int pty_multi_run(const pty_multi_problem* problems, int count, int num_iter, int use_graph,
+                  void* workspace, size_t workspace_bytes, pty_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,12 @@ class PtyGdParams(ctypes.Structure):
     _fields_ = [("num_iter", c_int), ("max_batch", c_int), ("fix_probe", c_int),
                 ("fix_probe_com", c_int), ("pure_phase_object", c_int),
                 ("step_size", c_float), ("normalization_min", c_float),
-                ("shrinkage_rad", c_float)]
+                ("shrinkage_rad", c_float), ("stage", c_int)]
+
+
+class PtyMultiProblem(ctypes.Structure):
+    _fields_ = [("problem", PtyProblem), ("params", PtyGdParams), ("orders", c_void_p),
+                ("errors", c_void_p)]
 
 
 _SIGNATURES = {
@@ -59,6 +64,10 @@ _SIGNATURES = {
     "pty_loss_gradients": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p,
                                    c_void_p, c_int, c_int, c_int, ctypes.c_double, c_void_p,
                                    c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_simulate_intensities": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
+                                         c_void_p, c_void_p]),
+    "pty_multi_workspace_bytes": (c_size_t, [c_void_p, c_int]),
+    "pty_multi_run": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p]),
     "pty_gd_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams)]),
     "pty_gd_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams), c_void_p,
                            c_void_p, c_void_p, c_size_t, c_void_p]),

@@ -224,6 +224,7 @@ struct FusedArgs {
   cf* fwd_out;             // optional [B][S][S] far field F (permuted), or null
   const cf* chi_in;        // probe-gradient pass: [B][S][S] chi (natural)
   cf* pgrad_out;           // probe-gradient pass: [B][S][S] (kernel layout)
+  float* sim_out;          // simulation pass: [B][S][S] |F|^2 (natural)
 };
 
 template <int S> struct Fused {
@@ -273,7 +274,7 @@ template <int S> struct Fused {
   // P0: anchor of slot b and the ramp table (threads tid < 2S)
   template <class L> static PTY_HD void p_setup(int tid, const FusedArgs& a, int b, Regs& r, L lds,
                                                 bool conj_ramp = false) {
-    int idx = a.order[b];
+    int idx = a.order ? a.order[b] : b;  // no order: slot b is position b
     PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;  // memory-safe on bad input
     const float px = a.positions[2 * idx], py = a.positions[2 * idx + 1];
@@ -361,11 +362,11 @@ template <int S> struct Fused {
                                                         L lds, const cf* tw) {
     p_colAinv_to_regs(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
-    cf* ps_b = a.ps_out + (size_t)b * S * S;
+    cf* ps_b = a.ps_out ? a.ps_out + (size_t)b * S * S : nullptr;
 #pragma unroll
     for (int m = 0; m < N1; ++m) {
       const int i = n2 + N2 * m;
-      ps_b[i * S + cc] = r.v[m];
+      if (ps_b) ps_b[i * S + cc] = r.v[m];
       r.v[m] = cmul(r.v[m], a.object[object_offset(a, r, i, cc)]);
     }
     P::template A<false>(n2, r.v, tw);
@@ -442,6 +443,18 @@ template <int S> struct Fused {
     }
   }
 
+  // simulation: row pass B completes F = fft2(psi); |F|^2 to natural order
+  template <class L> static PTY_HD void p_sim_out(int tid, const FusedArgs& a, int b, Regs& r, L lds,
+                                                  const cf* tw) {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
+    P::template B<false>(t, r.v, tw);
+    float* out = a.sim_out + (size_t)b * S * S + F::kidx(rr) * S;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) out[F::kidx(N1 * t + j)] = cabs2(r.v[j]);
+  }
+
   // P11: chi (natural order) out
   template <class L> static PTY_HD void p_outputs(int tid, const FusedArgs& a, int b, Regs& r,
                                                   L lds, const cf* tw) {
@@ -486,6 +499,28 @@ template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs
   run.reduce_err(b);  // ends with a barrier
 }
 
+// Forward model only: intensities |fft2(P_shift * O_patch)|^2 of slot b.
+template <int S, class Run> PTY_HD void simulate_position(Run& run, const FusedArgs& a, int b) {
+  using K = Fused<S>;
+  using Regs = typename K::Regs;
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_exitwave_colA(tid, a, b, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_sim_out(tid, a, b, r, lds, tw); });
+  run.barrier();
+}
+
 // Backward of the probe path for one position:
 //   pgrad_out[b] = fft2(conj(O_patch) * chi) * conj(T_b)   (kernel layout)
 template <int S, class Run> PTY_HD void probe_grad_position(Run& run, const FusedArgs& a, int b) {
@@ -528,7 +563,11 @@ struct GatherArgs {
   cf* grad;                // [Px][Py] float32 of the float64 sums
   float* spsum;            // [Px][Py]
   float* tilemax;          // [num_tiles] max of spsum over the tile
-  double* acc;             // optional [Px][Py][2]: add the float64 sums here instead
+  double* acc;             // optional [Px][Py][2]: running float64 sums (loss gradient)
+  double* run;             // optional [Px][Py][3]: running float64 sums carried across
+                           // stages of one mini-batch (re, im, |P|^2)
+  int run_first;           // this stage starts the running sums at zero
+  int run_last;            // this stage writes grad/spsum/tilemax from the sums
 };
 
 struct GatherHit {
@@ -582,6 +621,18 @@ struct Gather {
     const int dr = wrapi(R0 - x0 + S / 2, a.Px);
     const int dc = wrapi(C0 - y0 + S / 2, a.Py);
     return (dr < S || dr + tr > a.Px) && (dc < S || dc + tc > a.Py);
+  }
+
+  // running sums: zero, or carried over from the previous stage of the batch
+  static PTY_HD void p_init(int tid, const GatherArgs& a, int tile, Regs& r) {
+    r.re = 0.0; r.im = 0.0; r.s = 0.0;
+    if (!a.run || a.run_first) return;
+    int R0, C0;
+    origin(a, tile, R0, C0);
+    const int row = R0 + tid / GT_C, col = C0 + tid % GT_C;
+    if (row >= a.Px || col >= a.Py) return;
+    const double* q = a.run + ((size_t)row * a.Py + col) * 3;
+    r.re = q[0]; r.im = q[1]; r.s = q[2];
   }
 
   // phase 1: count hits among this thread's candidates of the segment
@@ -692,6 +743,9 @@ struct Gather {
       if (a.acc) {
         a.acc[2 * o] += r.re;
         a.acc[2 * o + 1] += r.im;
+      } else if (a.run && !a.run_last) {
+        double* q = a.run + o * 3;
+        q[0] = r.re; q[1] = r.im; q[2] = r.s;
       } else {
         a.grad[o] = mkcf((float)r.re, (float)r.im);
         m = (float)r.s;
@@ -706,7 +760,7 @@ struct Gather {
 template <class Run> PTY_HD void gather_tile(Run& run, const GatherArgs& a, int tile) {
   using G = Gather;
   using Regs = G::Regs;
-  run.phase([&](int tid, Regs& r, auto& sh) { r.re = 0.0; r.im = 0.0; r.s = 0.0; (void)tid; (void)sh; });
+  run.phase([&](int tid, Regs& r, auto& sh) { G::p_init(tid, a, tile, r); (void)sh; });
   for (int seg0 = 0; seg0 < a.B; seg0 += GATHER_SEG) {
     run.phase([&](int tid, Regs& r, auto& sh) { G::p_count(tid, a, tile, seg0, r, sh.scan(0)); });
     run.barrier();
@@ -734,7 +788,7 @@ template <class Run> PTY_HD void gather_tile(Run& run, const GatherArgs& a, int 
     run.barrier();
   }
   run.phase([&](int tid, Regs&, auto& sh) {
-    if (tid == 0 && a.tilemax) a.tilemax[tile] = sh.red().ld(0);
+    if (tid == 0 && a.tilemax && (!a.run || a.run_last)) a.tilemax[tile] = sh.red().ld(0);
   });
 }
 

@@ -45,7 +45,7 @@ int check_launch(const char* what) {
   return PTY_OK;
 }
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+__host__ __device__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 bool aligned8(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 7u) == 0; }
 
@@ -86,8 +86,7 @@ template <int S> struct GpuRunner {
   }
 };
 
-template <int S>
-__global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
+template <int S> __device__ __forceinline__ void fused_body(const FusedArgs& a, int b) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
   __shared__ cf tw[S];
   __shared__ double red[Frame<S>::T / 64];
@@ -96,7 +95,12 @@ __global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
   __syncthreads();
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
-  fused_position<S>(run, a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
+  fused_position<S>(run, a, b);
+}
+
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
+  fused_body<S>(a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
 }
 
 // backward of the probe path, one work group per position
@@ -110,6 +114,19 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_grad(FusedArgs a) {
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, nullptr, a};
   probe_grad_position<S>(run, a, a.b0 + (int)blockIdx.x);
+}
+
+// forward model only (synthetic data), one work group per position
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_simulate(FusedArgs a) {
+  __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
+  __shared__ cf tw[S];
+  const int tid = threadIdx.x;
+  Fused<S>::twiddles(tid, LdsPtr{tw});
+  __syncthreads();
+  typename Fused<S>::Regs r;
+  GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, nullptr, a};
+  simulate_position<S>(run, a, a.b0 + (int)blockIdx.x);
 }
 
 // sum over b of complex frames [B][n], float64, in 64-slot chunks
@@ -159,8 +176,8 @@ __global__ __launch_bounds__(256) void k_acc_to_c64(const double* __restrict__ a
 //   inverse: natural (or kernel-layout) in -> natural out, scaled 1/S^2
 // ---------------------------------------------------------------------------
 template <int S, bool INV>
-__global__ __launch_bounds__(Frame<S>::T) void k_fft2(const cf* __restrict__ in, cf* __restrict__ out,
-                                                      int natural_io) {
+__device__ __forceinline__ void fft2_body(const cf* __restrict__ in, cf* __restrict__ out, int natural_io,
+                                          int blk) {
   using F = Frame<S>;
   using K = Fused<S>;
   __shared__ cf lds_mem[F::LDS_WORDS];
@@ -168,8 +185,8 @@ __global__ __launch_bounds__(Frame<S>::T) void k_fft2(const cf* __restrict__ in,
   const int tid = threadIdx.x;
   K::twiddles(tid, LdsPtr{tw});
   __syncthreads();
-  const cf* src = in + (size_t)blockIdx.x * S * S;
-  cf* dst = out + (size_t)blockIdx.x * S * S;
+  const cf* src = in + (size_t)blk * S * S;
+  cf* dst = out + (size_t)blk * S * S;
   LdsPtr lds{lds_mem};
   typename K::Regs r;
   if (!INV) {
@@ -221,6 +238,12 @@ __global__ __launch_bounds__(Frame<S>::T) void k_fft2(const cf* __restrict__ in,
   }
 }
 
+template <int S, bool INV>
+__global__ __launch_bounds__(Frame<S>::T) void k_fft2(const cf* __restrict__ in, cf* __restrict__ out,
+                                                      int natural_io) {
+  fft2_body<S, INV>(in, out, natural_io, blockIdx.x);
+}
+
 template <int S>
 __global__ void k_permute(const float* __restrict__ in, float* __restrict__ out, long long total,
                           int to_kernel) {
@@ -251,12 +274,14 @@ struct GatherGpuRunner {
   __device__ __forceinline__ void barrier() { __syncthreads(); }
 };
 
-__global__ __launch_bounds__(GATHER_THREADS) void k_gather(GatherArgs a) {
+__device__ __forceinline__ void gather_body(const GatherArgs& a, int tile) {
   __shared__ GatherShared sh;
   Gather::Regs r;
   GatherGpuRunner run{r, GatherSharedView{&sh}, (int)threadIdx.x};
-  gather_tile(run, a, (int)blockIdx.x);
+  gather_tile(run, a, tile);
 }
+
+__global__ __launch_bounds__(GATHER_THREADS) void k_gather(GatherArgs a) { gather_body(a, (int)blockIdx.x); }
 
 __device__ float block_max(float v, float* sh) {
   sh[threadIdx.x] = v;
@@ -285,18 +310,25 @@ __device__ __forceinline__ cf step_add(cf o, cf g, float nrm, float step) {
   return mkcf(__fadd_rn(o.x, ux), __fadd_rn(o.y, uy));
 }
 
-__global__ __launch_bounds__(256) void k_object_update(cf* __restrict__ obj, const cf* __restrict__ grad,
-                                                       const float* __restrict__ spsum,
-                                                       const float* __restrict__ tilemax, int ntiles,
-                                                       long long npix, float step, float c1, float c2) {
+__device__ __forceinline__ void object_update_body(cf* __restrict__ obj, const cf* __restrict__ grad,
+                                                   const float* __restrict__ spsum,
+                                                   const float* __restrict__ tilemax, int ntiles,
+                                                   long long npix, float step, float c1, float c2, int blk,
+                                                   int nblk) {
   __shared__ float sh[256];
   float m = 0.f;
   for (int i = threadIdx.x; i < ntiles; i += blockDim.x) m = fmaxf(m, tilemax[i]);
   m = block_max(m, sh);
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
-       p += (long long)gridDim.x * blockDim.x) {
+  for (long long p = (long long)blk * blockDim.x + threadIdx.x; p < npix; p += (long long)nblk * blockDim.x) {
     obj[p] = step_add(obj[p], grad[p], overlap_norm(spsum[p], m, c1, c2), step);
   }
+}
+
+__global__ __launch_bounds__(256) void k_object_update(cf* __restrict__ obj, const cf* __restrict__ grad,
+                                                       const float* __restrict__ spsum,
+                                                       const float* __restrict__ tilemax, int ntiles,
+                                                       long long npix, float step, float c1, float c2) {
+  object_update_body(obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -313,10 +345,10 @@ struct ProbePartialArgs {
   double* part;  // [nchunk][S*S][3] (re, im, |O|^2)
 };
 
-__global__ __launch_bounds__(256) void k_probe_partial(ProbePartialArgs a) {
+__device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, int blk, int chunk) {
   const int S = a.S;
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int chunk = blockIdx.y;
+  const int pix = blk * blockDim.x + threadIdx.x;
+  if (chunk * PROBE_CHUNK >= a.B) return;
   if (pix >= S * S) return;
   const int i = pix / S, j = pix % S;
   const int fi = i < S / 2 ? i : i - S;
@@ -341,13 +373,16 @@ __global__ __launch_bounds__(256) void k_probe_partial(ProbePartialArgs a) {
   p[2] = so;
 }
 
+__global__ __launch_bounds__(256) void k_probe_partial(ProbePartialArgs a) {
+  probe_partial_body(a, blockIdx.x, blockIdx.y);
+}
+
 // per-pixel sum of the chunk partials; per-block max of sum |O|^2
-__global__ __launch_bounds__(256) void k_probe_reduce(const double* __restrict__ part, int nchunk,
-                                                      int npix, cf* __restrict__ gsum,
-                                                      float* __restrict__ osum,
-                                                      float* __restrict__ bmax) {
+__device__ __forceinline__ void probe_reduce_body(const double* __restrict__ part, int nchunk, int npix,
+                                                  cf* __restrict__ gsum, float* __restrict__ osum,
+                                                  float* __restrict__ bmax, int blk) {
   __shared__ float sh[256];
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pix = blk * blockDim.x + threadIdx.x;
   float so = 0.f;
   if (pix < npix) {
     double re = 0.0, im = 0.0, s = 0.0;
@@ -362,19 +397,32 @@ __global__ __launch_bounds__(256) void k_probe_reduce(const double* __restrict__
     osum[pix] = so;
   }
   const float m = block_max(so, sh);
-  if (threadIdx.x == 0) bmax[blockIdx.x] = m;
+  if (threadIdx.x == 0) bmax[blk] = m;
+}
+
+__global__ __launch_bounds__(256) void k_probe_reduce(const double* __restrict__ part, int nchunk,
+                                                      int npix, cf* __restrict__ gsum,
+                                                      float* __restrict__ osum,
+                                                      float* __restrict__ bmax) {
+  probe_reduce_body(part, nchunk, npix, gsum, osum, bmax, blockIdx.x);
+}
+
+__device__ __forceinline__ void probe_apply_body(cf* __restrict__ probe, const cf* __restrict__ gsum,
+                                                 const float* __restrict__ osum, const float* __restrict__ bmax,
+                                                 int nblk, int npix, float step, float c1, float c2, int blk) {
+  __shared__ float sh[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) m = fmaxf(m, bmax[i]);
+  m = block_max(m, sh);
+  const int pix = blk * blockDim.x + threadIdx.x;
+  if (pix < npix) probe[pix] = step_add(probe[pix], gsum[pix], overlap_norm(osum[pix], m, c1, c2), step);
 }
 
 __global__ __launch_bounds__(256) void k_probe_apply(cf* __restrict__ probe, const cf* __restrict__ gsum,
                                                      const float* __restrict__ osum,
                                                      const float* __restrict__ bmax, int nblk, int npix,
                                                      float step, float c1, float c2) {
-  __shared__ float sh[256];
-  float m = 0.f;
-  for (int i = threadIdx.x; i < nblk; i += blockDim.x) m = fmaxf(m, bmax[i]);
-  m = block_max(m, sh);
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix < npix) probe[pix] = step_add(probe[pix], gsum[pix], overlap_norm(osum[pix], m, c1, c2), step);
+  probe_apply_body(probe, gsum, osum, bmax, nblk, npix, step, c1, c2, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -382,14 +430,13 @@ __global__ __launch_bounds__(256) void k_probe_apply(cf* __restrict__ probe, con
 // ---------------------------------------------------------------------------
 constexpr int CONSTRAINT_BLOCKS = 256;
 
-__global__ __launch_bounds__(256) void k_constraint_partial(const cf* __restrict__ obj,
-                                                            const unsigned char* __restrict__ mask,
-                                                            long long npix, double* part) {
+__device__ __forceinline__ void constraint_partial_body(const cf* __restrict__ obj,
+                                                        const unsigned char* __restrict__ mask, long long npix,
+                                                        double* part, int blk, int nblk) {
   __shared__ double ssum[256];
   __shared__ double scnt[256];
   double s = 0.0, c = 0.0;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
-       p += (long long)gridDim.x * blockDim.x) {
+  for (long long p = (long long)blk * blockDim.x + threadIdx.x; p < npix; p += (long long)nblk * blockDim.x) {
     if (mask[p]) {
       const cf o = obj[p];
       s += (double)atan2f(o.y, o.x);
@@ -407,15 +454,20 @@ __global__ __launch_bounds__(256) void k_constraint_partial(const cf* __restrict
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = ssum[0];
-    part[2 * blockIdx.x + 1] = scnt[0];
+    part[2 * blk] = ssum[0];
+    part[2 * blk + 1] = scnt[0];
   }
 }
 
-__global__ __launch_bounds__(256) void k_constraint_apply(cf* __restrict__ obj, long long npix,
-                                                          const double* __restrict__ part, int nparts,
-                                                          int use_mask, float shrinkage_rad,
-                                                          int pure_phase) {
+__global__ __launch_bounds__(256) void k_constraint_partial(const cf* __restrict__ obj,
+                                                            const unsigned char* __restrict__ mask,
+                                                            long long npix, double* part) {
+  constraint_partial_body(obj, mask, npix, part, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void constraint_apply_body(cf* __restrict__ obj, long long npix,
+                                                      const double* __restrict__ part, int nparts, int use_mask,
+                                                      float shrinkage_rad, int pure_phase, int blk, int nblk) {
   __shared__ float shrink_sh;
   if (threadIdx.x == 0) {
     float shrink = shrinkage_rad;
@@ -432,8 +484,7 @@ __global__ __launch_bounds__(256) void k_constraint_apply(cf* __restrict__ obj, 
   __syncthreads();
   const float shrink = shrink_sh;
   const bool do_shrink = use_mask || shrinkage_rad > 0.f;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
-       p += (long long)gridDim.x * blockDim.x) {
+  for (long long p = (long long)blk * blockDim.x + threadIdx.x; p < npix; p += (long long)nblk * blockDim.x) {
     cf o = obj[p];
     if (do_shrink) {
       const float ph = atan2f(o.y, o.x) - shrink;
@@ -450,9 +501,15 @@ __global__ __launch_bounds__(256) void k_constraint_apply(cf* __restrict__ obj, 
   }
 }
 
+__global__ __launch_bounds__(256) void k_constraint_apply(cf* __restrict__ obj, long long npix,
+                                                          const double* __restrict__ part, int nparts,
+                                                          int use_mask, float shrinkage_rad,
+                                                          int pure_phase) {
+  constraint_apply_body(obj, npix, part, nparts, use_mask, shrinkage_rad, pure_phase, blockIdx.x, gridDim.x);
+}
+
 // probe centre of mass (corner-centred coordinates) and Fourier recentring
-template <int S>
-__global__ __launch_bounds__(Frame<S>::T) void k_probe_com(cf* __restrict__ probe) {
+template <int S> __device__ __forceinline__ void probe_com_body(cf* __restrict__ probe) {
   using F = Frame<S>;
   using K = Fused<S>;
   constexpr int T = F::T;
@@ -528,9 +585,14 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_com(cf* __restrict__ prob
   for (int m = 0; m < F::N1; ++m) probe[(n2 + F::N2 * m) * S + cc] = r.v[m];
 }
 
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_probe_com(cf* __restrict__ probe) {
+  probe_com_body<S>(probe);
+}
+
 // per-iteration error: fixed-order float64 sum, normalised
-__global__ __launch_bounds__(1024) void k_error_reduce(const double* __restrict__ err, int n,
-                                                       double norm, double* __restrict__ out) {
+__device__ __forceinline__ void error_reduce_body(const double* __restrict__ err, int n, double norm,
+                                                  double* __restrict__ out) {
   __shared__ double sh[1024];
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) s += err[i];
@@ -541,6 +603,11 @@ __global__ __launch_bounds__(1024) void k_error_reduce(const double* __restrict_
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = sh[0] / norm;
+}
+
+__global__ __launch_bounds__(1024) void k_error_reduce(const double* __restrict__ err, int n,
+                                                       double norm, double* __restrict__ out) {
+  error_reduce_body(err, n, norm, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -589,7 +656,7 @@ int launch_object_update(cf* obj, const cf* grad, const float* spsum, const floa
   return check_launch("k_object_update");
 }
 
-int probe_nchunk(int B) { return (B + PROBE_CHUNK - 1) / PROBE_CHUNK; }
+__host__ __device__ inline int probe_nchunk(int B) { return (B + PROBE_CHUNK - 1) / PROBE_CHUNK; }
 
 // partials [nchunk][S*S][3] doubles, then gsum [S*S] cf, osum [S*S] f32, bmax
 size_t probe_ws_bytes(int S, int B) {
@@ -657,13 +724,21 @@ std::vector<int> subdivide(int n, int max_batch) {
 }
 
 struct WsLayout {
-  size_t phat, ps, chi, err, grad, spsum, tilemax, ppart, cpart, total;
+  size_t phat, ps, chi, err, grad, spsum, tilemax, ppart, cpart, run, total;
 };
+
+// positions per fused launch: a multiple of 64 (probe-partial chunk) or the
+// whole mini-batch
+int stage_size(const pty_gd_params& q) {
+  if (q.stage <= 0 || q.stage >= q.max_batch) return q.max_batch;
+  const int s = ((q.stage + 63) / 64) * 64;
+  return s < q.max_batch ? s : q.max_batch;
+}
 
 WsLayout layout(const pty_problem& p, const pty_gd_params& q) {
   WsLayout L{};
   const size_t S2 = (size_t)p.S * p.S;
-  const size_t B = (size_t)q.max_batch;
+  const size_t B = (size_t)stage_size(q);
   const size_t npix = (size_t)p.Px * p.Py;
   size_t o = 0;
   L.phat = o; o = align256(o + S2 * 8);
@@ -675,6 +750,7 @@ WsLayout layout(const pty_problem& p, const pty_gd_params& q) {
   L.tilemax = o; o = align256(o + (size_t)num_tiles(p.Px, p.Py) * 4);
   L.ppart = o; o = align256(o + (q.fix_probe ? 0 : probe_ws_bytes(p.S, q.max_batch)));
   L.cpart = o; o = align256(o + (size_t)CONSTRAINT_BLOCKS * 16);
+  L.run = o; o = align256(o + ((int)B < q.max_batch ? npix * 24 : 0));
   L.total = o;
   return L;
 }
@@ -690,6 +766,8 @@ int validate_problem(const pty_problem* p) {
   if (!aligned8(p->object) || !aligned8(p->probe)) return fail(PTY_E_ARG, "complex buffers must be 8-byte aligned");
   return PTY_OK;
 }
+
+#include "pty_multi.inc"
 
 }  // namespace
 
@@ -905,6 +983,20 @@ int pty_loss_gradients(const void* object, const void* probe, int Px, int Py, co
   return PTY_OK;
 }
 
+int pty_simulate_intensities(const void* probe_hat, const void* object, int Px, int Py, const float* positions,
+                             int N, int S, float* out, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (Px < S || Py < S || N < 0) return fail(PTY_E_ARG, "bad simulate sizes");
+  if (!probe_hat || !object || !positions || !out) return fail(PTY_E_ARG, "null buffer in simulate");
+  if (N == 0) return PTY_OK;
+  FusedArgs a{};
+  a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions;
+  a.order = nullptr; a.N = N; a.B = N; a.Px = Px; a.Py = Py; a.b0 = 0; a.sim_out = out;
+  if (S == 64) hipLaunchKernelGGL(k_simulate<64>, dim3(N), dim3(Frame<64>::T), 0, st(stream), a);
+  else hipLaunchKernelGGL(k_simulate<128>, dim3(N), dim3(Frame<128>::T), 0, st(stream), a);
+  return check_launch("k_simulate");
+}
+
 size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params) {
   if (!problem || !params || params->max_batch <= 0) return 0;
   return layout(*problem, *params).total;
@@ -934,6 +1026,8 @@ int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const in
   float* tilemax = (float*)(ws + L.tilemax);
   double* ppart = (double*)(ws + L.ppart);
   double* cpart = (double*)(ws + L.cpart);
+  double* runbuf = (double*)(ws + L.run);
+  const int stage = stage_size(q);
   cf* obj = (cf*)p.object;
   cf* probe = (cf*)p.probe;
   const std::vector<int> batches = subdivide(p.N, q.max_batch);
@@ -949,22 +1043,31 @@ int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const in
         if ((rc = launch_fft(probe, phat, 1, p.S, false, 0, stream))) return rc;
         phat_valid = q.fix_probe != 0;
       }
-      FusedArgs a{};
-      a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes; a.order = ob;
-      a.N = p.N; a.B = nb; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
-      a.ps_out = ps; a.chi_out = chi; a.err = err + start; a.fwd_out = nullptr;
-      if ((rc = launch_fused(a, p.S, stream))) return rc;
-      if (!q.fix_probe) {
-        ProbePartialArgs pa{chi, obj, p.positions, ob, p.N, nb, p.S, p.Px, p.Py, ppart};
-        hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(nb)), dim3(256), 0, stream,
-                           pa);
-        if ((rc = check_launch("k_probe_partial"))) return rc;
+      for (int s0 = 0; s0 < nb; s0 += stage) {
+        const int ns = (nb - s0) < stage ? (nb - s0) : stage;
+        FusedArgs a{};
+        a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
+        a.order = ob + s0; a.N = p.N; a.B = ns; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
+        a.ps_out = ps; a.chi_out = chi; a.err = err + start + s0; a.fwd_out = nullptr;
+        if ((rc = launch_fused(a, p.S, stream))) return rc;
+        if (!q.fix_probe) {
+          ProbePartialArgs pa{chi, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
+                              ppart + (size_t)(s0 / PROBE_CHUNK) * p.S * p.S * 3};
+          hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(ns)), dim3(256), 0,
+                             stream, pa);
+          if ((rc = check_launch("k_probe_partial"))) return rc;
+        }
+        GatherArgs g{};
+        g.ps = ps; g.chi = chi; g.positions = p.positions; g.order = ob + s0; g.N = p.N; g.B = ns; g.S = p.S;
+        g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
+        g.tilemax = tilemax;
+        if (ns < nb) {
+          g.run = runbuf;
+          g.run_first = s0 == 0;
+          g.run_last = s0 + ns == nb;
+        }
+        if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
       }
-      GatherArgs g{};
-      g.ps = ps; g.chi = chi; g.positions = p.positions; g.order = ob; g.N = p.N; g.B = nb; g.S = p.S;
-      g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
-      g.tilemax = tilemax;
-      if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
       if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
                                      q.normalization_min, stream)))
         return rc;
@@ -983,6 +1086,118 @@ int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const in
     }
   }
   return PTY_OK;
+}
+
+}  // extern "C"
+
+// ---- many reconstructions in one launch sequence per iteration -------------
+extern "C" {
+
+size_t pty_multi_workspace_bytes(const pty_multi_problem* problems, int count) {
+  MultiPlan plan;
+  if (!problems || count <= 0 || multi_make_plan(problems, count, plan)) return 0;
+  return multi_layout(problems, plan).total;
+}
+
+int pty_multi_run(const pty_multi_problem* problems, int count, int num_iter, int use_graph, void* workspace,
+                  size_t workspace_bytes, pty_stream_t stream_) {
+  if (!problems || count <= 0 || num_iter < 0 || !workspace) return fail(PTY_E_ARG, "bad multi-run arguments");
+  MultiPlan plan;
+  int rc = multi_make_plan(problems, count, plan);
+  if (rc) return rc;
+  const MultiLayout L = multi_layout(problems, plan);
+  if (workspace_bytes < L.total) return fail(PTY_E_WORKSPACE, "workspace %zu < %zu", workspace_bytes, L.total);
+  hipStream_t stream = st(stream_);
+  char* ws = (char*)workspace;
+  // descriptors and batch tables
+  std::vector<ProblemDev> host(count);
+  std::vector<int> rounds;
+  std::vector<size_t> rstart(count);
+  for (int i = 0; i < count; ++i) {
+    rstart[i] = rounds.size();
+    rounds.insert(rounds.end(), plan.bstart[i].begin(), plan.bstart[i].end());
+    rounds.insert(rounds.end(), plan.bsize[i].begin(), plan.bsize[i].end());
+  }
+  int* dRounds = (int*)(ws + L.rounds);
+  for (int i = 0; i < count; ++i) {
+    const pty_problem& p = problems[i].problem;
+    const pty_gd_params& q = problems[i].params;
+    ProblemDev& d = host[i];
+    d.S = p.S; d.Px = p.Px; d.Py = p.Py; d.N = p.N; d.nbatch = (int)plan.bsize[i].size(); d.stage = plan.stage[i];
+    d.fix_probe = q.fix_probe; d.fix_probe_com = q.fix_probe_com; d.pure_phase = q.pure_phase_object;
+    d.use_mask = p.fov_mask_inverse != nullptr;
+    d.step = q.step_size; d.c1 = (float)(1.0 - (double)q.normalization_min); d.c2 = q.normalization_min;
+    d.shrink = q.shrinkage_rad;
+    d.norm = (double)(float)(p.mean_intensity * (float)p.N);
+    d.object = (cf*)p.object; d.probe = (cf*)p.probe; d.phat = (cf*)(ws + L.phat[i]);
+    d.positions = p.positions; d.amps = p.amplitudes; d.mask = p.fov_mask_inverse;
+    d.orders = problems[i].orders; d.errors = problems[i].errors;
+    d.bstart = dRounds + rstart[i];
+    d.bsize = dRounds + rstart[i] + plan.bsize[i].size();
+    d.ps = (cf*)(ws + L.ps[i]); d.chi = (cf*)(ws + L.chi[i]); d.err = (double*)(ws + L.err[i]);
+    d.grad = (cf*)(ws + L.grad[i]); d.spsum = (float*)(ws + L.spsum[i]); d.tilemax = (float*)(ws + L.tilemax[i]);
+    d.run = (double*)(ws + L.run[i]); d.ppart = (double*)(ws + L.ppart[i]); d.cpart = (double*)(ws + L.cpart[i]);
+  }
+  ProblemDev* dP = (ProblemDev*)(ws + L.desc);
+  int* dIter = (int*)(ws + L.iter);
+  int* dPl64 = (int*)(ws + L.plist64);
+  int* dPl128 = (int*)(ws + L.plist128);
+  hipError_t e = hipMemcpyAsync(dP, host.data(), sizeof(ProblemDev) * count, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && !rounds.empty())
+    e = hipMemcpyAsync(dRounds, rounds.data(), sizeof(int) * rounds.size(), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && !plan.plist[0].empty())
+    e = hipMemcpyAsync(dPl64, plan.plist[0].data(), sizeof(int) * plan.plist[0].size(), hipMemcpyHostToDevice,
+                       stream);
+  if (e == hipSuccess && !plan.plist[1].empty())
+    e = hipMemcpyAsync(dPl128, plan.plist[1].data(), sizeof(int) * plan.plist[1].size(), hipMemcpyHostToDevice,
+                       stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);  // host tables are released on return
+  if (e != hipSuccess) return fail(PTY_E_LAUNCH, "descriptor upload: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(km_iter_set, dim3(1), dim3(64), 0, stream, dIter, 0);
+  if ((rc = check_launch("km_iter_set"))) return rc;
+  if (!use_graph) {
+    for (int it = 0; it < num_iter; ++it)
+      if ((rc = multi_enqueue_iteration(plan, dP, dIter, dPl64, dPl128, stream))) return rc;
+    return PTY_OK;
+  }
+  // one iteration captured once, replayed num_iter times on a private stream
+  hipStream_t cs = nullptr;
+  hipEvent_t ready = nullptr, done = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(ready, stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(cs, ready, 0);
+  if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = multi_enqueue_iteration(plan, dP, dIter, dPl64, dPl128, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &graph);
+    if (!rc && e2 != hipSuccess) rc = fail(PTY_E_LAUNCH, "graph capture: %s", hipGetErrorString(e2));
+    if (!rc) {
+      e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e2 != hipSuccess) rc = fail(PTY_E_LAUNCH, "graph instantiate: %s", hipGetErrorString(e2));
+    }
+    for (int it = 0; !rc && it < num_iter; ++it) {
+      e2 = hipGraphLaunch(exec, cs);
+      if (e2 != hipSuccess) rc = fail(PTY_E_LAUNCH, "graph launch: %s", hipGetErrorString(e2));
+    }
+    if (!rc) {
+      e2 = hipEventRecord(done, cs);
+      if (e2 == hipSuccess) e2 = hipStreamWaitEvent(stream, done, 0);
+      if (e2 != hipSuccess) rc = fail(PTY_E_LAUNCH, "graph join: %s", hipGetErrorString(e2));
+    }
+    hipStreamSynchronize(cs);
+  } else {
+    rc = fail(PTY_E_LAUNCH, "graph setup: %s", hipGetErrorString(e));
+  }
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  if (ready) hipEventDestroy(ready);
+  if (done) hipEventDestroy(done);
+  if (cs) hipStreamDestroy(cs);
+  return rc;
 }
 
 }  // extern "C"

@@ -1,0 +1,123 @@
+"""Many independent reconstructions advanced together on one GPU.
+
+``MultiPtychography`` takes preprocessed ``SingleslicePtychography`` objects
+(any mix of 64 and 128 pixel frames, any scan and object sizes) and runs
+their GD iterations with ``pty_multi_run``: one launch per stage for all
+problems and, by default, one captured hipGraph replayed per iteration.
+Each problem's batch order is generated exactly as its own ``reconstruct()``
+would, and its result is bitwise identical to that sequential solve.
+
+The reference reconstructs one dataset at a time
+(py4DSTEM/process/phase/singleslice_ptychography.py:606-996); this runner has
+no reference counterpart.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import autotune
+from ._lib import PtyMultiProblem, stream_handle
+
+
+class MultiPtychography:
+    def __init__(self, reconstructions):
+        self.items = list(reconstructions)
+        if not self.items:
+            raise ValueError("no reconstructions given")
+        for p in self.items:
+            if not getattr(p, "_preprocessed", False):
+                raise ValueError("every reconstruction must be preprocessed")
+
+    def _runs(self, num_iter, kwargs):
+        runs = []
+        for p, kw in zip(self.items, kwargs):
+            kw = dict(kw, num_iter=num_iter, progress_bar=False, store_iterations=False)
+            runs.append(p._prepare_run(_allocate=False, **kw))
+        return runs
+
+    def reconstruct(self, num_iter, reconstruct_kwargs=None, use_graph=True):
+        """num_iter GD iterations of every problem.  reconstruct_kwargs: one
+        dict (shared) or a list of dicts with reconstruct() keywords."""
+        if reconstruct_kwargs is None:
+            reconstruct_kwargs = {}
+        if isinstance(reconstruct_kwargs, dict):
+            reconstruct_kwargs = [reconstruct_kwargs] * len(self.items)
+        runs = self._runs(num_iter, reconstruct_kwargs)
+        self.run_prepared(runs, num_iter, use_graph)
+        for p, run in zip(self.items, runs):
+            p._finish_run(run)
+        return self
+
+    @staticmethod
+    def run_prepared(runs, num_iter, use_graph=True):
+        P = len(runs)
+        arr = (PtyMultiProblem * P)()
+        for i, run in enumerate(runs):
+            arr[i].problem = run["problem"]
+            arr[i].params = run["params"]
+            arr[i].orders = ctypes.c_void_p(run["orders"].data_ptr())
+            arr[i].errors = ctypes.c_void_p(run["errors"].data_ptr())
+        lib = _lib.load()
+        nbytes = lib.pty_multi_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), P)
+        if nbytes == 0:
+            raise RuntimeError(f"pty_multi_workspace_bytes failed: {_lib.last_error()}")
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=runs[0]["orders"].device)
+        _lib.call("pty_multi_run", ctypes.cast(arr, ctypes.c_void_p), P, int(num_iter),
+                  int(bool(use_graph)), ctypes.c_void_p(ws.data_ptr()), ws.numel(), stream_handle())
+
+    # -- autotuning ------------------------------------------------------------
+    def signature(self):
+        return json.dumps(sorted((int(p._region_of_interest_shape[0]), int(p._num_diffraction_patterns),
+                                  tuple(int(v) for v in p._object_shape)) for p in self.items))
+
+    def tune(self, batch_candidates=(256, 512, 1024), stage_candidates=(0, 256), iters=2,
+             reconstruct_kwargs=None, use_cache=True):
+        """Pick the mini-batch size (when the caller leaves it free) and the
+        stage size by timing `iters` iterations of every candidate on copies
+        of the problems' current state.  Returns (batch, stage, timings)."""
+        key = "multi:" + self.signature()
+        if use_cache:
+            hit = autotune.lookup(key)
+            if hit:
+                self._apply(hit["batch"], hit["stage"])
+                return hit["batch"], hit["stage"], hit.get("timings", {})
+        kw = dict(reconstruct_kwargs or {})
+        saved = [(p._object_dev.clone() if hasattr(p, "_object_dev") else None,
+                  p._probe_dev.clone() if hasattr(p, "_probe_dev") else None,
+                  list(getattr(p, "error_iterations", [])), hasattr(p, "error")) for p in self.items]
+
+        def trial(cand):
+            batch, stage = cand
+            self._apply(batch, stage)
+            runs = self._runs(iters, [dict(kw, max_batch_size=batch, reset=True)] * len(self.items))
+            self.run_prepared(runs, iters)
+            torch.cuda.synchronize()
+
+        cands = [(b, s) for b in batch_candidates for s in stage_candidates if s == 0 or s < b]
+        best, timings = autotune.best_of(cands, trial, repeats=1)
+        for p, (o, pr, errs, had) in zip(self.items, saved):
+            if o is not None:
+                p._object_dev = o
+                p._probe_dev = pr
+            else:
+                for a in ("_object_dev", "_probe_dev"):
+                    if hasattr(p, a):
+                        delattr(p, a)
+            p.error_iterations = errs
+            if not had and hasattr(p, "error"):
+                del p.error
+        self._apply(*best)
+        tim = {f"{b}/{s}": t for (b, s), t in timings.items()}
+        autotune.remember(key, {"batch": best[0], "stage": best[1], "timings": tim})
+        return best[0], best[1], tim
+
+    def _apply(self, batch, stage):
+        self.batch = batch
+        for p in self.items:
+            p.set_stage(stage)

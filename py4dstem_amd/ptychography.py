@@ -434,6 +434,20 @@ class SingleslicePtychography:
             del ps, chi
         return overlap.cpu().numpy()
 
+    # -- execution tuning (results do not depend on it) --------------------------
+    def set_stage(self, stage):
+        """Positions per fused launch inside a mini-batch (0 = whole batch).
+        A performance setting only: the update is bitwise identical."""
+        self._stage = int(stage)
+        return self
+
+    def _stage_for(self, batch):
+        stage = getattr(self, "_stage", None)
+        if stage is None:
+            from .autotune import default_stage
+            stage = default_stage(int(self._region_of_interest_shape[0]), batch)
+        return stage
+
     # -- reconstruct ------------------------------------------------------------
     def _reset_reconstruction(self, store_iterations, reset):
         """ptychographic_methods.py:252-294."""
@@ -486,6 +500,35 @@ class SingleslicePtychography:
                     store_iterations=False, progress_bar=True, reset=None, device=None,
                     clear_fft_cache=None, object_type=None):
         """singleslice_ptychography.py:606-996 (GD).  Returns self."""
+        kwargs = {k: v for k, v in locals().items() if k != "self"}
+        run = self._prepare_run(**kwargs)
+        self._execute_run(run)
+        return self._finish_run(run)
+
+    def _prepare_run(self, num_iter=8, reconstruction_method="gradient-descent",
+                    reconstruction_parameter=1.0, reconstruction_parameter_a=None,
+                    reconstruction_parameter_b=None, reconstruction_parameter_c=None,
+                    max_batch_size=None, seed_random=None, step_size=0.5, normalization_min=1,
+                    positions_step_size=0.5, pure_phase_object=False, fix_probe_com=True,
+                    fix_probe=False, fix_probe_aperture=False, constrain_probe_amplitude=False,
+                    constrain_probe_amplitude_relative_radius=0.5,
+                    constrain_probe_amplitude_relative_width=0.05,
+                    constrain_probe_fourier_amplitude=False,
+                    constrain_probe_fourier_amplitude_max_width_pixels=3.0,
+                    constrain_probe_fourier_amplitude_constant_intensity=False, fix_positions=True,
+                    fix_positions_com=True, max_position_update_distance=None,
+                    max_position_total_distance=None, global_affine_transformation=False,
+                    gaussian_filter_sigma=None, gaussian_filter=True, fit_probe_aberrations=False,
+                    fit_probe_aberrations_max_angular_order=4,
+                    fit_probe_aberrations_max_radial_order=4,
+                    fit_probe_aberrations_remove_initial=False,
+                    fit_probe_aberrations_using_scikit_image=True, butterworth_filter=True,
+                    q_lowpass=None, q_highpass=None, butterworth_order=2, tv_denoise=True,
+                    tv_denoise_weight=None, tv_denoise_inner_iter=40, object_positivity=True,
+                    shrinkage_rad=0.0, fix_potential_baseline=True, detector_fourier_mask=None,
+                    store_iterations=False, progress_bar=True, reset=None, device=None,
+                    clear_fft_cache=None, object_type=None, _allocate=True):
+        """Validation, batch orders and device state for reconstruct()."""
         self.set_device(device, clear_fft_cache)
         if not getattr(self, "_preprocessed", False):
             raise ValueError("preprocess() must run before reconstruct()")
@@ -566,16 +609,25 @@ class SingleslicePtychography:
         per_call = 1 if (store_iterations or progress_bar) else max(num_iter, 1)
         params = PtyGdParams(per_call, B, int(bool(fix_probe)), int(bool(fix_probe_com)),
                              int(bool(pure_phase_object)), float(step_size),
-                             float(normalization_min), float(shrinkage_rad))
-        ws = torch.empty(K.gd_workspace_bytes(problem, params), dtype=torch.uint8, device=dev)
+                             float(normalization_min), float(shrinkage_rad),
+                             int(self._stage_for(B)))
+        ws = (torch.empty(K.gd_workspace_bytes(problem, params), dtype=torch.uint8, device=dev)
+              if _allocate else None)
         orders_dev = torch.from_numpy(orders).to(dev)
         errors_dev = torch.zeros(max(num_iter, 1), dtype=torch.float64, device=dev)
         self._last_run = dict(problem=problem, params=params, workspace=ws, orders=orders_dev,
-                              errors=errors_dev, mask=mask_dev)
+                              errors=errors_dev, mask=mask_dev, num_iter=num_iter,
+                              per_call=per_call, store_iterations=store_iterations,
+                              progress_bar=progress_bar)
+        return self._last_run
 
+    def _execute_run(self, run):
+        problem, params, ws = run["problem"], run["params"], run["workspace"]
+        orders_dev, errors_dev = run["orders"], run["errors"]
+        num_iter, per_call = run["num_iter"], run["per_call"]
         it = 0
         bar = None
-        if progress_bar and num_iter > 0:
+        if run["progress_bar"] and num_iter > 0:
             try:
                 from tqdm import tqdm
                 bar = tqdm(total=num_iter, desc="Reconstructing object and probe", unit=" iter")
@@ -586,7 +638,7 @@ class SingleslicePtychography:
             params.num_iter = n
             K.gd_run(problem, params, orders_dev[it:], errors_dev[it:], ws)
             it += n
-            if store_iterations:
+            if run["store_iterations"]:
                 self.object_iterations.append(self._object_dev.cpu().numpy().copy())
                 self.probe_iterations.append(
                     np.fft.fftshift(self._probe_dev.cpu().numpy(), axes=(-2, -1)))
@@ -595,7 +647,9 @@ class SingleslicePtychography:
         if bar is not None:
             bar.close()
 
-        errors = errors_dev[:num_iter].cpu().numpy()
+    def _finish_run(self, run):
+        num_iter = run["num_iter"]
+        errors = run["errors"][:num_iter].cpu().numpy()
         self.error_iterations.extend(float(e) for e in errors)
         self._object = self._object_dev.cpu().numpy()
         self._probe = self._probe_dev.cpu().numpy()

@@ -90,6 +90,41 @@ def simulate_intensities(obj, probe, positions_px, chunk=512):
     return out
 
 
+def simulate_intensities_gpu(obj, probe, positions_px, chunk=8192):
+    """Noise-free intensities (N, S, S) float32 from the HIP forward-model
+    kernel (float32 arithmetic); returns a torch tensor on the GPU."""
+    import torch
+
+    from . import _lib, kernels
+
+    dev = _lib.require_gpu()
+    S = probe.shape[0]
+    o = torch.from_numpy(np.ascontiguousarray(obj, dtype=np.complex64)).to(dev)
+    phat = kernels.probe_spectrum(torch.from_numpy(np.ascontiguousarray(probe, dtype=np.complex64)).to(dev))
+    pos = torch.from_numpy(np.ascontiguousarray(positions_px, dtype=np.float32)).to(dev)
+    N = pos.shape[0]
+    out = torch.empty((N, S, S), dtype=torch.float32, device=dev)
+    for s in range(0, N, chunk):
+        n = min(chunk, N - s)
+        _lib.call("pty_simulate_intensities", _lib.ptr(phat), _lib.ptr(o), o.shape[0], o.shape[1],
+                  _lib.ptr(pos[s:s + n]), n, S, _lib.ptr(out[s:s + n]), _lib.stream_handle())
+    return out
+
+
+def poisson_datacube_gpu(intensities, grid_shape, scan_step_ang, dk_inv_ang, seed):
+    """Poisson-sample GPU intensities, centre the patterns, return a DataCube."""
+    import torch
+
+    g = torch.Generator(device=intensities.device)
+    g.manual_seed(int(seed))
+    counts = torch.poisson(intensities.clamp_min(0), generator=g)
+    data = torch.fft.fftshift(counts, dim=(-2, -1)).cpu().numpy()
+    S = data.shape[-1]
+    cal = Calibration(R_pixel_size=scan_step_ang, R_pixel_units="A",
+                      Q_pixel_size=dk_inv_ang, Q_pixel_units="A^-1")
+    return DataCube(data.reshape(tuple(grid_shape) + (S, S)), cal)
+
+
 def to_datacube(intensities, grid_shape, scan_step_ang, dk_inv_ang, rng=None, poisson=True):
     """Poisson-sample, centre the patterns and wrap in a calibrated DataCube."""
     data = rng.poisson(intensities).astype(np.float32) if poisson else intensities.astype(np.float32)
@@ -174,6 +209,54 @@ def production_workload(seed=0, scan=(128, 128), S=128, object_px=720, poisson=T
                             step_size=0.5, normalization_min=1, seed_random=seed,
                             progress_bar=False),
     }
+
+
+def many_workloads(count=64, seed=0, scan_range=(32, 96), frames=(64, 128),
+                   defocus_range=(100.0, 400.0), num_iter=30, batch=None, on_gpu=True):
+    """Independent seeded problems for batched reconstruction: square scans
+    with edge uniform in scan_range, frames alternating between the given
+    sizes (half each), defocus uniform in defocus_range, Gaussian-atom
+    objects sized by the reconstruction's own default rule.  64 px frames use
+    0.2 A sampling and 1.0 A steps, 128 px frames 0.1 A and 0.4 A."""
+    from .preprocess import initial_object_shape
+
+    rng = np.random.default_rng(seed)
+    energy, semiangle, rolloff = 300e3, 20.0, 2.0
+    lam = electron_wavelength_angstrom(energy)
+    out = []
+    for i in range(count):
+        S = frames[i % len(frames)]
+        n = int(rng.integers(scan_range[0], scan_range[1] + 1))
+        defocus = float(rng.uniform(*defocus_range))
+        px = 0.2 if S == 64 else 0.1
+        step = 1.0 if S == 64 else 0.4
+        dk = 1.0 / (S * px)
+        sampling = real_space_sampling(energy, (dk * lam * 1e3,) * 2, (S, S))
+        pos, pad = scan_positions_in_pixels(None, (n, n), (step, step), 0.0, False, sampling, (S, S))
+        shape = initial_object_shape(pos, pad, np.array([S, S]))
+        pos = center_positions(pos.astype(np.float32), shape)
+        obj, _ = gaussian_atoms_phase_object(shape, sampling[0], 2.5, 0.3, 0.15, 0.1, rng)
+        dose = 1e5 * step * step
+        probe = scaled_probe(energy, S, sampling, semiangle, rolloff, dose, defocus=defocus)
+        if on_gpu:
+            I = simulate_intensities_gpu(obj, probe, pos)
+            dc = poisson_datacube_gpu(I, (n, n), step, dk, seed * 1000 + i)
+            del I
+        else:
+            I = simulate_intensities(obj.astype(np.complex128), probe, pos)
+            dc = to_datacube(I, (n, n), step, dk, rng, True)
+        out.append({
+            "datacube": dc, "object_true": obj,
+            "init": dict(energy=energy, semiangle_cutoff=semiangle, rolloff=rolloff,
+                         defocus=defocus, verbose=False),
+            "preprocess": dict(force_com_rotation=0, force_com_transpose=False,
+                               force_com_shifts=(S / 2, S / 2), plot_center_of_mass=False,
+                               plot_rotation=False, plot_probe_overlaps=False),
+            "reconstruct": dict(num_iter=num_iter, max_batch_size=batch, fix_probe=False,
+                                step_size=0.5, normalization_min=1, seed_random=seed * 1000 + i,
+                                progress_bar=False),
+        })
+    return out
 
 
 def object_nmse(recon, truth, mask=None):

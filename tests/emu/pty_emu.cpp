@@ -248,6 +248,38 @@ int emu_gather(int S, const float* ps, const float* chi, const float* positions,
 
 int emu_gather_num_tiles(int Px, int Py) { return gather_num_tiles(Px, Py); }
 
+// the same scatter in stages of `stage` slots with float64 sums carried over
+int emu_gather_staged(int S, const float* ps, const float* chi, const float* positions, int N,
+                      const int* order, int B, int stage, int Px, int Py, float* grad, float* spsum,
+                      float* tilemax, double* runbuf) {
+  for (int s0 = 0; s0 < B; s0 += stage) {
+    const int ns = (B - s0) < stage ? (B - s0) : stage;
+    GatherArgs a{};
+    a.ps = (const cf*)ps + (size_t)s0 * S * S; a.chi = (const cf*)chi + (size_t)s0 * S * S;
+    a.positions = positions; a.order = order + s0;
+    a.N = N; a.B = ns; a.S = S; a.Px = Px; a.Py = Py; a.tiles_y = gather_tiles_y(Py);
+    a.grad = (cf*)grad; a.spsum = spsum; a.tilemax = tilemax;
+    a.run = runbuf; a.run_first = s0 == 0; a.run_last = s0 + ns == B;
+    for (int tile = 0; tile < gather_num_tiles(Px, Py); ++tile) {
+      GatherEmu run;
+      gather_tile(run, a, tile);
+    }
+  }
+  return 0;
+}
+
+// forward model: out[b] = |fft2(P_shift_b O_b)|^2 (natural order)
+int emu_simulate(int S, const float* probe_hat, const float* object, int Px, int Py, const float* positions,
+                 int N, float* out) {
+  FusedArgs a{};
+  a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions;
+  a.N = N; a.B = N; a.Px = Px; a.Py = Py; a.sim_out = out;
+  if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < N; ++b) simulate_position<64>(run, a, b); }
+  else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < N; ++b) simulate_position<128>(run, a, b); }
+  else return -1;
+  return 0;
+}
+
 // probe-gradient pass: out[b] = fft2(conj(O_b) chi_b) conj(T_b), kernel layout
 int emu_probe_grad(int S, const float* chi, const float* object, int Px, int Py, const float* positions,
                    int N, const int* order, int B, float* out) {

@@ -160,3 +160,53 @@ def test_probe_gradient_pass_matches_oracle(emu, small_case):
     T = po.fourier_translation_operator(frac, (S, S))
     ref = np.fft.fft2(np.conj(obj[rows, cols]) * chi) * np.conj(T)
     assert rel(nat, ref) < 1e-6
+
+
+def test_staged_scatter_is_bitwise_identical(emu, small_case):
+    """A mini-batch scattered in stages with carried float64 sums equals the
+    single-pass scatter bit for bit (same per-pixel summation order)."""
+    _, o = small_case
+    S = 64
+    rng = np.random.default_rng(14)
+    B = 300
+    order = rng.permutation(o._num_diffraction_patterns)[:B].astype(np.int32)
+    pos = o._positions_px
+    ps = (rng.standard_normal((B, S, S)) + 1j * rng.standard_normal((B, S, S))).astype(np.complex64)
+    chi = (rng.standard_normal((B, S, S)) + 1j * rng.standard_normal((B, S, S))).astype(np.complex64)
+    shape = o._object.shape
+    nt = emu.emu_gather_num_tiles(*shape)
+    outs = []
+    for stage in (B, 64, 128):
+        grad = np.zeros(shape, np.complex64)
+        spsum = np.zeros(shape, np.float32)
+        tmax = np.zeros(nt, np.float32)
+        run = np.zeros(shape + (3,), np.float64)
+        emu.emu_reset()
+        emu.emu_gather_staged(S, cptr(ps), cptr(chi), cptr(pos), len(pos), cptr(order), B, stage,
+                              shape[0], shape[1], cptr(grad), cptr(spsum), cptr(tmax), cptr(run))
+        assert emu.emu_violations() == 0, emu.emu_first_violation()
+        outs.append((grad, spsum, tmax))
+    for g, s_, t in outs[1:]:
+        np.testing.assert_array_equal(g, outs[0][0])
+        np.testing.assert_array_equal(s_, outs[0][1])
+        np.testing.assert_array_equal(t, outs[0][2])
+
+
+def test_simulation_pass_matches_forward_model(emu, small_case):
+    """Forward-model pass: |fft2(P_shift_b O_b)|^2 in natural order, equal to
+    the oracle's forward model (and race-free)."""
+    _, o = small_case
+    S = 64
+    rng = np.random.default_rng(15)
+    obj = (o._object * np.exp(1j * 0.2 * rng.standard_normal(o._object.shape))).astype(np.complex64)
+    pos = o._positions_px[:6].copy()
+    k = kidx(emu, S)
+    probe = o._probe.astype(np.complex64)
+    phat = np.ascontiguousarray(np.fft.fft2(probe)[np.ix_(k, k)].astype(np.complex64))
+    out = np.zeros((6, S, S), np.float32)
+    emu.emu_reset()
+    emu.emu_simulate(S, cptr(phat), cptr(obj), obj.shape[0], obj.shape[1], cptr(pos), 6, cptr(out))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    rows, cols = po.patch_indices(pos, (S, S), obj.shape)
+    ref = np.abs(np.fft.fft2(po.fft_shift(probe, pos - np.round(pos)) * obj[rows, cols])) ** 2
+    assert rel(out, ref) < 1e-6
