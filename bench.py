@@ -15,7 +15,7 @@ all problems, one graph replay per iteration) against the same solves run one
 after another with reconstruct(); prints aggregate position-updates/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds T]
-                       [--workload production|many] [--problems P]
+                       [--workload production|many|kernels] [--problems P]
 """
 
 from __future__ import annotations
@@ -43,6 +43,20 @@ def algorithmic_bytes_per_update(S, Px, Py, B):
     """20 S^2 (amplitude read, object-patch gather, object-gradient scatter)
     + object and probe read+written once per mini-batch, amortised."""
     return 20 * S * S + 16 * (Px * Py + S * S) / B
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE / WRITE_SIZE passes scaled by a copy-kernel calibration), or
+    None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "counters_hot_kernels_scan128_frame128.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel} = "):
+                return int(line.rsplit("=", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_model():
@@ -166,6 +180,59 @@ def bench_many(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_kernels(args):
+    """Hot kernels of one production mini-batch (1024 positions, 128 px frames,
+    720x720 object) launched `--kernel-reps` times each, plus a plain copy of
+    256 MiB: the workload for hardware-counter passes.  The copy calibrates
+    FETCH_SIZE / WRITE_SIZE against a known byte count."""
+    import torch
+
+    from py4dstem_amd import kernels as K
+    from py4dstem_amd.preprocess import center_positions, real_space_sampling, scan_positions_in_pixels
+    from py4dstem_amd.probe import electron_wavelength_angstrom
+    from py4dstem_amd.synthetic import scaled_probe
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    S, P, B, n = 128, 720, 1024, 128
+    rng = np.random.default_rng(0)
+    lam = electron_wavelength_angstrom(300e3)
+    sampling = real_space_sampling(300e3, (lam * 1e3 / (S * 0.1),) * 2, (S, S))
+    pos, _ = scan_positions_in_pixels(None, (n, n), (0.4, 0.4), 0.0, False, sampling, (S, S))
+    pos = center_positions(pos.astype(np.float32), (P, P))
+    probe = scaled_probe(300e3, S, sampling, 20.0, 2.0, 1.6e5, defocus=200.0).astype(np.complex64)
+    obj = np.exp(1j * rng.uniform(0, 0.3, (P, P))).astype(np.complex64)
+    amps = rng.uniform(0, 20, (n * n, S, S)).astype(np.float32)
+    o = torch.from_numpy(obj).to(dev)
+    pr = torch.from_numpy(probe).to(dev)
+    posd = torch.from_numpy(pos).to(dev)
+    ampd = torch.from_numpy(amps).to(dev)
+    del amps
+    order = torch.from_numpy(rng.permutation(n * n)[:B].astype(np.int32)).to(dev)
+    phat = K.probe_spectrum(pr)
+    src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(args.kernel_reps):
+        dst.copy_(src)
+    for _ in range(args.kernel_reps):
+        ps, chi, err = K.forward_adjoint(phat, o, posd, ampd, order)
+    for _ in range(args.kernel_reps):
+        K.scatter(ps, chi, posd, order, (P, P))
+    for _ in range(args.kernel_reps):
+        K.probe_update(pr.clone(), chi, o, posd, order, 0.5, 1.0)
+    torch.cuda.synchronize()
+    print(json.dumps({
+        "copy_bytes": 2 * src.numel() * 4,
+        "k_fused_alg_bytes": B * (4 + 8 + 8 + 8) * S * S,
+        "k_gather_alg_bytes": B * 16 * S * S + P * P * 12,
+        "k_probe_partial_alg_bytes": B * 8 * S * S,
+        "note": "k_fused: amplitudes read, object patch read, P_shift and chi written; "
+                "k_gather: P_shift and chi read, gradient and overlap sum written; "
+                "k_probe_partial: chi read (object patch reads hit the caches)",
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,7 +241,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--scan", type=int, default=128)
-    ap.add_argument("--workload", choices=("production", "many"), default="production")
+    ap.add_argument("--workload", choices=("production", "many", "kernels"), default="production")
     ap.add_argument("--problems", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -183,6 +250,8 @@ def main():
         sys.exit(2)
     if args.workload == "many":
         return bench_many(args)
+    if args.workload == "kernels":
+        return bench_kernels(args)
 
     import torch
 
@@ -276,7 +345,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic("k_fused") if (S, B) == (128, 1024) else None,
             "kernel_avg_us": round(k_avg * 1e6, 2),
             "bytes_per_update": per_update,
         },
