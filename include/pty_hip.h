@@ -76,7 +76,7 @@ int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px,
  * called at ptychographic_methods.py:1859-1886):
  *   grad = scatter(conj(ps) * chi), spsum = scatter(|ps|^2);
  *   tilemax[t] = max spsum over object tile t, pty_scatter_num_tiles(Px, Py)
- *   entries. */
+ *   entries.  S may also be 32 (stitching of learned-initialiser patches). */
 int pty_scatter_num_tiles(int Px, int Py);
 int pty_scatter_patches(const void* ps, const void* chi, const float* positions, int N,
                         const int* order, int B, int S, int Px, int Py, void* grad, float* spsum,
@@ -192,6 +192,70 @@ typedef struct pty_multi_problem {
 size_t pty_multi_workspace_bytes(const pty_multi_problem* problems, int count);
 int pty_multi_run(const pty_multi_problem* problems, int count, int num_iter, int use_graph,
                   void* workspace, size_t workspace_bytes, pty_stream_t stream);
+
+/* ---- learned object initialiser ------------------------------------------ */
+
+/* No reference counterpart: the reference starts from a flat object or a
+ * caller-supplied guess (ptychographic_methods.py:40-79, the object the
+ * initialiser replaces).  These entry points run a 3x3 convolutional network
+ * (BF16 operands on MFMA, FP32 accumulation) that maps 3x3 scan
+ * neighbourhoods of diffraction patterns to 32 x 32 object patches.
+ *
+ * Activations are NHWC bfloat16 (raw 16-bit words) with C = 16 or 64; 16-byte
+ * aligned.  A 3x3 convolution has zero padding equal to its dilation:
+ *   in_y = out_y * stride + (ky - 1) * dil, Ho = (Hi - 1) / stride + 1.
+ * Master weights are float32 [Co][9][Ci] (tap = 3 ky + kx); the kernels read
+ * two bfloat16 copies: forward wf[Co][Kf] (k = tap Ci + ci) and backward
+ * wb[Ci][Kb] (k = tap Co + co), K rounded up to 32, padding zero. */
+typedef struct pty_conv_desc {
+  int n;               /* samples */
+  int Hi, Wi, Ci;      /* input */
+  int Ho, Wo, Co;      /* output */
+  int stride, dil;     /* stride 1 or 2, dilation 1..16 */
+} pty_conv_desc;
+
+/* 16-bit words of the forward (backward_copy = 0) or backward weight copy */
+size_t pty_conv_weight_words(const pty_conv_desc* d, int backward_copy);
+/* y = act(conv(x) + bias), act = ReLU when relu != 0; outputs bf16 and/or f32 */
+int pty_conv_forward(const pty_conv_desc* d, const unsigned short* x, const unsigned short* wf,
+                     const float* bias, int relu, unsigned short* y_bf16, float* y_f32,
+                     pty_stream_t stream);
+/* dx = conv^T(dy), times (x_act > 0) when x_act is given (ReLU of the input) */
+int pty_conv_backward_data(const pty_conv_desc* d, const unsigned short* dy,
+                           const unsigned short* wb, const unsigned short* x_act,
+                           unsigned short* dx_bf16, float* dx_f32, pty_stream_t stream);
+/* dw [Co][9][Ci] and dbias [Co] (optional), deterministic reduction */
+size_t pty_conv_backward_weight_workspace_bytes(const pty_conv_desc* d);
+int pty_conv_backward_weight(const pty_conv_desc* d, const unsigned short* x,
+                             const unsigned short* dy, float* dw, float* dbias, void* workspace,
+                             size_t workspace_bytes, pty_stream_t stream);
+/* one Adam step (bias-corrected, step >= 1) on weights and biases; rewrites wf, wb */
+int pty_conv_adam(const pty_conv_desc* d, float lr, float beta1, float beta2, float eps, int step,
+                  float* w, float* m, float* v, const float* grad, float* bias, float* bias_m,
+                  float* bias_v, const float* bias_grad, unsigned short* wf, unsigned short* wb,
+                  pty_stream_t stream);
+/* wf, wb from master weights */
+int pty_conv_pack_weights(const pty_conv_desc* d, const float* w, unsigned short* wf,
+                          unsigned short* wb, pty_stream_t stream);
+/* Network input [count][64][64][16]: channel j < 9 is frame neighbours[s][j]
+ * binned S/64 x S/64 from the centred pattern, log1p(counts) minus the
+ * sample mean; channels 9..15 zero.  frames: counts, natural corner-centred
+ * order (kernel_layout = 0) or amplitudes in the kernel layout (counts = A^2,
+ * kernel_layout = 1).  S = 64 or 128. */
+int pty_net_inputs(const float* frames, int N, int S, int kernel_layout, const int* neighbours,
+                   int count, unsigned short* x, pty_stream_t stream);
+/* loss = mean over [count][32][32][2] of (y - target)^2, y = channels 0 (amplitude)
+ * and 1 (phase) of the last layer's f32 output [count][32][32][16];
+ * dy (bf16, 16 channels) = d loss / d y */
+size_t pty_net_loss_workspace_bytes(int count);
+int pty_net_loss(const float* y, const float* target, int count, unsigned short* dy,
+                 double* loss, void* workspace, size_t workspace_bytes, pty_stream_t stream);
+/* stitching operands for pty_scatter_patches with S = 32: ps = sqrt(w),
+ * chi = sqrt(w) a e^{i phi}, corner-centred, w = sin^2 taper */
+int pty_net_patches(const float* y, int count, void* ps, void* chi, pty_stream_t stream);
+/* object = weighted_sum / weight_sum where weight_sum > floor_weight, else 1 */
+int pty_net_finish(const void* weighted_sum, const float* weight_sum, int Px, int Py,
+                   float floor_weight, void* object, pty_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,11 @@ class PtyGdParams(ctypes.Structure):
                 ("shrinkage_rad", c_float), ("stage", c_int)]
 
 
+class PtyConvDesc(ctypes.Structure):
+    _fields_ = [("n", c_int), ("Hi", c_int), ("Wi", c_int), ("Ci", c_int), ("Ho", c_int),
+                ("Wo", c_int), ("Co", c_int), ("stride", c_int), ("dil", c_int)]
+
+
 class PtyMultiProblem(ctypes.Structure):
     _fields_ = [("problem", PtyProblem), ("params", PtyGdParams), ("orders", c_void_p),
                 ("errors", c_void_p)]
@@ -68,6 +73,25 @@ _SIGNATURES = {
                                          c_void_p, c_void_p]),
     "pty_multi_workspace_bytes": (c_size_t, [c_void_p, c_int]),
     "pty_multi_run": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p]),
+    "pty_conv_weight_words": (c_size_t, [ctypes.POINTER(PtyConvDesc), c_int]),
+    "pty_conv_forward": (c_int, [ctypes.POINTER(PtyConvDesc), c_void_p, c_void_p, c_void_p, c_int,
+                                 c_void_p, c_void_p, c_void_p]),
+    "pty_conv_backward_data": (c_int, [ctypes.POINTER(PtyConvDesc), c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p]),
+    "pty_conv_backward_weight_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyConvDesc)]),
+    "pty_conv_backward_weight": (c_int, [ctypes.POINTER(PtyConvDesc), c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_conv_adam": (c_int, [ctypes.POINTER(PtyConvDesc), c_float, c_float, c_float, c_float, c_int,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pty_conv_pack_weights": (c_int, [ctypes.POINTER(PtyConvDesc), c_void_p, c_void_p, c_void_p,
+                                      c_void_p]),
+    "pty_net_inputs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "pty_net_loss_workspace_bytes": (c_size_t, [c_int]),
+    "pty_net_loss": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size_t,
+                             c_void_p]),
+    "pty_net_patches": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "pty_net_finish": (c_int, [c_void_p, c_void_p, c_int, c_int, c_float, c_void_p, c_void_p]),
     "pty_gd_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams)]),
     "pty_gd_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams), c_void_p,
                            c_void_p, c_void_p, c_size_t, c_void_p]),

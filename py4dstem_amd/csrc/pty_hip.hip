@@ -13,6 +13,7 @@
 //                    recentring; k_error_reduce per-iteration error.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -825,7 +826,8 @@ int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px,
 
 int pty_scatter_patches(const void* ps, const void* chi, const float* positions, int N, const int* order, int B,
                         int S, int Px, int Py, void* grad, float* spsum, float* tilemax, pty_stream_t stream) {
-  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  // 32 x 32 windows are the learned initialiser's stitching patches
+  if (S != 32 && !frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (Px < S || Py < S || N <= 0 || B < 0) return fail(PTY_E_ARG, "bad scatter sizes");
   if (!ps || !chi || !positions || !order || !grad || !spsum || !tilemax) return fail(PTY_E_ARG, "null buffer in scatter");
   GatherArgs a{};
@@ -1201,3 +1203,5 @@ int pty_multi_run(const pty_multi_problem* problems, int count, int num_iter, in
 }
 
 }  // extern "C"
+
+#include "pty_net.inc"

@@ -434,6 +434,30 @@ class SingleslicePtychography:
             del ps, chi
         return overlap.cpu().numpy()
 
+    # -- learned starting object ----------------------------------------------------
+    def initialize_object_learned(self, model=None, stride=2, judge_positions=2048):
+        """Replace the starting object with the learned initialiser's stitched
+        prediction (py4dstem_amd.learned_init) when its data error on
+        `judge_positions` seeded positions is below the current starting
+        object's; otherwise keep it (flat-object fallback).  Call after
+        preprocess() and before reconstruct(); the choice and timings are in
+        ``learned_init_info``.  Returns self."""
+        from . import learned_init
+        if not getattr(self, "_preprocessed", False):
+            raise ValueError("preprocess() must run before initialize_object_learned()")
+        if model is None or isinstance(model, str):
+            model = learned_init.ConvNet.load(model or learned_init.DEFAULT_MODEL)
+        obj, used, info = learned_init.initialize_object(self, model, stride=stride,
+                                                         judge_positions=judge_positions)
+        self.learned_init_info = info
+        if used:
+            self._object = obj.cpu().numpy()
+            self._object_initial = self._object.copy()
+            for attr in ("_object_dev", "_probe_dev"):
+                if hasattr(self, attr):
+                    delattr(self, attr)
+        return self
+
     # -- execution tuning (results do not depend on it) --------------------------
     def set_stage(self, stage):
         """Positions per fused launch inside a mini-batch (0 = whole batch).

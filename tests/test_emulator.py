@@ -210,3 +210,34 @@ def test_simulation_pass_matches_forward_model(emu, small_case):
     rows, cols = po.patch_indices(pos, (S, S), obj.shape)
     ref = np.abs(np.fft.fft2(po.fft_shift(probe, pos - np.round(pos)) * obj[rows, cols])) ** 2
     assert rel(out, ref) < 1e-6
+
+
+def test_scatter_of_32px_stitching_windows(emu):
+    """The scatter with S = 32 (learned-initialiser stitching): weighted
+    patches of one object average back to that object where covered."""
+    rng = np.random.default_rng(16)
+    S, P = 32, 96
+    obj = np.exp(1j * rng.uniform(0, 0.5, (P, P))).astype(np.complex64)
+    g = np.arange(16, 81, 8)
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    pos = np.stack([gx.ravel(), gy.ravel()], 1).astype(np.float32)
+    B = len(pos)
+    u = np.arange(S)
+    w1 = np.sin(np.pi * (u + 0.5) / S) ** 2
+    w = np.fft.ifftshift(np.outer(w1, w1))            # corner-centred window
+    rows, cols = po.patch_indices(pos, (S, S), (P, P))
+    patches = obj[rows, cols]
+    ps = np.broadcast_to(np.sqrt(w), (B, S, S)).astype(np.complex64).copy()
+    chi = (np.sqrt(w) * patches).astype(np.complex64)
+    order = np.arange(B, dtype=np.int32)
+    grad = np.zeros((P, P), np.complex64)
+    wsum = np.zeros((P, P), np.float32)
+    tmax = np.zeros(emu.emu_gather_num_tiles(P, P), np.float32)
+    emu.emu_reset()
+    emu.emu_gather(S, cptr(ps), cptr(chi), cptr(pos), B, cptr(order), B, P, P, cptr(grad),
+                   cptr(wsum), cptr(tmax))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    ref = po.scatter_bincount(np.conj(ps) * chi, pos, (S, S), (P, P))
+    assert rel(grad, ref) < 1e-6
+    inner = (slice(16, 80), slice(16, 80))
+    assert rel(grad[inner] / wsum[inner], obj[inner]) < 1e-6
