@@ -310,8 +310,9 @@ class TrainingScans:
                     wgt = (f[:, 0] if di else 1 - f[:, 0]) * (f[:, 1] if dj else 1 - f[:, 1])
                     np.add.at(splat[s], ((i0[:, 0] + di) % P, (i0[:, 1] + dj) % P), peak * wgt)
         sp = torch.from_numpy(splat).to(self.dev)
+        # gauss_k is the transform of exp(-r^2 / 2 sigma^2) (peak 1): each
+        # atom adds `peak` rad at its centre
         phase = torch.fft.ifft2(torch.fft.fft2(sp) * self.gauss_k).real
-        phase = phase / (2 * np.pi * self.sigma_px ** 2)
         return torch.polar(torch.ones_like(phase), phase).to(torch.complex64)
 
     def chunk(self, n_scans):
@@ -425,3 +426,137 @@ def initialize_object(p, net=None, stride=2, judge_positions=2048, seed=0):
     info = {"seconds": time.perf_counter() - t0, "error_learned": e_learned, "error_flat": e_flat,
             "patches": int(len(ij)), "used_learned": bool(use)}
     return (learned if use else flat), bool(use), info
+
+
+# -- evaluation at production shapes -------------------------------------------------
+def evaluation_workload(seed, device=None, scan=128, object_px=720, dose=1e6, step_px=4):
+    """A held-out object from the training distribution at production shapes
+    (128x128 raster at 0.4 A, 128 px frames at 0.1 A, 720^2 object), simulated
+    on the GPU with Poisson noise at `dose` e/A^2.  Same dictionary layout as
+    py4dstem_amd.synthetic.production_workload."""
+    from .datacube import Calibration, DataCube
+    from .preprocess import center_positions
+
+    dev = device if device is not None else require_gpu()
+    gen = TrainingScans(dev, seed=seed, object_px=object_px)
+    obj = gen.objects(1)[0].contiguous()
+    g = np.arange(scan) * step_px
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    pos = center_positions(np.stack([gx.ravel(), gy.ravel()], 1).astype(np.float32) + 64.0,
+                           (object_px, object_px))
+    posd = torch.from_numpy(pos).to(dev)
+    N, S = len(pos), gen.S
+    counts = torch.empty((N, S, S), dtype=torch.float32, device=dev)
+    for s0 in range(0, N, 4096):
+        n = min(4096, N - s0)
+        _call("pty_simulate_intensities", ptr(gen.phat), ptr(obj), object_px, object_px,
+              ptr(posd[s0:s0 + n]), n, S, ptr(counts[s0:s0 + n]), stream_handle())
+    counts.mul_(float(dose * (step_px * gen.sampling) ** 2))
+    torch.manual_seed(seed)
+    counts = torch.poisson(counts)
+    data = torch.fft.fftshift(counts, dim=(-2, -1)).cpu().numpy().reshape(scan, scan, S, S)
+    del counts
+    dk = 1.0 / (S * gen.sampling)
+    cal = Calibration(R_pixel_size=step_px * gen.sampling, R_pixel_units="A",
+                      Q_pixel_size=dk, Q_pixel_units="A^-1")
+    return {
+        "datacube": DataCube(data, cal), "object_true": obj.cpu().numpy(),
+        "init": dict(energy=300e3, semiangle_cutoff=20.0, rolloff=2.0, defocus=200.0,
+                     initial_object_guess=np.ones((object_px, object_px), np.complex64),
+                     verbose=False),
+        "preprocess": dict(force_com_rotation=0, force_com_transpose=False,
+                           force_com_shifts=(S / 2, S / 2), plot_center_of_mass=False,
+                           plot_rotation=False, plot_probe_overlaps=False),
+        "reconstruct": dict(max_batch_size=1024, fix_probe=False, step_size=0.5,
+                            normalization_min=1, progress_bar=False),
+    }
+
+
+def time_to_nmse(w, net=None, max_iter=12, target=1e-3, seed=0):
+    """Flat against learned start on one workload: per-iteration object NMSE
+    (scanned field), iterations and wall time to NMSE <= target (wall time =
+    initialiser time + iterations x measured seconds per iteration)."""
+    from .ptychography import SingleslicePtychography
+    from .synthetic import object_nmse
+
+    out = {}
+    for kind in ("flat", "learned"):
+        p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+        p.preprocess(**w["preprocess"])
+        t_init = 0.0
+        info = None
+        if kind == "learned":
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            p.initialize_object_learned(net)
+            torch.cuda.synchronize()
+            t_init = time.perf_counter() - t
+            info = p.learned_init_info
+        mask = p._object_fov_mask
+        nmse = [object_nmse(p._object, w["object_true"], mask)]
+        secs = []
+        for it in range(max_iter):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            p.reconstruct(num_iter=1, reset=(it == 0), seed_random=seed + it, **w["reconstruct"])
+            torch.cuda.synchronize()
+            secs.append(time.perf_counter() - t)
+            nmse.append(object_nmse(p.object, w["object_true"], mask))
+        hit = next((k for k, v in enumerate(nmse) if v <= target), None)
+        t_iter = float(np.median(secs))
+        out[kind] = {"nmse": nmse, "iterations": hit, "s_per_iteration": t_iter,
+                     "init_seconds": t_init,
+                     "seconds": None if hit is None else t_init + hit * t_iter, "init_info": info}
+    return out
+
+
+def main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="train / evaluate the learned object initialiser")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--scans", type=int, default=20000)
+    t.add_argument("--scans-per-chunk", type=int, default=16)
+    t.add_argument("--samples-per-chunk", type=int, default=1024)
+    t.add_argument("--batch", type=int, default=256)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--time-limit", type=float, default=None)
+    t.add_argument("--out", default=DEFAULT_MODEL)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--model", default=DEFAULT_MODEL)
+    e.add_argument("--seeds", type=int, nargs="+", default=[1001, 1002])
+    e.add_argument("--max-iter", type=int, default=12)
+    e.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(f"[learned-init] {msg}", flush=True)
+
+    if a.cmd == "train":
+        t0 = time.perf_counter()
+        net, hist = train(num_scans=a.scans, scans_per_chunk=a.scans_per_chunk,
+                          samples_per_chunk=a.samples_per_chunk, batch=a.batch, lr=a.lr,
+                          seed=a.seed, log=log, time_limit=a.time_limit)
+        net.save(a.out, scans=a.scans, seed=a.seed, steps=net.step,
+                 seconds=time.perf_counter() - t0, loss_history=np.asarray(hist))
+        log(f"saved {a.out} after {net.step} steps, {time.perf_counter() - t0:.1f} s")
+    else:
+        net = ConvNet.load(a.model)
+        res = {}
+        for s in a.seeds:
+            log(f"evaluation seed {s}")
+            w = evaluation_workload(s)
+            res[s] = time_to_nmse(w, net, max_iter=a.max_iter)
+            for kind, r in res[s].items():
+                log(f"  {kind}: iterations {r['iterations']} seconds {r['seconds']} "
+                    f"nmse {[f'{v:.2e}' for v in r['nmse'][:6]]} info {r['init_info']}")
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1, default=float)
+
+
+if __name__ == "__main__":
+    main()

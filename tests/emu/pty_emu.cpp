@@ -21,15 +21,24 @@ static void note_violation(const char* what) {
 
 using namespace pty;
 
+// Per LDS word: the writer / reader thread of the current phase (wt, rd; -2 =
+// several readers) and the wave that wrote / read it since the last
+// work-group barrier (ww, rw; -1 none, -2 several waves).  Across a wave
+// barrier only the same wave may touch a word again; anything else is a race.
+constexpr int WAVE = 64;
+
 struct LdsCheck {
   cf* mem;
   int n;
   int* wt;
   int* rd;
+  int* ww;
+  int* rw;
   const int* cur;
   cf ld(int i) const {
     if (i < 0 || i >= n) { note_violation("lds load out of range"); return mkcf(0, 0); }
     if (wt[i] >= 0 && wt[i] != *cur) note_violation("lds read of a word another thread wrote");
+    if (ww[i] != -1 && ww[i] != *cur / WAVE) note_violation("lds read of a word another wave wrote (wave barrier only)");
     if (rd[i] == -1) rd[i] = *cur; else if (rd[i] != *cur) rd[i] = -2;
     return mem[i];
   }
@@ -37,6 +46,8 @@ struct LdsCheck {
     if (i < 0 || i >= n) { note_violation("lds store out of range"); return; }
     if (wt[i] >= 0 && wt[i] != *cur) note_violation("lds write-write race");
     if (rd[i] == -2 || (rd[i] >= 0 && rd[i] != *cur)) note_violation("lds write of a word another thread read");
+    if (ww[i] != -1 && ww[i] != *cur / WAVE) note_violation("lds write of a word another wave wrote (wave barrier only)");
+    if (rw[i] != -1 && rw[i] != *cur / WAVE) note_violation("lds write of a word another wave read (wave barrier only)");
     wt[i] = *cur;
     mem[i] = v;
   }
@@ -46,21 +57,21 @@ template <int S> struct EmuRunner {
   using K = Fused<S>;
   static constexpr int T = Frame<S>::T;
   std::vector<cf> mem;
-  std::vector<int> wt, rd;
+  std::vector<int> wt, rd, ww, rw;
   std::vector<typename K::Regs> regs;
   std::vector<cf> tw;
   int cur = -1;
   const FusedArgs* a;
   EmuRunner(const FusedArgs* args)
       : mem(K::LDS_TOTAL, mkcf(0, 0)), wt(K::LDS_TOTAL, -1), rd(K::LDS_TOTAL, -1),
-        regs(T), tw(S), a(args) {
+        ww(K::LDS_TOTAL, -1), rw(K::LDS_TOTAL, -1), regs(T), tw(S), a(args) {
     for (int t = 0; t < S; ++t) {
       struct TwSt { cf* p; void st(int i, cf v) const { p[i] = v; } } view{tw.data()};
       K::twiddles(t, view);
     }
   }
   template <class Fn> void phase(Fn fn) {
-    LdsCheck view{mem.data(), (int)mem.size(), wt.data(), rd.data(), &cur};
+    LdsCheck view{mem.data(), (int)mem.size(), wt.data(), rd.data(), ww.data(), rw.data(), &cur};
     for (int t = 0; t < T; ++t) {
       cur = t;
       fn(t, regs[t], view, tw.data());
@@ -70,6 +81,19 @@ template <int S> struct EmuRunner {
   void barrier() {
     std::fill(wt.begin(), wt.end(), -1);
     std::fill(rd.begin(), rd.end(), -1);
+    std::fill(ww.begin(), ww.end(), -1);
+    std::fill(rw.begin(), rw.end(), -1);
+  }
+  // fold this phase's accesses into the per-wave record, then open a new phase
+  void wave_barrier() {
+    auto merge = [](int have, int w) { return have == -1 ? w : (have == w ? w : -2); };
+    for (size_t i = 0; i < wt.size(); ++i) {
+      if (wt[i] >= 0) ww[i] = merge(ww[i], wt[i] / WAVE);
+      if (rd[i] >= 0) rw[i] = merge(rw[i], rd[i] / WAVE);
+      if (rd[i] == -2) rw[i] = -2;  // several readers: conservatively several waves
+      wt[i] = -1;
+      rd[i] = -1;
+    }
   }
   void reduce_err(int b) {
     double s = 0.0;
@@ -178,11 +202,11 @@ template <int S> static void fft_fwd(const cf* in, cf* out) {
     Pass<S>::template A<false>(n2, r.v, tw);
     for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
   });
-  run.barrier();
+  run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
-  run.barrier();
+  run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
     for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
@@ -203,11 +227,11 @@ template <int S> static void fft_inv(const cf* in, cf* out) {
     Pass<S>::template B<true>(t, r.v, tw);
     for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
   });
-  run.barrier();
+  run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
-  run.barrier();
+  run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
     K::p_colAinv_to_regs(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
