@@ -137,69 +137,33 @@ template <int S> struct Geo;
 template <> struct Geo<64> { static constexpr int N1 = 8, N2 = 8; };
 template <> struct Geo<128> { static constexpr int N1 = 16, N2 = 8; };
 
-// XOR swizzle of the low four column bits of LDS word (row, col).  A line
-// (row or column) is owned by the N2 = 8 consecutive lanes of one wave, so
-// every 16-lane group of a 64-bit LDS access touches two neighbouring lines
-// and eight q values.  The four access patterns of the passes are
-//   row A (row r, col q + N2 m)   row B (row r, col N1 q + j)
-//   col A (row q + N2 m, col c)   col B (row N1 q + j, col c)
-// with r, c even-based pairs.  The bank (word mod 16) is a GF(2)-linear map of
-// the row and column bits chosen so that each pattern's four varying bits map
-// onto the four bank bits independently: no 16-lane group has a conflict.
-template <int S> struct Swz;
-template <> struct Swz<64> {
-  // bank = [c0^c3, c1^c4^r1^r3, c2^c5^r2^r4, c3^r0^r5]
-  static PTY_HD int xc(int c) { return ((c >> 3) & 1) | (((c >> 4) & 1) << 1) | (((c >> 5) & 1) << 2); }
-  static PTY_HD int xr(int r) {
-    return ((((r >> 1) ^ (r >> 3)) & 1) << 1) | ((((r >> 2) ^ (r >> 4)) & 1) << 2) | (((r ^ (r >> 5)) & 1) << 3);
-  }
-};
-template <> struct Swz<128> {
-  // bank = [c0^c4, c1^c5^r1^r4, c2^c6^r2^r5, c3^r0^r6]
-  static PTY_HD int xc(int c) { return ((c >> 4) & 1) | (((c >> 5) & 1) << 1) | (((c >> 6) & 1) << 2); }
-  static PTY_HD int xr(int r) {
-    return ((((r >> 1) ^ (r >> 4)) & 1) << 1) | ((((r >> 2) ^ (r >> 5)) & 1) << 2) | (((r ^ (r >> 6)) & 1) << 3);
-  }
-};
-
 template <int S> struct Frame {
   static constexpr int N1 = Geo<S>::N1;
   static constexpr int N2 = Geo<S>::N2;
   static constexpr int R = N1 / N2;   // pass-B groups per thread
   static constexpr int T = S * N2;    // threads per frame
-  static constexpr int LDS_WORDS = S * S;
-  // The swizzle only flips column bits 0..3, by functions of the row and of
-  // column bits that it leaves unchanged or flips by row bits alone, so it is
-  // a bijection inside each row.  It is linear over GF(2) (no constant term).
-  static PTY_HD int xr(int r) { return Swz<S>::xr(r); }
-  static PTY_HD int xc(int c) { return Swz<S>::xc(c); }
-  static PTY_HD int at(int row, int col) { return row * S + (col ^ xr(row) ^ xc(col)); }
-  // The four pass patterns as a per-thread base and a per-element constant
-  // (m, j are unrolled loop indices): by linearity
-  //   at(r, q + N2 m)   = rowA_base(r, q) ^ ((N2 m) ^ xc(N2 m))
-  //   at(r, N1 t + j)   = rowB_base(r, t) ^ (j ^ xc(j))
-  //   at(q + N2 m, c)   = (colA_base(c, q) ^ xr(N2 m)) + N2 m S
-  //   at(N1 t + j, c)   = (colB_base(c, t) ^ xr(j)) + j S
-  // (q < N2, j < N1; the XORed constants are below S, the added ones multiples of S).
-  static PTY_HD int rowA_base(int r, int q) { return r * S + (q ^ xr(r) ^ xc(q)); }
-  static PTY_HD int rowA_at(int base, int m) { return base ^ ((N2 * m) ^ xc(N2 * m)); }
-  static PTY_HD int rowB_base(int r, int t) { return r * S + ((N1 * t) ^ xr(r) ^ xc(N1 * t)); }
-  static PTY_HD int rowB_at(int base, int j) { return base ^ (j ^ xc(j)); }
-  static PTY_HD int colA_base(int c, int q) { return q * S + (c ^ xr(q) ^ xc(c)); }
-  static PTY_HD int colA_at(int base, int m) { return (base ^ xr(N2 * m)) + N2 * m * S; }
-  static PTY_HD int colB_base(int c, int t) { return (N1 * t) * S + (c ^ xr(N1 * t) ^ xc(c)); }
-  static PTY_HD int colB_at(int base, int j) { return (base ^ xr(j)) + j * S; }
+  static constexpr int PITCH = S + S / N1; // LDS row pitch in complex words
+  static constexpr int LDS_WORDS = S * PITCH;
+  // One pad word after every N1-word block of a row.  Pass A (stride N2)
+  // and pass B (contiguous runs of N1 starting at N1*t) then hit distinct
+  // banks, the row pitch S + S/N1 = S + 8 words staggers neighbouring rows,
+  // and every per-thread access is base + compile-time offset.
+  static PTY_HD int phys(int p) { return p + p / N1; }
+  static PTY_HD int at(int row, int col) { return row * PITCH + phys(col); }
   // natural frequency index held at permuted position p
   static PTY_HD int kidx(int p) { return p / N2 + N1 * (p % N2); }
   // fftfreq integer of natural index k
   static PTY_HD int fidx(int k) { return k < S / 2 ? k : k - S; }
-  // thread -> (line, q): a line's N2 lanes are consecutive lanes of one
-  // wave for rows and for columns, so the exchange between the two passes
-  // of a line stays inside the wave (wave barrier, no work-group barrier)
+  // thread -> (line, q) for row passes and column passes.  A row's N2 lanes
+  // are consecutive lanes of one wave, so the exchange between the two
+  // passes along rows stays inside the wave (a wave barrier suffices).  A
+  // column's lanes are S apart (one wave covers 64 consecutive columns, which
+  // keeps global accesses of column passes row-contiguous); exchanges between
+  // column passes cross waves and need a work-group barrier.
   static PTY_HD int row_line(int tid) { return tid / N2; }
   static PTY_HD int row_q(int tid) { return tid % N2; }
-  static PTY_HD int col_line(int tid) { return tid / N2; }
-  static PTY_HD int col_q(int tid) { return tid % N2; }
+  static PTY_HD int col_line(int tid) { return tid % S; }
+  static PTY_HD int col_q(int tid) { return tid / S; }
 };
 
 PTY_HD int wrapi(int a, int n) {
@@ -275,6 +239,8 @@ template <int S> struct Fused {
 
   struct Regs {
     cf v[N1];
+    float amp[N1]; // this thread's measured amplitudes (row pass B layout),
+                   // loaded at setup so their latency hides behind the FFTs
     double err;
     int x0, y0;   // patch anchor, wrapped into [0, Px) x [0, Py)
     int idx;      // position index of this batch slot
@@ -314,7 +280,7 @@ template <int S> struct Fused {
 
   // P0: anchor of slot b and the ramp table (threads tid < 2S)
   template <class L> static PTY_HD void p_setup(int tid, const FusedArgs& a, int b, Regs& r, L lds,
-                                                bool conj_ramp = false) {
+                                                bool conj_ramp = false, bool load_amps = false) {
     int idx = a.order ? a.order[b] : b;  // no order: slot b is position b
     PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;  // memory-safe on bad input
@@ -324,6 +290,11 @@ template <int S> struct Fused {
     r.x0 = wrapi((int)rx, a.Px);
     r.y0 = wrapi((int)ry, a.Py);
     r.err = 0.0;
+    if (load_amps) {
+      const float* A = a.amps + (size_t)idx * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) r.amp[j] = A[j];
+    }
     if (tid < 2 * S) {
       const int k = tid < S ? tid : tid - S;
       const float frac = tid < S ? px - rx : py - ry;
@@ -345,39 +316,39 @@ template <int S> struct Fused {
     }
     P::template B<true>(t, r.v, tw);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) lds.st(F::rowB_at(F::rowB_base(rr, t), j), r.v[j]);
+    for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
   }
 
   // generic passes reading and writing LDS
   template <bool INV, class L> static PTY_HD void p_rowA(int tid, Regs& r, L lds, const cf* tw) {
     const int rr = F::row_line(tid), n2 = F::row_q(tid);
 #pragma unroll
-    for (int m = 0; m < N1; ++m) r.v[m] = lds.ld(F::rowA_at(F::rowA_base(rr, n2), m));
+    for (int m = 0; m < N1; ++m) r.v[m] = lds.ld(F::at(rr, n2 + N2 * m));
     P::template A<INV>(n2, r.v, tw);
 #pragma unroll
-    for (int m = 0; m < N1; ++m) lds.st(F::rowA_at(F::rowA_base(rr, n2), m), r.v[m]);
+    for (int m = 0; m < N1; ++m) lds.st(F::at(rr, n2 + N2 * m), r.v[m]);
   }
   template <bool INV, class L> static PTY_HD void p_rowB(int tid, Regs& r, L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::rowB_at(F::rowB_base(rr, t), j));
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<INV>(t, r.v, tw);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) lds.st(F::rowB_at(F::rowB_base(rr, t), j), r.v[j]);
+    for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
   }
   template <bool INV, class L> static PTY_HD void p_colB(int tid, Regs& r, L lds, const cf* tw) {
     const int cc = F::col_line(tid), t = F::col_q(tid);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::colB_at(F::colB_base(cc, t), j));
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(N1 * t + j, cc));
     P::template B<INV>(t, r.v, tw);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) lds.st(F::colB_at(F::colB_base(cc, t), j), r.v[j]);
+    for (int j = 0; j < N1; ++j) lds.st(F::at(N1 * t + j, cc), r.v[j]);
   }
   // column pass A' leaving the natural-order result in registers
   template <class L> static PTY_HD void p_colAinv_to_regs(int tid, Regs& r, L lds, const cf* tw) {
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
 #pragma unroll
-    for (int m = 0; m < N1; ++m) r.v[m] = lds.ld(F::colA_at(F::colA_base(cc, n2), m));
+    for (int m = 0; m < N1; ++m) r.v[m] = lds.ld(F::at(n2 + N2 * m, cc));
     P::template A<true>(n2, r.v, tw);
     const float s = 1.0f / (float)(S * S);
 #pragma unroll
@@ -412,7 +383,7 @@ template <int S> struct Fused {
     }
     P::template A<false>(n2, r.v, tw);
 #pragma unroll
-    for (int m = 0; m < N1; ++m) lds.st(F::colA_at(F::colA_base(cc, n2), m), r.v[m]);
+    for (int m = 0; m < N1; ++m) lds.st(F::at(n2 + N2 * m, cc), r.v[m]);
   }
 
   // P8: row pass B (completing fft2), amplitude projection, row pass B'
@@ -420,16 +391,15 @@ template <int S> struct Fused {
                                                   L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::rowB_at(F::rowB_base(rr, t), j));
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
-    const float* A = a.amps + (size_t)r.idx * S * S + rr * S + N1 * t;
     cf* far = a.fwd_out ? a.fwd_out + (size_t)b * S * S + rr * S + N1 * t : nullptr;
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
       const cf f = r.v[j];
       if (far) far[j] = f;
-      const float amp = A[j];
+      const float amp = r.amp[j];
       const float m2 = cabs2(f);
       cf g;
       float mag;
@@ -448,7 +418,7 @@ template <int S> struct Fused {
     r.err += acc;
     P::template B<true>(t, r.v, tw);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) lds.st(F::rowB_at(F::rowB_base(rr, t), j), r.v[j]);
+    for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
   }
 
   // probe gradient, first phase: g = conj(O_patch) * chi from global memory,
@@ -464,7 +434,7 @@ template <int S> struct Fused {
     }
     P::template A<false>(n2, r.v, tw);
 #pragma unroll
-    for (int m = 0; m < N1; ++m) lds.st(F::colA_at(F::colA_base(cc, n2), m), r.v[m]);
+    for (int m = 0; m < N1; ++m) lds.st(F::at(n2 + N2 * m, cc), r.v[m]);
   }
 
   // probe gradient, last phase: row pass B completes fft2(g); multiply by the
@@ -473,7 +443,7 @@ template <int S> struct Fused {
                                                     L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::rowB_at(F::rowB_base(rr, t), j));
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
     const cf ex = lds.ld(RAMP + F::kidx(rr));
     cf* out = a.pgrad_out + (size_t)b * S * S + rr * S + N1 * t;
@@ -489,7 +459,7 @@ template <int S> struct Fused {
                                                   const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::rowB_at(F::rowB_base(rr, t), j));
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
     float* out = a.sim_out + (size_t)b * S * S + F::kidx(rr) * S;
 #pragma unroll
@@ -510,24 +480,24 @@ template <int S> struct Fused {
 // The phase sequence for one scan position.  ``run`` executes a phase for
 // every thread (GPU: this thread; emulator: all thread ids in turn).
 // ``run.barrier()`` is a work-group barrier, needed where data cross waves
-// (row pass <-> column pass, the ramp table); ``run.wave_barrier()`` only
-// orders the LDS accesses of one wave, enough between two passes along the
-// same lines.  Five work-group barriers per position.
+// (the ramp table, column passes, row <-> column); ``run.wave_barrier()``
+// only orders the LDS accesses of one wave, enough between two passes along
+// rows.  Nine work-group barriers per position.
 template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
   using K = Fused<S>;
   using Regs = typename K::Regs;
   // ifft2(probe_hat * ramp): B', A' along rows, then B', A' along columns
-  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, false, true); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
-  run.wave_barrier();
+  run.barrier();
   // shifted probe in registers; exit wave; fft2 starts with columns
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_exitwave_colA(tid, a, b, r, lds, tw); });
-  run.wave_barrier();
+  run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
@@ -538,7 +508,7 @@ template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
-  run.wave_barrier();
+  run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_outputs(tid, a, b, r, lds, tw); });
   run.reduce_err(b);  // ends with a barrier
 }
@@ -554,9 +524,9 @@ template <int S, class Run> PTY_HD void simulate_position(Run& run, const FusedA
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
-  run.wave_barrier();
+  run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_exitwave_colA(tid, a, b, r, lds, tw); });
-  run.wave_barrier();
+  run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
@@ -573,7 +543,7 @@ template <int S, class Run> PTY_HD void probe_grad_position(Run& run, const Fuse
   run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, true); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_pgrad_colA(tid, a, b, r, lds, tw); });
-  run.wave_barrier();
+  run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });

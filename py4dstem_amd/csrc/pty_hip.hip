@@ -4,10 +4,10 @@
 //   k_fused          one work group per position: ifft2(probe_hat * ramp),
 //                    periodic object gather, exit wave, fft2, amplitude
 //                    projection, ifft2, adjoint products.  The frame lives in
-//                    LDS (S=128: 128 KiB of the 160 KiB per CU, XOR-swizzled);
-//                    each row or column is owned by 8 lanes of one wave, so
-//                    only the row <-> column transposes need work-group
-//                    barriers (five per position).
+//                    LDS (S=128: 136 KiB of the 160 KiB per CU); a row is
+//                    owned by 8 lanes of one wave, so passes along rows are
+//                    separated by wave barriers, everything else by
+//                    work-group barriers (nine per position).
 //   k_gather         one work group per 8 x 32 object tile: float64 sums of
 //                    the staged patches in batch order (deterministic).
 //   k_object_update  normalised object step.
@@ -209,15 +209,15 @@ __device__ __forceinline__ void fft2_body(const cf* __restrict__ in, cf* __restr
     for (int m = 0; m < F::N1; ++m) r.v[m] = src[(n2 + F::N2 * m) * S + cc];
     Pass<S>::template A<false>(n2, r.v, tw);
 #pragma unroll
-    for (int m = 0; m < F::N1; ++m) lds.st(F::colA_at(F::colA_base(cc, n2), m), r.v[m]);
-    wave_sync();
+    for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
+    __syncthreads();
     K::template p_colB<false>(tid, r, lds, tw);
     __syncthreads();
     K::template p_rowA<false>(tid, r, lds, tw);
     wave_sync();
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
-    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::rowB_at(F::rowB_base(rr, t), j));
+    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
     Pass<S>::template B<false>(t, r.v, tw);
     if (natural_io) {
       const int kr = F::kidx(rr);
@@ -239,12 +239,12 @@ __device__ __forceinline__ void fft2_body(const cf* __restrict__ in, cf* __restr
     }
     Pass<S>::template B<true>(t, r.v, tw);
 #pragma unroll
-    for (int j = 0; j < F::N1; ++j) lds.st(F::rowB_at(F::rowB_base(rr, t), j), r.v[j]);
+    for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
     wave_sync();
     K::template p_rowA<true>(tid, r, lds, tw);
     __syncthreads();
     K::template p_colB<true>(tid, r, lds, tw);
-    wave_sync();
+    __syncthreads();
     K::p_colAinv_to_regs(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
 #pragma unroll
@@ -295,24 +295,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int tile) {
   gather_tile(run, a, tile);
 }
 
-// Work groups are dispatched to the 8 XCDs round-robin.  Remap them so that
-// each XCD processes a contiguous run of tiles: neighbouring tiles read the
-// same patch cache lines at their edges, which then hit in that XCD's L2.
-// Grid = xcd_grid(n); block b gets tile (b % 8) * ceil(n / 8) + b / 8, or -1.
-constexpr int NUM_XCD = 8;
-__host__ __device__ inline int xcd_grid(int n) { return (n + NUM_XCD - 1) / NUM_XCD * NUM_XCD; }
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-  const int per = (n + NUM_XCD - 1) / NUM_XCD;
-  const int k = b / NUM_XCD;
-  if (k >= per) return -1;
-  const int t = (b % NUM_XCD) * per + k;
-  return t < n ? t : -1;
-}
-
-__global__ __launch_bounds__(GATHER_THREADS) void k_gather(GatherArgs a) {
-  const int t = xcd_tile((int)blockIdx.x, gather_num_tiles(a.Px, a.Py));
-  if (t >= 0) gather_body(a, t);
-}
+__global__ __launch_bounds__(GATHER_THREADS) void k_gather(GatherArgs a) { gather_body(a, (int)blockIdx.x); }
 
 __device__ float block_max(float v, float* sh) {
   sh[threadIdx.x] = v;
@@ -386,30 +369,17 @@ __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, in
   const int fj = j < S / 2 ? j : j - S;
   const int b1 = min(a.B, (chunk + 1) * PROBE_CHUNK);
   double re = 0.0, im = 0.0, so = 0.0;
-  // eight positions' loads in flight, then the float64 sums in batch order
-  constexpr int U = 8;
-  int b = chunk * PROBE_CHUNK;
-  for (; b < b1; b += U) {
-    cf o[U], c[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int bb = b + u < b1 ? b + u : b;
-      int idx = a.order[bb];
-      idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
-      const int x0 = (int)rintf(a.positions[2 * idx]);
-      const int y0 = (int)rintf(a.positions[2 * idx + 1]);
-      o[u] = a.object[(size_t)wrapi(x0 + fi, a.Px) * a.Py + wrapi(y0 + fj, a.Py)];
-      c[u] = a.chi[((size_t)bb * S + i) * S + j];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (b + u >= b1) break;
-      const float ao = hypotf(o[u].x, o[u].y);
-      so += (double)(ao * ao);
-      const cf g = cconjmul(o[u], c[u]);
-      re += (double)g.x;
-      im += (double)g.y;
-    }
+  for (int b = chunk * PROBE_CHUNK; b < b1; ++b) {
+    int idx = a.order[b];
+    idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
+    const int x0 = (int)rintf(a.positions[2 * idx]);
+    const int y0 = (int)rintf(a.positions[2 * idx + 1]);
+    const cf o = a.object[(size_t)wrapi(x0 + fi, a.Px) * a.Py + wrapi(y0 + fj, a.Py)];
+    const float ao = hypotf(o.x, o.y);
+    so += (double)(ao * ao);
+    const cf g = cconjmul(o, a.chi[((size_t)b * S + i) * S + j]);
+    re += (double)g.x;
+    im += (double)g.y;
   }
   double* p = a.part + ((size_t)chunk * S * S + pix) * 3;
   p[0] = re;
@@ -598,7 +568,7 @@ template <int S> __device__ __forceinline__ void probe_com_body(cf* __restrict__
   // forward fft2 (columns then rows) with the ramp applied in kernel layout
   Pass<S>::template A<false>(n2, r.v, tw);
 #pragma unroll
-  for (int m = 0; m < F::N1; ++m) lds.st(F::colA_at(F::colA_base(cc, n2), m), r.v[m]);
+  for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
   __syncthreads();
   K::template p_colB<false>(tid, r, lds, tw);
   __syncthreads();
@@ -607,7 +577,7 @@ template <int S> __device__ __forceinline__ void probe_com_body(cf* __restrict__
   {
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
-    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::rowB_at(F::rowB_base(rr, t), j));
+    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
     Pass<S>::template B<false>(t, r.v, tw);
     const cf ex = K::ramp1(F::kidx(rr), shift[0]);
 #pragma unroll
@@ -617,7 +587,7 @@ template <int S> __device__ __forceinline__ void probe_com_body(cf* __restrict__
     }
     Pass<S>::template B<true>(t, r.v, tw);
 #pragma unroll
-    for (int j = 0; j < F::N1; ++j) lds.st(F::rowB_at(F::rowB_base(rr, t), j), r.v[j]);
+    for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
   }
   __syncthreads();
   K::template p_rowA<true>(tid, r, lds, tw);
@@ -682,7 +652,7 @@ int launch_fft(const cf* in, cf* out, int batch, int S, bool inv, int natural, h
 int num_tiles(int Px, int Py) { return gather_num_tiles(Px, Py); }
 
 int launch_gather(const GatherArgs& a, int Px, int Py, hipStream_t stream) {
-  hipLaunchKernelGGL(k_gather, dim3(xcd_grid(num_tiles(Px, Py))), dim3(GATHER_THREADS), 0, stream, a);
+  hipLaunchKernelGGL(k_gather, dim3(num_tiles(Px, Py)), dim3(GATHER_THREADS), 0, stream, a);
   return check_launch("k_gather");
 }
 

@@ -202,7 +202,7 @@ template <int S> static void fft_fwd(const cf* in, cf* out) {
     Pass<S>::template A<false>(n2, r.v, tw);
     for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
   });
-  run.wave_barrier();
+  run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
@@ -231,7 +231,7 @@ template <int S> static void fft_inv(const cf* in, cf* out) {
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
-  run.wave_barrier();
+  run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
     K::p_colAinv_to_regs(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
