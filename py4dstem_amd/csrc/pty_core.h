@@ -253,7 +253,7 @@ template <int S> struct Fused {
 
   // twiddle table into LDS (tid < S)
   template <class L> static PTY_HD void twiddles(int tid, L tw) {
-    if (tid < S) {
+    if ((unsigned)tid < (unsigned)S) {
 #if defined(__HIP_DEVICE_COMPILE__)
       double sn, cs;
       sincospi(-2.0 * tid / S, &sn, &cs);
@@ -280,7 +280,8 @@ template <int S> struct Fused {
 
   // P0: anchor of slot b and the ramp table (threads tid < 2S)
   template <class L> static PTY_HD void p_setup(int tid, const FusedArgs& a, int b, Regs& r, L lds,
-                                                bool conj_ramp = false, bool load_amps = false) {
+                                                bool conj_ramp = false, bool load_amps = false,
+                                                bool load_probe = false) {
     int idx = a.order ? a.order[b] : b;  // no order: slot b is position b
     PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;  // memory-safe on bad input
@@ -295,6 +296,13 @@ template <int S> struct Fused {
 #pragma unroll
       for (int j = 0; j < N1; ++j) r.amp[j] = A[j];
     }
+    // this thread's probe spectrum values for the first row pass; the loads
+    // stay in flight across the barrier that follows (it waits for LDS only)
+    if (load_probe) {
+      const cf* ph = a.probe_hat + F::row_line(tid) * S + N1 * F::row_q(tid);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) r.v[j] = ph[j];
+    }
     if (tid < 2 * S) {
       const int k = tid < S ? tid : tid - S;
       const float frac = tid < S ? px - rx : py - ry;
@@ -308,11 +316,10 @@ template <int S> struct Fused {
                                                         L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
     const cf ex = lds.ld(RAMP + F::kidx(rr));
-    const cf* ph = a.probe_hat + rr * S + N1 * t;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
       const cf e = cmul(ex, lds.ld(RAMP + S + F::kidx(N1 * t + j)));
-      r.v[j] = cmul(ph[j], e);
+      r.v[j] = cmul(r.v[j], e);  // probe spectrum loaded at setup
     }
     P::template B<true>(t, r.v, tw);
 #pragma unroll
@@ -487,7 +494,7 @@ template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs
   using K = Fused<S>;
   using Regs = typename K::Regs;
   // ifft2(probe_hat * ramp): B', A' along rows, then B', A' along columns
-  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, false, true); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, false, true, true); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
   run.wave_barrier();
@@ -517,7 +524,7 @@ template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs
 template <int S, class Run> PTY_HD void simulate_position(Run& run, const FusedArgs& a, int b) {
   using K = Fused<S>;
   using Regs = typename K::Regs;
-  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, false, false, true); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
   run.wave_barrier();
@@ -562,8 +569,8 @@ template <int S, class Run> PTY_HD void probe_grad_position(Run& run, const Fuse
 // pixel.  Per pixel the sum order is the batch order, as in the reference's
 // np.bincount; the float32 products are formed exactly as the reference does.
 // ---------------------------------------------------------------------------
-constexpr int GT_R = 8;                  // tile rows
-constexpr int GT_C = 32;                 // tile columns
+constexpr int GT_R = 4;                  // tile rows
+constexpr int GT_C = 64;                 // tile columns (512-byte patch-row reads)
 constexpr int GATHER_THREADS = GT_R * GT_C;
 constexpr int GATHER_CAND = 4;           // candidates tested per thread per segment
 constexpr int GATHER_SEG = GATHER_THREADS * GATHER_CAND;

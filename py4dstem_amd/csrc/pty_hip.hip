@@ -8,7 +8,7 @@
 //                    owned by 8 lanes of one wave, so passes along rows are
 //                    separated by wave barriers, everything else by
 //                    work-group barriers (nine per position).
-//   k_gather         one work group per 8 x 32 object tile: float64 sums of
+//   k_gather         one work group per 4 x 64 object tile: float64 sums of
 //                    the staged patches in batch order (deterministic).
 //   k_object_update  normalised object step.
 //   k_probe_partial / k_probe_update   probe step from staged terms.
@@ -105,8 +105,9 @@ template <int S> __device__ __forceinline__ void fused_body(const FusedArgs& a, 
   __shared__ cf tw[S];
   __shared__ double red[Frame<S>::T / 64];
   const int tid = threadIdx.x;
-  Fused<S>::twiddles(tid, LdsPtr{tw});
-  __syncthreads();
+  // threads [2S, 3S) are idle in the setup phase; its closing barrier
+  // publishes the table
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
   fused_position<S>(run, a, b);
@@ -123,8 +124,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_grad(FusedArgs a) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
   __shared__ cf tw[S];
   const int tid = threadIdx.x;
-  Fused<S>::twiddles(tid, LdsPtr{tw});
-  __syncthreads();
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, nullptr, a};
   probe_grad_position<S>(run, a, a.b0 + (int)blockIdx.x);
@@ -136,8 +136,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_simulate(FusedArgs a) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
   __shared__ cf tw[S];
   const int tid = threadIdx.x;
-  Fused<S>::twiddles(tid, LdsPtr{tw});
-  __syncthreads();
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, nullptr, a};
   simulate_position<S>(run, a, a.b0 + (int)blockIdx.x);
