@@ -16,6 +16,7 @@
 //                    recentring; k_error_reduce per-iteration error.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
