@@ -15,7 +15,7 @@ all problems, one graph replay per iteration) against the same solves run one
 after another with reconstruct(); prints aggregate position-updates/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds T]
-                       [--workload production|many|kernels] [--problems P]
+                       [--workload production|many|kernels|learned-init] [--problems P]
 """
 
 from __future__ import annotations
@@ -233,6 +233,48 @@ def bench_kernels(args):
     }), flush=True)
 
 
+def bench_learned_init(args):
+    """Time to object NMSE <= 1e-3, learned start against flat start, on
+    held-out objects of the initialiser's distribution at production shapes
+    (128x128 scan, 128 px frames, 720x720 object)."""
+    from py4dstem_amd import learned_init as li
+
+    net = li.ConvNet.load()
+    seeds = list(range(2001, 2001 + args.problems))
+    per = {}
+    for s in seeds:
+        log(f"learned-init evaluation seed {s}")
+        w = li.evaluation_workload(s)
+        per[s] = li.time_to_nmse(w, net, max_iter=args.steps)
+        for kind in ("flat", "learned"):
+            r = per[s][kind]
+            log(f"  {kind}: iterations {r['iterations']} seconds {r['seconds']} nmse0 {r['nmse'][0]:.3e}")
+    def total(kind):
+        v = [per[s][kind]["seconds"] for s in seeds]
+        return None if any(x is None for x in v) else sum(v)
+    t_flat, t_learned = total("flat"), total("learned")
+    speedup = (t_flat / t_learned) if (t_flat and t_learned) else None
+    out = {
+        "metric": "time-to-NMSE<=1e-3, learned vs flat start",
+        "value": round(speedup, 3) if speedup else None,
+        "unit": "x (flat seconds / learned seconds)",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 network (fp32 accumulation), fp32 (complex64) reconstruction",
+        "data": "synthetic (held-out Gaussian-atom objects, GPU-simulated Poisson patterns)",
+        "config": {"workload": "learned initialiser vs flat start, 128x128 scan, 128 px frames, "
+                               "720x720 object, mini-batch 1024, probe update",
+                   "seeds": seeds},
+        "per_seed": {str(s): {k: {kk: per[s][k][kk] for kk in ("iterations", "seconds", "init_seconds",
+                                                              "s_per_iteration")}
+                              | {"nmse_start": per[s][k]["nmse"][0], "nmse_1": per[s][k]["nmse"][1],
+                                 "used_learned": (per[s][k]["init_info"] or {}).get("used_learned")}
+                              for k in ("flat", "learned")} for s in seeds},
+        "flat_seconds": t_flat, "learned_seconds": t_learned,
+    }
+    print(json.dumps(out, default=float), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,7 +283,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--scan", type=int, default=128)
-    ap.add_argument("--workload", choices=("production", "many", "kernels"), default="production")
+    ap.add_argument("--workload", choices=("production", "many", "kernels", "learned-init"),
+                    default="production")
     ap.add_argument("--problems", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -252,6 +295,8 @@ def main():
         return bench_many(args)
     if args.workload == "kernels":
         return bench_kernels(args)
+    if args.workload == "learned-init":
+        return bench_learned_init(args)
 
     import torch
 

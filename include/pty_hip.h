@@ -139,6 +139,25 @@ int pty_simulate_intensities(const void* probe_hat, const void* object, int Px, 
                              const float* positions, int N, int S, float* out,
                              pty_stream_t stream);
 
+/* ---- preprocessing ------------------------------------------------------- */
+
+/* Per-pattern centre of mass (row, column) of float32 [N][S][S] intensities,
+ * optionally multiplied by mask [S][S] (phase_base_class.py:613-767, the
+ * vectorised branch); float64 accumulation. */
+int pty_pattern_com(const float* intensities, int N, int S, const float* mask, float* com_x,
+                    float* com_y, pty_stream_t stream);
+
+/* Corner-centred amplitudes of the shifted patterns
+ * (phase_base_class.py:1347-1460 with preprocess/utils.py:59-121): output
+ * pattern k comes from input select[k] (NULL: k); it is shifted periodically
+ * and bilinearly by (shift_x, shift_y)[input] (= -fitted CoM) with the
+ * reference's float32 arithmetic, then sqrt(max(I, 0)) is written in the
+ * kernel layout [N_out][S][S]; sums[k] = float64 sum of the shifted pattern
+ * (the host accumulates the mean intensity from them in pattern order). */
+int pty_shift_amplitudes(const float* intensities, int N_in, int S, const int* select, int N_out,
+                         const float* shift_x, const float* shift_y, float* amplitudes,
+                         double* sums, pty_stream_t stream);
+
 /* ---- whole reconstruction loop ------------------------------------------ */
 
 typedef struct pty_problem {

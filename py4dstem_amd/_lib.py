@@ -71,6 +71,9 @@ _SIGNATURES = {
                                    c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pty_simulate_intensities": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                                          c_void_p, c_void_p]),
+    "pty_pattern_com": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pty_shift_amplitudes": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p]),
     "pty_multi_workspace_bytes": (c_size_t, [c_void_p, c_int]),
     "pty_multi_run": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p]),
     "pty_conv_weight_words": (c_size_t, [ctypes.POINTER(PtyConvDesc), c_int]),

@@ -1218,3 +1218,4 @@ int pty_multi_run(const pty_multi_problem* problems, int count, int num_iter, in
 }  // extern "C"
 
 #include "pty_net.inc"
+#include "pty_prep.inc"

@@ -175,3 +175,37 @@ __all__ = ["fft2", "to_kernel_layout", "probe_spectrum", "forward_adjoint", "sca
            "object_update", "probe_update", "object_constraints", "probe_com_constraint",
            "make_problem", "gd_workspace_bytes", "gd_run", "PtyProblem", "PtyGdParams",
            "kernel_index", "num_tiles"]
+
+
+def pattern_com(frames, mask=None):
+    """Per-pattern centre of mass (row, column) of float32 [N, S, S] frames on
+    the device; returns two float32 NumPy arrays."""
+    _req(frames, torch.float32, name="frames")
+    N, S, _ = frames.shape
+    m = None
+    if mask is not None:
+        m = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.float32)).to(frames.device)
+    cx = torch.empty(N, dtype=torch.float32, device=frames.device)
+    cy = torch.empty_like(cx)
+    _lib.call("pty_pattern_com", ptr(frames), N, S, ptr(m), ptr(cx), ptr(cy), stream_handle())
+    return cx.cpu().numpy(), cy.cpu().numpy()
+
+
+def shift_amplitudes(frames, shift_x, shift_y, select=None):
+    """Kernel-layout amplitudes of the bilinearly shifted patterns and the
+    float64 per-pattern sums (see pty_shift_amplitudes)."""
+    check_frame(frames.shape[-1])
+    _req(frames, torch.float32, name="frames")
+    n_in, S, _ = frames.shape
+    dev = frames.device
+    sx = torch.from_numpy(np.ascontiguousarray(shift_x, dtype=np.float32)).to(dev)
+    sy = torch.from_numpy(np.ascontiguousarray(shift_y, dtype=np.float32)).to(dev)
+    if sx.numel() != n_in or sy.numel() != n_in:
+        raise ValueError("one shift per input pattern is required")
+    sel = None if select is None else torch.from_numpy(np.ascontiguousarray(select, dtype=np.int32)).to(dev)
+    n_out = n_in if sel is None else sel.numel()
+    amps = torch.empty((n_out, S, S), dtype=torch.float32, device=dev)
+    sums = torch.empty(n_out, dtype=torch.float64, device=dev)
+    _lib.call("pty_shift_amplitudes", ptr(frames), n_in, S, ptr(sel), n_out, ptr(sx), ptr(sy), ptr(amps),
+              ptr(sums), stream_handle())
+    return amps, sums

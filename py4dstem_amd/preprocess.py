@@ -91,17 +91,6 @@ def real_space_sampling(energy, angular_sampling, frame_shape):
 # process/calibration/origin.py:76-190; process/fit/fit.py:38-128)
 # ---------------------------------------------------------------------------
 
-def intensities_center_of_mass(intensities, dp_mask=None):
-    """Per-pattern CoM (float32 sums, as the vectorised reference branch)."""
-    kx = np.arange(intensities.shape[-2], dtype=np.float32)
-    ky = np.arange(intensities.shape[-1], dtype=np.float32)
-    kya, kxa = np.meshgrid(ky, kx)
-    masked = intensities * dp_mask if dp_mask is not None else intensities
-    total = np.sum(masked, axis=(-2, -1))
-    return (np.sum(masked * kxa[None, None], axis=(-2, -1)) / total,
-            np.sum(masked * kya[None, None], axis=(-2, -1)) / total)
-
-
 def fit_origin_plane(com, mask=None):
     """Least-squares plane through a CoM map on [0,1]-normalised coordinates.
 

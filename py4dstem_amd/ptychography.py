@@ -6,8 +6,10 @@ py4DSTEM's class (py4DSTEM/process/phase/singleslice_ptychography.py:42-996),
 so callers such as ``PtychographyOptimizer``
 (phase/parameter_optimize.py:580-660) can switch by import.
 
-The one-time ``preprocess()`` set-up runs on the host (NumPy/SciPy) plus one
-GPU pass for the probe-overlap field; every ``reconstruct()`` iteration runs
+``preprocess()`` uploads the patterns once; their centre of mass, the
+bilinear shift + sqrt into kernel-layout amplitudes and the probe-overlap
+field run in libpty_hip.so, the small per-scan fits (origin plane, rotation
+search, FOV mask filter) on the host.  Every ``reconstruct()`` iteration runs
 in libpty_hip.so (one native call per run).  Options of the reference that
 this build does not implement raise ``NotImplementedError`` rather than being
 silently ignored.
@@ -29,7 +31,6 @@ from .preprocess import (
     extract_calibrations,
     fit_origin_plane,
     initial_object_shape,
-    intensities_center_of_mass,
     real_space_sampling,
     scan_positions_in_pixels,
     solve_com_rotation,
@@ -55,31 +56,6 @@ def subdivide_into_batches(num_items, max_batch):
         raise RuntimeError("num_batches may not be larger than num_items")
     base, extra = divmod(num_items, num_batches)
     return [base + 1] * extra + [base] * (num_batches - extra)
-
-
-def bilinear_shift_periodic(frames, xshift, yshift):
-    """Per-frame bilinear periodic shift (py4DSTEM/preprocess/utils.py:59-121,
-    bilinear=True): out = sum of four rolled copies with float32 weights."""
-    n, qx, qy = frames.shape
-    xF = np.floor(xshift).astype(int)
-    yF = np.floor(yshift).astype(int)
-    wx = (xshift - xF).astype(np.float32)
-    wy = (yshift - yF).astype(np.float32)
-    ix = np.arange(qx)
-    iy = np.arange(qy)
-    b = np.arange(n)[:, None, None]
-
-    def rolled(dx, dy):
-        rows = (ix[None, :] - (xF + dx)[:, None]) % qx
-        cols = (iy[None, :] - (yF + dy)[:, None]) % qy
-        return frames[b, rows[:, :, None], cols[:, None, :]]
-
-    one = np.float32(1)
-    w00 = ((one - wx) * (one - wy))[:, None, None]
-    w10 = (wx * (one - wy))[:, None, None]
-    w01 = ((one - wx) * wy)[:, None, None]
-    w11 = (wx * wy)[:, None, None]
-    return rolled(0, 0) * w00 + rolled(1, 0) * w10 + rolled(0, 1) * w01 + rolled(1, 1) * w11
 
 
 class SingleslicePtychography:
@@ -270,6 +246,11 @@ class SingleslicePtychography:
         if self._semiangle_cutoff_pixels:
             self._semiangle_cutoff = self._semiangle_cutoff_pixels * self._angular_sampling[0]
 
+        # the patterns go to the GPU once; centre of mass and normalisation
+        # run there (phase_base_class.py:613-767, 1347-1460)
+        dev = require_gpu()
+        frames_dev = torch.from_numpy(np.ascontiguousarray(intensities).reshape(-1, S, S)).to(dev)
+
         # centre of mass and origin fit (phase_base_class.py:613-767)
         if force_com_measured:
             com_mx = np.asarray(force_com_measured[0], dtype=np.float32)
@@ -278,7 +259,9 @@ class SingleslicePtychography:
             if dp_mask is not None and dp_mask.shape != intensities.shape[-2:]:
                 raise ValueError(f"Mask shape should be (Qx,Qy):{intensities.shape[-2:]}, "
                                  f"not {dp_mask.shape}")
-            com_mx, com_my = intensities_center_of_mass(intensities, dp_mask)
+            com_mx, com_my = K.pattern_com(frames_dev, dp_mask)
+            com_mx = com_mx.reshape(intensities.shape[:2])
+            com_my = com_my.reshape(intensities.shape[:2])
         if force_com_shifts is None:
             if fit_function is not None:
                 finite = np.isfinite(com_mx)
@@ -300,15 +283,16 @@ class SingleslicePtychography:
             self._com_normalized_x, self._com_normalized_y, rotation_angles_deg,
             maximize_divergence, force_com_rotation, force_com_transpose, self._verbose)
 
-        # corner-centred amplitudes (phase_base_class.py:1347-1460)
-        self._amplitudes, self._mean_diffraction_intensity = self._normalize_intensities(
-            intensities, com_fx, com_fy)
+        # corner-centred amplitudes (phase_base_class.py:1347-1460), kernel layout
+        self._amplitudes_dev, self._mean_diffraction_intensity = self._normalize_intensities(
+            frames_dev, com_fx, com_fy)
+        self._amplitudes_host = None
         self._crop_mask, self._crop_mask_shape = None, intensities.shape[-2:]
-        del intensities
-        self._num_diffraction_patterns = self._amplitudes.shape[0]
-        self._amplitudes_shape = np.array(self._amplitudes.shape[-2:])
+        del intensities, frames_dev
+        self._num_diffraction_patterns = self._amplitudes_dev.shape[0]
+        self._amplitudes_shape = np.array([S, S])
         self._resample_exit_waves = False
-        self._region_of_interest_shape = np.array(self._amplitudes.shape[-2:])
+        self._region_of_interest_shape = np.array([S, S])
 
         # positions, object, probe
         pos, self._object_padding_px = scan_positions_in_pixels(
@@ -347,11 +331,7 @@ class SingleslicePtychography:
         else:
             self._probe_initial_aperture = None
 
-        # device-resident data for the loop
-        dev = require_gpu()
-        amps = torch.from_numpy(np.ascontiguousarray(self._amplitudes)).to(dev)
-        self._amplitudes_dev = K.to_kernel_layout(amps)
-        del amps
+        # device-resident positions for the loop (amplitudes are already there)
         self._positions_dev = torch.from_numpy(self._positions_px.copy()).to(dev)
 
         # probe-overlap field and FOV mask (singleslice_ptychography.py:504-537)
@@ -371,29 +351,33 @@ class SingleslicePtychography:
                 delattr(self, attr)
         return self
 
-    def _normalize_intensities(self, intensities, com_fx, com_fy, chunk=1024):
-        """Shift each pattern by -CoM (bilinear, periodic), accumulate the
-        float32 mean intensity pattern by pattern, take sqrt(max(I, 0))."""
-        rx, ry, qx, qy = intensities.shape
-        flat = intensities.reshape(-1, qx, qy)
+    def _normalize_intensities(self, frames_dev, com_fx, com_fy):
+        """Shift each pattern by -CoM (bilinear, periodic), take
+        sqrt(max(I, 0)) into the kernel layout on the GPU; the float32 mean
+        intensity is accumulated pattern by pattern in scan order, as the
+        reference loop does."""
+        n_in, S, _ = frames_dev.shape
         fx = (-np.asarray(com_fx, dtype=np.float32)).reshape(-1)
         fy = (-np.asarray(com_fy, dtype=np.float32)).reshape(-1)
-        keep = (np.ones(rx * ry, dtype=bool) if self._positions_mask is None
-                else self._positions_mask.reshape(-1))
-        out = np.empty((int(keep.sum()), qx, qy), dtype=np.float32)
+        sel = None
+        if self._positions_mask is not None:
+            sel = np.nonzero(self._positions_mask.reshape(-1))[0].astype(np.int32)
+        amps, sums = K.shift_amplitudes(frames_dev, fx, fy, sel)
         mean_intensity = np.float32(0)
-        k = 0
-        for s in range(0, rx * ry, chunk):
-            sel = np.nonzero(keep[s:s + chunk])[0] + s
-            if sel.size == 0:
-                continue
-            shifted = bilinear_shift_periodic(flat[sel], fx[sel], fy[sel])
-            for v in np.sum(shifted, axis=(-2, -1)):
-                mean_intensity += v
-            out[k:k + sel.size] = np.sqrt(np.maximum(shifted, 0))
-            k += sel.size
-        mean_intensity /= out.shape[0]
-        return out, mean_intensity
+        for v in sums.cpu().numpy().astype(np.float32):
+            mean_intensity += v
+        mean_intensity /= amps.shape[0]
+        return amps, mean_intensity
+
+    @property
+    def _amplitudes(self):
+        """Host copy of the corner-centred amplitudes [N][S][S] (natural
+        order), downloaded from the device on first use."""
+        if getattr(self, "_amplitudes_host", None) is None:
+            if not hasattr(self, "_amplitudes_dev"):
+                raise AttributeError("_amplitudes: preprocess() has not run")
+            self._amplitudes_host = K.to_kernel_layout(self._amplitudes_dev, inverse=True).cpu().numpy()
+        return self._amplitudes_host
 
     def _initialize_probe(self, initial_probe):
         """ptychographic_methods.py:1011-1102."""

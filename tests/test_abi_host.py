@@ -72,17 +72,6 @@ def test_product_positions_match_oracle(small_case):
     o._rotation_best_rad, o._rotation_best_transpose = 0.0, False
 
 
-def test_product_normalisation_matches_oracle():
-    from py4dstem_amd.ptychography import bilinear_shift_periodic
-    rng = np.random.default_rng(3)
-    frames = rng.uniform(0, 50, (9, 32, 32)).astype(np.float32)
-    sx = (-rng.uniform(10, 20, 9)).astype(np.float32)
-    sy = (-rng.uniform(10, 20, 9)).astype(np.float32)
-    got = bilinear_shift_periodic(frames, sx, sy)
-    for n in range(9):
-        np.testing.assert_array_equal(got[n], po.get_shifted_ar(frames[n], sx[n], sy[n]))
-
-
 def test_product_rotation_search_matches_closed_form():
     """A CoM field that is a pure gradient rotated by 30 degrees has zero
     curl at -30 degrees after derotation."""
