@@ -309,19 +309,39 @@ __device__ float block_max(float v, float* sh) {
   return m;
 }
 
-// 1 / sqrt(1e-16 + (c1 s)^2 + (c2 m)^2) in float32, without contraction
+// Correctly rounded float32 square root of x >= 0 (IEEE / NumPy semantics).
+// The hardware estimate r is within one ulp.  The midpoints between r and its
+// float neighbours have at most 25 significant bits, so their squares are
+// exact in float64; comparing x with them selects the correctly rounded
+// result (x, a float, can never equal such a square).  x is 0 or normal here.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  if (!(x > 0.f)) return x == 0.f ? x : 0.f;
+  const float r = __builtin_amdgcn_sqrtf(x);
+  const float lo = __uint_as_float(__float_as_uint(r) - 1u);
+  const float hi = __uint_as_float(__float_as_uint(r) + 1u);
+  const double m_lo = ((double)r + (double)lo) * 0.5, m_hi = ((double)r + (double)hi) * 0.5;
+  const double xd = (double)x;
+  if (xd < m_lo * m_lo) return lo;
+  if (xd > m_hi * m_hi) return hi;
+  return r;
+}
+
+// 1 / sqrt(1e-16 + (c1 s)^2 + (c2 m)^2) in float32, every operation rounded
+// (contraction off, correctly rounded sqrt and division)
 __device__ __forceinline__ float overlap_norm(float s, float m, float c1, float c2) {
-  const float a = __fmul_rn(c1, s);
-  float t = __fadd_rn(1e-16f, __fmul_rn(a, a));
-  const float b = __fmul_rn(c2, m);
-  t = __fadd_rn(t, __fmul_rn(b, b));
-  return 1.0f / sqrtf(t);
+#pragma clang fp contract(off)
+  const float a = c1 * s;
+  float t = 1e-16f + a * a;
+  const float b = c2 * m;
+  t = t + b * b;
+  return 1.0f / sqrt_rn(t);
 }
 
 __device__ __forceinline__ cf step_add(cf o, cf g, float nrm, float step) {
-  const float ux = __fmul_rn(step, __fmul_rn(g.x, nrm));
-  const float uy = __fmul_rn(step, __fmul_rn(g.y, nrm));
-  return mkcf(__fadd_rn(o.x, ux), __fadd_rn(o.y, uy));
+#pragma clang fp contract(off)
+  const float ux = step * (g.x * nrm);
+  const float uy = step * (g.y * nrm);
+  return mkcf(o.x + ux, o.y + uy);
 }
 
 __device__ __forceinline__ void object_update_body(cf* __restrict__ obj, const cf* __restrict__ grad,
