@@ -10,9 +10,9 @@ reconstruction) into a starting object.  The iterative solve remains the
 judge: ``initialize_object`` keeps the learned start only when its data error
 is below the flat object's, otherwise the flat start is used.
 
-Layers (Ci -> Co, stride, dilation): 9(16) -> 64 /1 d1, 64 -> 64 /2 d1,
-64 -> 64 d2, d4, d8 (receptive field spans the whole 32x32 patch), 64 -> 2(16)
-d1 linear.  ReLU after layers 1-5.
+Layers (Ci -> Co, stride, dilation): 9(16) -> 64 /2 d1 (64x64 -> 32x32),
+64 -> 64 d1, d2, d4, d8 (receptive field spans the whole 32x32 patch),
+64 -> 2(16) d1 linear.  ReLU after layers 1-5.
 
 Training data come from the forward model itself (the HIP simulation kernel):
 random Gaussian-atom objects (sigma 0.3 A) on random 2-4 A square lattices
@@ -37,7 +37,7 @@ from . import kernels as K
 from ._lib import PtyConvDesc, ptr, require_gpu, stream_handle
 
 # (Ci, Co, stride, dilation, relu)
-ARCH = ((16, 64, 1, 1, True), (64, 64, 2, 1, True), (64, 64, 1, 2, True),
+ARCH = ((16, 64, 2, 1, True), (64, 64, 1, 1, True), (64, 64, 1, 2, True),
         (64, 64, 1, 4, True), (64, 64, 1, 8, True), (64, 16, 1, 1, False))
 IN_HW, OUT_HW = 64, 32
 REAL_IN, REAL_OUT = 9, 2
@@ -47,8 +47,8 @@ DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
 
 
 def _hw(li):
-    """(Hi, Ho) of layer li."""
-    return (IN_HW, IN_HW) if li == 0 else ((IN_HW, OUT_HW) if li == 1 else (OUT_HW, OUT_HW))
+    """(Hi, Ho) of layer li: the first layer halves 64 -> 32."""
+    return (IN_HW, OUT_HW) if li == 0 else (OUT_HW, OUT_HW)
 
 
 def conv_desc(li, n):
@@ -381,14 +381,23 @@ def train(num_scans=20000, scans_per_chunk=16, samples_per_chunk=1024, batch=256
 
 
 # -- initialising a reconstruction -------------------------------------------------
-def _data_error(p, obj, probe, order, chunk=1024):
-    from .autograd import _loss_call
-    loss, _, _ = _loss_call(obj, probe, p._positions_dev, p._amplitudes_dev, order, chunk, 1.0,
-                            False, False)
-    return float(loss.item())
+def _data_errors(p, objects, probe, order):
+    """Data error sum_b sum (|F_b| - A_b)^2 over `order` for each object (one
+    workspace, one synchronisation)."""
+    S = probe.shape[-1]
+    Px, Py = objects[0].shape
+    n = order.numel()
+    ws_bytes = _lib.load().pty_loss_workspace_bytes(S, Px, Py, n, n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=probe.device)
+    losses = torch.empty(len(objects), dtype=torch.float64, device=probe.device)
+    for i, obj in enumerate(objects):
+        _call("pty_loss_gradients", ptr(obj), ptr(probe), Px, Py, ptr(p._positions_dev),
+              p._positions_dev.shape[0], ptr(p._amplitudes_dev), ptr(order), n, n, S, 1.0,
+              ptr(losses[i:i + 1]), None, None, ptr(ws), ws.numel(), stream_handle())
+    return losses.cpu().numpy().tolist()
 
 
-def initialize_object(p, net=None, stride=2, judge_positions=2048, seed=0):
+def initialize_object(p, net=None, stride=4, judge_positions=512, seed=0):
     """Learned starting object for a preprocessed SingleslicePtychography
     with a raster scan: network on every `stride`-th position per scan axis,
     stitched; kept only if its data error over `judge_positions` seeded
@@ -411,17 +420,17 @@ def initialize_object(p, net=None, stride=2, judge_positions=2048, seed=0):
     centres = torch.from_numpy((ij[:, 0] * ny + ij[:, 1]).astype(np.int64)).to(dev)
     nbr = torch.from_numpy(neighbour_table((nx, ny), ij)).to(dev)
     x = net_inputs(p._amplitudes_dev, 1, nbr)
-    y = net.predict(x)
+    y = net.predict(x, chunk=8192)
     learned = stitch(y, p._positions_dev[centres], tuple(p._object_shape))
     probe = torch.from_numpy(np.ascontiguousarray(p._probe_initial, dtype=np.complex64)).to(dev)
     rng = np.random.default_rng(seed)
     n = min(judge_positions, p._num_diffraction_patterns)
     order = torch.from_numpy(rng.choice(p._num_diffraction_patterns, n, replace=False)
                              .astype(np.int32)).to(dev)
-    flat = torch.from_numpy(np.ascontiguousarray(p._object_initial, dtype=np.complex64)).to(dev)
-    e_learned = _data_error(p, learned, probe, order)
-    e_flat = _data_error(p, flat, probe, order)
-    torch.cuda.synchronize()
+    flat = getattr(p, "_object_initial_dev", None)
+    if flat is None:
+        flat = torch.from_numpy(np.ascontiguousarray(p._object_initial, dtype=np.complex64)).to(dev)
+    e_learned, e_flat = _data_errors(p, (learned, flat), probe, order)
     use = e_learned < e_flat
     info = {"seconds": time.perf_counter() - t0, "error_learned": e_learned, "error_flat": e_flat,
             "patches": int(len(ij)), "used_learned": bool(use)}
@@ -480,6 +489,12 @@ def time_to_nmse(w, net=None, max_iter=12, target=1e-3, seed=0):
     from .synthetic import object_nmse
 
     out = {}
+    # one untimed pass so that kernel code objects are loaded before timing
+    p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+    p.preprocess(**w["preprocess"])
+    p.initialize_object_learned(net)
+    p.reconstruct(num_iter=1, reset=True, seed_random=seed, **w["reconstruct"])
+    torch.cuda.synchronize()
     for kind in ("flat", "learned"):
         p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
         p.preprocess(**w["preprocess"])

@@ -102,13 +102,11 @@ class MultiPtychography:
         cands = [(b, s) for b in batch_candidates for s in stage_candidates if s == 0 or s < b]
         best, timings = autotune.best_of(cands, trial, repeats=1)
         for p, (o, pr, errs, had) in zip(self.items, saved):
-            if o is not None:
-                p._object_dev = o
-                p._probe_dev = pr
-            else:
-                for a in ("_object_dev", "_probe_dev"):
-                    if hasattr(p, a):
-                        delattr(p, a)
+            for attr, val in (("_object_dev", o), ("_probe_dev", pr)):
+                if val is not None:
+                    setattr(p, attr, val)
+                elif hasattr(p, attr):
+                    delattr(p, attr)
             p.error_iterations = errs
             if not had and hasattr(p, "error"):
                 del p.error

@@ -309,6 +309,7 @@ class SingleslicePtychography:
         if self._object.shape[0] < S or self._object.shape[1] < S:
             raise ValueError(f"object {self._object.shape} is smaller than the {S}x{S} probe window")
         if store_initial_arrays:
+            self._object_initial_dev = None
             self._object_initial = self._object.copy()
             self._object_type_initial = self._object_type
         self._object_shape = self._object.shape
@@ -346,9 +347,13 @@ class SingleslicePtychography:
             self._object_fov_mask = np.asarray(object_fov_mask)
         self._object_fov_mask_inverse = np.invert(self._object_fov_mask)
         self._preprocessed = True
-        for attr in ("_object_dev", "_probe_dev"):
-            if hasattr(self, attr):
-                delattr(self, attr)
+        if hasattr(self, "_probe_dev"):
+            del self._probe_dev
+        # the starting object stays resident: resets and the learned
+        # initialiser's judge use device copies
+        self._object_dev = torch.from_numpy(np.ascontiguousarray(self._object, dtype=np.complex64)).to(dev)
+        if store_initial_arrays:
+            self._object_initial_dev = self._object_dev.clone()
         return self
 
     def _normalize_intensities(self, frames_dev, com_fx, com_fy):
@@ -419,7 +424,7 @@ class SingleslicePtychography:
         return overlap.cpu().numpy()
 
     # -- learned starting object ----------------------------------------------------
-    def initialize_object_learned(self, model=None, stride=2, judge_positions=2048):
+    def initialize_object_learned(self, model=None, stride=4, judge_positions=512):
         """Replace the starting object with the learned initialiser's stitched
         prediction (py4dstem_amd.learned_init) when its data error on
         `judge_positions` seeded positions is below the current starting
@@ -437,9 +442,11 @@ class SingleslicePtychography:
         if used:
             self._object = obj.cpu().numpy()
             self._object_initial = self._object.copy()
-            for attr in ("_object_dev", "_probe_dev"):
-                if hasattr(self, attr):
-                    delattr(self, attr)
+            # the next run (and any reset) starts from the device copy
+            self._object_initial_dev = obj
+            self._object_dev = obj.clone()
+            if hasattr(self, "_probe_dev"):
+                del self._probe_dev
         return self
 
     # -- execution tuning (results do not depend on it) --------------------------
@@ -472,6 +479,9 @@ class SingleslicePtychography:
             for attr in ("_object_dev", "_probe_dev"):
                 if hasattr(self, attr):
                     delattr(self, attr)
+            init_dev = getattr(self, "_object_initial_dev", None)
+            if init_dev is not None:
+                self._object_dev = init_dev.clone()  # same values as _object_initial
         elif reset is None:
             if hasattr(self, "error"):
                 warnings.warn("Continuing reconstruction from previous result. "
@@ -607,6 +617,7 @@ class SingleslicePtychography:
         if not hasattr(self, "_object_dev"):
             self._object_dev = torch.from_numpy(np.ascontiguousarray(self._object,
                                                                      dtype=np.complex64)).to(dev)
+        if not hasattr(self, "_probe_dev"):
             self._probe_dev = torch.from_numpy(np.ascontiguousarray(self._probe,
                                                                     dtype=np.complex64)).to(dev)
         use_mask = fix_potential_baseline and self._object_fov_mask_inverse.sum() > 0

@@ -20,7 +20,8 @@ def test_neighbour_table_is_clamped_3x3_raster():
 
 
 def test_architecture_receptive_field_covers_the_patch():
-    # receptive field on the 32x32 output grid after the stride-2 layer
+    # receptive field on the 32x32 output grid after the stride-2 first layer
+    assert ARCH[0][2] == 2
     rf = 1
     for ci, co, stride, dil, _ in ARCH[1:]:
         rf += 2 * dil
