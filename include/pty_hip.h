@@ -261,8 +261,10 @@ int pty_conv_pack_weights(const pty_conv_desc* d, const float* w, unsigned short
  * sample mean; channels 9..15 zero.  frames: counts, natural corner-centred
  * order (kernel_layout = 0) or amplitudes in the kernel layout (counts = A^2,
  * kernel_layout = 1).  S = 64 or 128. */
+size_t pty_net_inputs_workspace_bytes(int count);
 int pty_net_inputs(const float* frames, int N, int S, int kernel_layout, const int* neighbours,
-                   int count, unsigned short* x, pty_stream_t stream);
+                   int count, unsigned short* x, void* workspace, size_t workspace_bytes,
+                   pty_stream_t stream);
 /* loss = mean over [count][32][32][2] of (y - target)^2, y = channels 0 (amplitude)
  * and 1 (phase) of the last layer's f32 output [count][32][32][16];
  * dy (bf16, 16 channels) = d loss / d y */

@@ -89,7 +89,9 @@ _SIGNATURES = {
                               c_void_p, c_void_p, c_void_p, c_void_p]),
     "pty_conv_pack_weights": (c_int, [ctypes.POINTER(PtyConvDesc), c_void_p, c_void_p, c_void_p,
                                       c_void_p]),
-    "pty_net_inputs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "pty_net_inputs_workspace_bytes": (c_size_t, [c_int]),
+    "pty_net_inputs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                               c_size_t, c_void_p]),
     "pty_net_loss_workspace_bytes": (c_size_t, [c_int]),
     "pty_net_loss": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_size_t,
                              c_void_p]),

@@ -226,8 +226,10 @@ def net_inputs(frames, kernel_layout, nbr):
     N, S, _ = frames.shape
     n = nbr.shape[0]
     x = _bf16((n, IN_HW, IN_HW, 16), frames.device)
-    _call("pty_net_inputs", ptr(frames), N, S, int(kernel_layout), ptr(nbr), n, ptr(x),
-          stream_handle())
+    nbytes = _lib.load().pty_net_inputs_workspace_bytes(n)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=frames.device)
+    _call("pty_net_inputs", ptr(frames), N, S, int(kernel_layout), ptr(nbr), n, ptr(x), ptr(ws),
+          nbytes, stream_handle())
     return x
 
 
