@@ -192,6 +192,50 @@ size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* p
 int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const int* orders,
                double* errors, void* workspace, size_t workspace_bytes, pty_stream_t stream);
 
+/* ---- position correction ---------------------------------------------------
+ * reconstruct(fix_positions=False): after each mini-batch's object/probe
+ * update, phase/ptychographic_methods.py:2071-2225 (_position_correction,
+ * called at phase/singleslice_ptychography.py:913-926) moves every position
+ * of the batch by the least-squares step of the measured-minus-estimated
+ * intensities on the two one-pixel object-shift partials; at the end of each
+ * iteration phase/ptychographic_constraints.py:1292-1311 optionally restores
+ * the scan's centre of mass.  Limits are in object pixels (the reference's
+ * Angstrom values divided by |sampling|). */
+typedef struct pty_position_params {
+  float* positions;               /* [N][2] = the problem's positions, updated in place */
+  const float* positions_initial; /* [N][2] positions at the start of the run */
+  float step_size;                /* positions_step_size */
+  int clip_update;                /* max_position_update_distance given */
+  double max_update;
+  int clip_total;                 /* max_position_total_distance given */
+  double max_total;
+  int center_of_mass;             /* 1: positions -= mean(positions) - initial_com each iteration */
+  float initial_com[2];           /* float32 mean of positions_initial */
+  int* singular;                  /* device int or NULL: +1 per singular 2x2 system (the
+                                     reference raises LinAlgError); the update is then 0 */
+} pty_position_params;
+
+size_t pty_gd_positions_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);
+
+/* pty_gd_run with position correction (params->stage is ignored: every fused
+ * launch covers the whole mini-batch). */
+int pty_gd_run_positions(const pty_problem* problem, const pty_gd_params* params,
+                         const pty_position_params* positions, const int* orders, double* errors,
+                         void* workspace, size_t workspace_bytes, pty_stream_t stream);
+
+/* One batch of _position_correction (phase/ptychographic_methods.py:2071-2225):
+ * ps = the batch's shifted probes and far = fft2(P_shift * O_patch) of the
+ * forward pass (pty_forward_adjoint_batch's outputs), object = the object
+ * after the update; updates pp->positions[order[b]].  sums (optional, device,
+ * [B][5] float64): sum p_x^2, p_x p_y, p_y^2, p_x d, p_y d per slot. */
+int pty_position_correction_batch(const pty_position_params* pp, const void* ps, const void* far,
+                                  const void* object, int Px, int Py, const float* amplitudes, int N,
+                                  const int* order, int B, int S, double* sums, pty_stream_t stream);
+
+/* positions -= float32 mean(positions) - initial_com
+ * (phase/ptychographic_constraints.py:1292-1311); initial_com is a host pointer. */
+int pty_positions_com(float* positions, int N, const float* initial_com, pty_stream_t stream);
+
 /* ---- many reconstructions together --------------------------------------- */
 
 /* One entry per independent reconstruction; params.num_iter is ignored. */

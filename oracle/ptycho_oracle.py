@@ -40,6 +40,12 @@ PINNING = {
     "loss_gradients": "FP64 central finite differences (object and probe pixels)",
     "object_constraints": "mean phase outside FOV is 0 after shrinkage; |O|<=1",
     "probe_com_constraint": "probe CoM ~ 0 after recentring",
+    "position_partials": "brute-force DFT loops on tiny frames with np.roll'ed objects",
+    "position_correction": "exact recovery of a planted least-squares step; clipping "
+                           "closed forms; sign/magnitude on a sub-pixel displaced scan",
+    "positions_constraints": "scan centre of mass restored exactly; exact affine maps "
+                             "recovered (with outliers) by the RANSAC fit",
+    "affine_ransac": "exact recovery of planted affine maps with 10% outliers",
     "reconstruct": "composition of the above; c64 vs c128 sensitivity measured",
 }
 
@@ -367,6 +373,93 @@ def scatter_bincount(patches, positions_px, roi_shape, obj_shape):
 # Minimal data model used by the oracle (duck-typed like py4DSTEM's DataCube)
 # ---------------------------------------------------------------------------
 
+# ---------------------------------------------------------------------------
+# Global affine model of the scan positions (phase/utils.py:595-825)
+# ---------------------------------------------------------------------------
+# A transform is the tuple (scale0, scale1, shear1, angle, t0, t1); the
+# identity keeps Python floats, a fitted one float32 translations, as the
+# reference's AffineTransform does (the dtype of the result follows from it).
+AFFINE_IDENTITY = (1.0, 1.0, 0.0, 0.0, 0.0, 0.0)
+
+
+def affine_matrix(tf):
+    """scale @ shear @ rotation, each a float32 2x2 (utils.py:673-702)."""
+    scale0, scale1, shear1, angle = tf[:4]
+    c, s = np.cos(angle), np.sin(angle)
+    scale = np.array([[scale0, 0.0], [0.0, scale1]], dtype="float32")
+    shear = np.array([[1.0, 0.0], [shear1, 1.0]], dtype="float32")
+    rot = np.array([[+c, -s], [+s, +c]], dtype="float32")
+    return scale @ shear @ rot
+
+
+def affine_apply(tf, x, origin=(0, 0)):
+    """utils.py:726-731: ((x - origin) @ M) + (t + origin)."""
+    origin = np.asarray(origin, dtype=np.float32)
+    M = affine_matrix(tf).astype(np.float32)
+    return ((x - origin) @ M) + (np.array((tf[4], tf[5])) + origin)
+
+
+def affine_from_matrix(T):
+    """Graphics Gems II 7.1 decomposition of a 3x2 (or 2x2) least-squares
+    matrix (utils.py:639-671); a degenerate row gives the identity."""
+    R = T[:2, :2].copy()
+    scale0 = np.linalg.norm(R[0])
+    if scale0 <= 0:
+        return AFFINE_IDENTITY
+    R[0] /= scale0
+    shear1 = R[0] @ R[1]
+    R[1] -= shear1 * R[0]
+    scale1 = np.linalg.norm(R[1])
+    if scale1 <= 0:
+        return AFFINE_IDENTITY
+    R[1] /= scale1
+    shear1 /= scale1
+    angle = np.arccos(R[0, 0])
+    t0, t1 = T[2] if T.shape[0] > 2 else (0.0, 0.0)
+    return (float(scale0), float(scale1), float(shear1), float(angle), t0, t1)
+
+
+def affine_fit(positions0, positions1, origin=(0, 0), translation_allowed=True):
+    """Least-squares affine map positions0 -> positions1 and its residual
+    norm (utils.py:743-770)."""
+    origin = np.asarray(origin, dtype=np.float32)
+    try:
+        if translation_allowed:
+            a = np.pad(positions0 - origin, ((0, 0), (0, 1)), constant_values=1)
+        else:
+            a = positions0 - origin
+        b = positions1 - origin
+        aT = a.conj().swapaxes(-1, -2)
+        x = np.linalg.inv(aT @ a) @ aT @ b
+        tf = affine_from_matrix(x)
+    except np.linalg.LinAlgError:
+        tf = AFFINE_IDENTITY
+    error = np.linalg.norm(affine_apply(tf, positions0, origin) - positions1)
+    return tf, error
+
+
+def affine_ransac(positions0, positions1, origin=(0, 0), translation_allowed=True,
+                  min_sample=64, max_error=16, min_consensus=0.75, max_iter=20):
+    """RANSAC over `max_iter` random subsets drawn from the global NumPy
+    generator (utils.py:773-825); returns (best transform, its residual)."""
+    best_fitness = np.inf
+    transform = AFFINE_IDENTITY
+    for subset in np.random.choice(a=len(positions0), size=(max_iter, min_sample), replace=True):
+        subset = np.unique(subset)
+        candidate, _ = affine_fit(positions0[subset], positions1[subset], origin,
+                                  translation_allowed)
+        position_error = np.linalg.norm(affine_apply(candidate, positions0, origin) - positions1,
+                                        axis=-1)
+        inliers = position_error <= max_error
+        if np.sum(inliers) / len(inliers) >= min_consensus:
+            candidate, fitness = affine_fit(positions0[inliers], positions1[inliers], origin,
+                                            translation_allowed)
+            if fitness < best_fitness:
+                best_fitness = fitness
+                transform = candidate
+    return transform, best_fitness
+
+
 class _Cal:
     def __init__(self, R_size, R_units, Q_size, Q_units):
         self._r = (R_size, R_units)
@@ -685,6 +778,61 @@ class OracleSingleslicePtychography:
         g_probe = -2.0 * S2 * np.sum(back, axis=0)
         return err, g_obj, g_probe
 
+    # -- position correction ------------------------------------------------
+    def position_partials(self, obj, positions_px, shifted_probes, overlap, amplitudes):
+        """Intensity residual and the two finite-difference intensity partials
+        of ``_position_correction`` (ptychographic_methods.py:2117-2186):
+        F = fft2(overlap), d = A^2 - |F|^2 and, for the object patch moved one
+        pixel along each axis, 2 Re((F - F_shifted) conj(F)).  Returns
+        (d, partial_x, partial_y), each [B, S*S]."""
+        mode = self._mode
+        rows, cols = patch_indices(positions_px, self._region_of_interest_shape, obj.shape)
+        overlap_fft = _fft2(overlap, mode)
+        overlap_fft_conj = np.conj(overlap_fft)
+        estimated_intensity = np.abs(overlap_fft) ** 2
+        measured_intensity = amplitudes ** 2
+        flat_shape = (measured_intensity.shape[0], -1)
+        difference_intensity = (measured_intensity - estimated_intensity).reshape(flat_shape)
+        overlap_dx = shifted_probes * obj[(rows + 1) % obj.shape[0], cols]
+        overlap_dy = shifted_probes * obj[rows, (cols + 1) % obj.shape[1]]
+        overlap_dx_fft = overlap_fft - _fft2(overlap_dx, mode)
+        overlap_dy_fft = overlap_fft - _fft2(overlap_dy, mode)
+        partial_dx = 2 * np.real(overlap_dx_fft * overlap_fft_conj)
+        partial_dy = 2 * np.real(overlap_dy_fft * overlap_fft_conj)
+        return (difference_intensity, partial_dx.reshape(flat_shape),
+                partial_dy.reshape(flat_shape))
+
+    def position_correction(self, obj, positions_px, shifted_probes, overlap, amplitudes,
+                            positions_px_initial, positions_step_size,
+                            max_position_update_distance=None, max_position_total_distance=None):
+        """``_position_correction`` (ptychographic_methods.py:2071-2225) for a
+        single-mode probe without resampling: per position the least-squares
+        step u = (A^T A)^-1 A^T d with A = [partial_x, partial_y], scaled by
+        the step size, optionally clipped, subtracted from the positions.
+        `positions_px` is the batch's copy and is updated in place."""
+        diff, pdx, pdy = self.position_partials(obj, positions_px, shifted_probes, overlap,
+                                                amplitudes)
+        coefficients_matrix = np.dstack((pdx, pdy))
+        coefficients_matrix_T = coefficients_matrix.conj().swapaxes(-1, -2)
+        positions_update = (np.linalg.inv(coefficients_matrix_T @ coefficients_matrix)
+                            @ coefficients_matrix_T @ diff[..., None])
+        positions_update = positions_update[..., 0] * positions_step_size
+        sampling = self.sampling
+        if max_position_update_distance is not None:
+            max_position_update_distance /= np.sqrt(sampling[0] ** 2 + sampling[1] ** 2)
+            update_norms = np.linalg.norm(positions_update, axis=1)
+            outlier_ind = update_norms > max_position_update_distance
+            positions_update[outlier_ind] /= (update_norms[outlier_ind, None]
+                                              / max_position_update_distance)
+        if max_position_total_distance is not None:
+            max_position_total_distance /= np.sqrt(sampling[0] ** 2 + sampling[1] ** 2)
+            deltas = (positions_px - positions_px_initial) - positions_update
+            dsts = np.linalg.norm(deltas, axis=1)
+            outlier_ind = dsts > max_position_total_distance
+            positions_update[outlier_ind] = 0
+        positions_px -= positions_update
+        return positions_px
+
     # -- constraints --------------------------------------------------------
     @staticmethod
     def object_constraints(obj, object_mask, shrinkage_rad=0.0, pure_phase_object=False):
@@ -707,12 +855,32 @@ class OracleSingleslicePtychography:
         x0, y0 = get_CoM(intensity, corner_centered=True)
         return fft_shift(probe, -np.array([x0, y0]), self._mode)
 
+    @staticmethod
+    def positions_constraints(current_positions, initial_positions, fix_positions,
+                              fix_positions_com, global_affine_transformation):
+        """ptychographic_constraints.py:1292-1377.  `fix_positions_com` is the
+        flag as the reconstruct loop passes it (its own flag and not
+        fix_positions, singleslice_ptychography.py:956): the centre-of-mass
+        step runs when that flag is False."""
+        if not fix_positions:
+            if not fix_positions_com:
+                current_positions -= current_positions.mean(0) - initial_positions.mean(0)
+            if global_affine_transformation:
+                origin = initial_positions.mean(0)
+                tf, _ = affine_ransac(initial_positions, current_positions, origin,
+                                      min_sample=initial_positions.shape[0] // 10)
+                current_positions = affine_apply(tf, initial_positions, origin)
+        return current_positions
+
     # -- reconstruct --------------------------------------------------------
     def reconstruct(self, num_iter=8, max_batch_size=None, seed_random=None,
                     step_size=0.5, normalization_min=1, fix_probe=False,
                     fix_probe_com=True, fix_potential_baseline=True,
                     shrinkage_rad=0.0, pure_phase_object=False,
-                    store_iterations=False, reset=None, **kwargs):
+                    store_iterations=False, reset=None, positions_step_size=0.5,
+                    fix_positions=True, fix_positions_com=True,
+                    max_position_update_distance=None, max_position_total_distance=None,
+                    global_affine_transformation=False, **kwargs):
         """phase/singleslice_ptychography.py:606-996 for GD."""
         # ptychographic_methods.py:252-294
         if store_iterations and (not hasattr(self, "object_iterations") or reset):
@@ -754,12 +922,20 @@ class OracleSingleslicePtychography:
                 self._object, self._probe = self.adjoint(
                     self._object, self._probe, op, sp, pos, ew, step_size,
                     normalization_min, fix_probe)
+                if not fix_positions:  # singleslice_ptychography.py:913-926
+                    self._positions_px[idx] = self.position_correction(
+                        self._object, pos, sp, sp * op, self._amplitudes[idx],
+                        self._positions_px_initial[idx], positions_step_size,
+                        max_position_update_distance, max_position_total_distance)
                 error += err
             error /= self._mean_diffraction_intensity * N
             self._object = self.object_constraints(self._object, object_mask,
                                                    shrinkage_rad, pure_phase_object)
             if fix_probe_com and not fix_probe:
                 self._probe = self.probe_com_constraint(self._probe)
+            self._positions_px = self.positions_constraints(
+                self._positions_px, self._positions_px_initial, fix_positions,
+                fix_positions_com and not fix_positions, global_affine_transformation)
             self.error_iterations.append(float(error))
             if store_iterations:
                 self.object_iterations.append(np.asarray(self._object).copy())

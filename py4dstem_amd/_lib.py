@@ -34,6 +34,13 @@ class PtyGdParams(ctypes.Structure):
                 ("shrinkage_rad", c_float), ("stage", c_int)]
 
 
+class PtyPositionParams(ctypes.Structure):
+    _fields_ = [("positions", c_void_p), ("positions_initial", c_void_p), ("step_size", c_float),
+                ("clip_update", c_int), ("max_update", ctypes.c_double), ("clip_total", c_int),
+                ("max_total", ctypes.c_double), ("center_of_mass", c_int),
+                ("initial_com", c_float * 2), ("singular", c_void_p)]
+
+
 class PtyConvDesc(ctypes.Structure):
     _fields_ = [("n", c_int), ("Hi", c_int), ("Wi", c_int), ("Ci", c_int), ("Ho", c_int),
                 ("Wo", c_int), ("Co", c_int), ("stride", c_int), ("dil", c_int)]
@@ -100,6 +107,15 @@ _SIGNATURES = {
     "pty_gd_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams)]),
     "pty_gd_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams), c_void_p,
                            c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_gd_positions_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem),
+                                                    ctypes.POINTER(PtyGdParams)]),
+    "pty_gd_run_positions": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams),
+                                     ctypes.POINTER(PtyPositionParams), c_void_p, c_void_p,
+                                     c_void_p, c_size_t, c_void_p]),
+    "pty_position_correction_batch": (c_int, [ctypes.POINTER(PtyPositionParams), c_void_p,
+                                              c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
+                                              c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "pty_positions_com": (c_int, [c_void_p, c_int, ctypes.POINTER(c_float), c_void_p]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -327,7 +327,7 @@ template <int S> struct Fused {
   }
 
   // generic passes reading and writing LDS
-  template <bool INV, class L> static PTY_HD void p_rowA(int tid, Regs& r, L lds, const cf* tw) {
+  template <bool INV, class L, class RG> static PTY_HD void p_rowA(int tid, RG& r, L lds, const cf* tw) {
     const int rr = F::row_line(tid), n2 = F::row_q(tid);
 #pragma unroll
     for (int m = 0; m < N1; ++m) r.v[m] = lds.ld(F::at(rr, n2 + N2 * m));
@@ -335,7 +335,7 @@ template <int S> struct Fused {
 #pragma unroll
     for (int m = 0; m < N1; ++m) lds.st(F::at(rr, n2 + N2 * m), r.v[m]);
   }
-  template <bool INV, class L> static PTY_HD void p_rowB(int tid, Regs& r, L lds, const cf* tw) {
+  template <bool INV, class L, class RG> static PTY_HD void p_rowB(int tid, RG& r, L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
@@ -343,7 +343,7 @@ template <int S> struct Fused {
 #pragma unroll
     for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
   }
-  template <bool INV, class L> static PTY_HD void p_colB(int tid, Regs& r, L lds, const cf* tw) {
+  template <bool INV, class L, class RG> static PTY_HD void p_colB(int tid, RG& r, L lds, const cf* tw) {
     const int cc = F::col_line(tid), t = F::col_q(tid);
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(N1 * t + j, cc));
@@ -557,6 +557,212 @@ template <int S, class Run> PTY_HD void probe_grad_position(Run& run, const Fuse
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_pgrad_out(tid, a, b, r, lds, tw); });
   run.barrier();  // the ramp table and frame are rewritten by the next position
+}
+
+// ---------------------------------------------------------------------------
+// Position correction (phase/ptychographic_methods.py:2071-2225), one work
+// group per scan position of the mini-batch, after the batch's object update.
+// With F = fft2(P_shift * O_old patch) kept by the fused pass and
+// G_x, G_y = fft2(P_shift * O_new patch moved one pixel along rows / columns):
+//   p_x = 2 Re((F - G_x) conj F),  p_y = 2 Re((F - G_y) conj F),
+//   d = A^2 - |F|^2                                  (float32, per pixel)
+//   u = [[sum p_x^2, sum p_x p_y], [., sum p_y^2]]^-1 [sum p_x d, sum p_y d]
+// (float64 sums), then positions -= step * u with the optional clips.  All
+// sums run over every frequency, so the permuted order of F, G and the
+// amplitudes needs no reordering.
+// ---------------------------------------------------------------------------
+struct PosArgs {
+  const cf* ps;                    // [B][S][S] shifted probes of the batch (natural)
+  const cf* far;                   // [B][S][S] F of the fused pass (permuted)
+  const cf* object;                // [Px][Py] object after the batch's update
+  const float* amps;               // [N][S][S] measured amplitudes (permuted)
+  float* positions;                // [N][2] slot b updates its own entry only
+  const float* positions_initial;  // [N][2]
+  const int* order;                // [B] position index of each slot
+  int N, B, Px, Py;
+  float step;                      // positions_step_size
+  int clip_update, clip_total;     // max_position_update/total_distance given
+  double max_update, max_total;    // those limits in pixels
+  double* sums;                    // optional [B][5]: p_x^2, p_x p_y, p_y^2, p_x d, p_y d
+  int* singular;                   // incremented per singular 2x2 system
+};
+
+constexpr int POS_SUMS = 5;
+
+// 2x2 solve, step, clips and the position write of one slot (one thread).
+// Float32 where the reference's arrays are float32 (step product, norms,
+// distance test, the subtraction), float64 where it compares against or
+// divides by the float64 limits.
+PTY_HD void position_update(const PosArgs& a, int b, int idx, const double* s) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  if (a.sums) {
+    for (int k = 0; k < POS_SUMS; ++k) a.sums[(size_t)b * POS_SUMS + k] = s[k];
+  }
+  const double det = s[0] * s[2] - s[1] * s[1];
+  float fx = 0.f, fy = 0.f;
+  if (det != 0.0) {
+    const double ux = (s[2] * s[3] - s[1] * s[4]) / det;
+    const double uy = (s[0] * s[4] - s[1] * s[3]) / det;
+    fx = (float)ux * a.step;
+    fy = (float)uy * a.step;
+  } else if (a.singular) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicAdd(a.singular, 1);
+#else
+    ++*a.singular;
+#endif
+  }
+  if (a.clip_update) {
+    const float n = sqrtf(fx * fx + fy * fy);
+    if ((double)n > a.max_update) {
+      const double q = (double)n / a.max_update;
+      fx = (float)((double)fx / q);
+      fy = (float)((double)fy / q);
+    }
+  }
+  const float cx = a.positions[2 * idx], cy = a.positions[2 * idx + 1];
+  if (a.clip_total) {
+    const float dx = (cx - a.positions_initial[2 * idx]) - fx;
+    const float dy = (cy - a.positions_initial[2 * idx + 1]) - fy;
+    if ((double)sqrtf(dx * dx + dy * dy) > a.max_total) {
+      fx = 0.f;
+      fy = 0.f;
+    }
+  }
+  a.positions[2 * idx] = cx - fx;
+  a.positions[2 * idx + 1] = cy - fy;
+}
+
+template <int S> struct PosCorr {
+  using F = Frame<S>;
+  using P = Pass<S>;
+  static constexpr int N1 = F::N1, N2 = F::N2, T = F::T;
+  static constexpr int LDS_TOTAL = F::LDS_WORDS;
+
+  struct Regs {
+    cf v[N1];
+    cf f[N1];      // F at this thread's row-pass-B positions
+    float amp[N1]; // measured amplitudes at the same positions
+    float px[N1];  // p_x at the same positions
+    double s[POS_SUMS];
+    int x0, y0, idx;
+  };
+
+  // anchor of slot b; F and the amplitudes stay in flight across the FFT
+  static PTY_HD void p_setup(int tid, const PosArgs& a, int b, Regs& r) {
+    int idx = a.order ? a.order[b] : b;
+    PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
+    idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
+    r.idx = idx;
+    r.x0 = wrapi((int)rintf(a.positions[2 * idx]), a.Px);
+    r.y0 = wrapi((int)rintf(a.positions[2 * idx + 1]), a.Py);
+    const size_t o = F::row_line(tid) * S + N1 * F::row_q(tid);
+    const float* A = a.amps + (size_t)idx * S * S + o;
+    const cf* Fb = a.far + (size_t)b * S * S + o;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      r.amp[j] = A[j];
+      r.f[j] = Fb[j];
+    }
+#pragma unroll
+    for (int k = 0; k < POS_SUMS; ++k) r.s[k] = 0.0;
+  }
+
+  // column pass A of fft2(P_shift * O[rows + DX, cols + DY]) from global memory
+  template <int DX, int DY, class L>
+  static PTY_HD void p_colA(int tid, const PosArgs& a, int b, Regs& r, L lds, const cf* tw) {
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+    const cf* ps_b = a.ps + (size_t)b * S * S;
+    int col = r.y0 + F::fidx(cc) + DY;
+    col += col < 0 ? a.Py : 0;
+    col -= col >= a.Py ? a.Py : 0;
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int i = n2 + N2 * m;
+      int row = r.x0 + F::fidx(i) + DX;
+      row += row < 0 ? a.Px : 0;
+      row -= row >= a.Px ? a.Px : 0;
+      PTY_CHECK(row >= 0 && row < a.Px && col >= 0 && col < a.Py, "object index out of range");
+      r.v[m] = cmul(ps_b[i * S + cc], a.object[(unsigned)row * (unsigned)a.Py + (unsigned)col]);
+    }
+    P::template A<false>(n2, r.v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) lds.st(F::at(n2 + N2 * m, cc), r.v[m]);
+  }
+
+  // 2 Re((F - G) conj F), the reference's complex64 product (real part)
+  static PTY_HD float partial(cf f, cf g) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const cf d = csub(f, g);
+    return 2.f * (d.x * f.x + d.y * f.y);
+  }
+
+  // row pass B completes G_x; p_x kept in registers
+  template <class L> static PTY_HD void p_partial_x(int tid, Regs& r, L lds, const cf* tw) {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
+    P::template B<false>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.px[j] = partial(r.f[j], r.v[j]);
+  }
+
+  // row pass B completes G_y; the five float64 sums of this thread
+  template <class L> static PTY_HD void p_partial_y_sums(int tid, Regs& r, L lds, const cf* tw) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
+    P::template B<false>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      const double px = r.px[j];
+      const double py = partial(r.f[j], r.v[j]);
+      const double d = r.amp[j] * r.amp[j] - cabs2(r.f[j]);  // float32 difference
+      r.s[0] += px * px;
+      r.s[1] += px * py;
+      r.s[2] += py * py;
+      r.s[3] += px * d;
+      r.s[4] += py * d;
+    }
+  }
+};
+
+// The phase sequence of one position's correction (runner as for
+// fused_position; ``run.reduce_sums`` adds r.s over the work group and hands
+// the totals to thread 0).  Nine work-group barriers.
+template <int S, class Run> PTY_HD void position_correction_position(Run& run, const PosArgs& a, int b) {
+  using K = PosCorr<S>;
+  using Regs = typename K::Regs;
+  using FK = Fused<S>;
+  run.phase([&](int tid, Regs& r, auto, const cf*) { K::p_setup(tid, a, b, r); });
+  run.barrier();  // twiddle table
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colA<1, 0>(tid, a, b, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { FK::template p_colB<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { FK::template p_rowA<false>(tid, r, lds, tw); });
+  run.wave_barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_partial_x(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colA<0, 1>(tid, a, b, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { FK::template p_colB<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { FK::template p_rowA<false>(tid, r, lds, tw); });
+  run.wave_barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_partial_y_sums(tid, r, lds, tw); });
+  double tot[POS_SUMS];
+  run.reduce_sums(tot);  // ends with a barrier; totals valid in thread 0
+  run.phase([&](int tid, Regs& r, auto, const cf*) {
+    if (tid == 0) position_update(a, b, r.idx, tot);
+  });
 }
 
 // ---------------------------------------------------------------------------

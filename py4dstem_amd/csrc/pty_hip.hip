@@ -143,6 +143,98 @@ __global__ __launch_bounds__(Frame<S>::T) void k_simulate(FusedArgs a) {
   simulate_position<S>(run, a, a.b0 + (int)blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// position correction, one work group per position of the mini-batch
+// ---------------------------------------------------------------------------
+template <int S> struct GpuPosRunner {
+  using K = PosCorr<S>;
+  static constexpr int T = Frame<S>::T;
+  typename K::Regs& r;
+  LdsPtr lds;
+  const cf* tw;
+  int tid;
+  double* red;  // [T / 64][POS_SUMS]
+  template <class Fn> __device__ __forceinline__ void phase(Fn fn) {
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    fn(t, r, lds, tw);
+  }
+  __device__ __forceinline__ void barrier() { __syncthreads(); }
+  __device__ __forceinline__ void wave_barrier() { wave_sync(); }
+  // wave sums by butterfly, then thread 0 adds the waves in order
+  __device__ __forceinline__ void reduce_sums(double* tot) {
+#pragma unroll
+    for (int k = 0; k < POS_SUMS; ++k) {
+      double v = r.s[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if ((tid & 63) == 0) red[(tid >> 6) * POS_SUMS + k] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+      for (int k = 0; k < POS_SUMS; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < T / 64; ++w) s += red[w * POS_SUMS + k];
+        tot[k] = s;
+      }
+    }
+  }
+};
+
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_position_correction(PosArgs a) {
+  __shared__ cf lds_mem[PosCorr<S>::LDS_TOTAL];
+  __shared__ cf tw[S];
+  __shared__ double red[Frame<S>::T / 64 * POS_SUMS];
+  const int tid = threadIdx.x;
+  Fused<S>::twiddles(tid, LdsPtr{tw});  // published by the setup barrier
+  typename PosCorr<S>::Regs r;
+  GpuPosRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red};
+  position_correction_position<S>(run, a, (int)blockIdx.x);
+}
+
+// positions -= mean(positions) - com (phase/ptychographic_constraints.py:1292-1311)
+// with the reference's arithmetic: NumPy's mean over axis 0 of a float32
+// [N][2] array adds the rows one after another in float32, then divides by N.
+// One thread adds them in that order from LDS-staged chunks (a dependent
+// chain of N adds per axis, tens of microseconds for a 128 x 128 scan, once
+// per iteration); every thread then applies the float32 shift.
+constexpr int COM_CHUNK = 4096;
+
+__global__ __launch_bounds__(1024) void k_positions_com(float* __restrict__ pos, int N, float com_x, float com_y) {
+#pragma clang fp contract(off)
+  __shared__ float2 stage[COM_CHUNK];
+  __shared__ float shift[2];
+  const int tid = threadIdx.x;
+  float sx = 0.f, sy = 0.f;
+  for (int base = 0; base < N; base += COM_CHUNK) {
+    const int n = (N - base) < COM_CHUNK ? (N - base) : COM_CHUNK;
+    for (int i = tid; i < n; i += 1024) stage[i] = make_float2(pos[2 * (base + i)], pos[2 * (base + i) + 1]);
+    __syncthreads();
+    if (tid == 0) {
+      for (int i = 0; i < n; ++i) {
+        const float2 v = stage[i];
+        sx += v.x;
+        sy += v.y;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    // float32 / N, correctly rounded (the float64 quotient rounds once more
+    // to the same float32: 53 >= 2 * 24 + 2)
+    shift[0] = (float)((double)sx / (double)N) - com_x;
+    shift[1] = (float)((double)sy / (double)N) - com_y;
+  }
+  __syncthreads();
+  const float dx = shift[0], dy = shift[1];
+  for (int i = tid; i < N; i += 1024) {
+    pos[2 * i] -= dx;
+    pos[2 * i + 1] -= dy;
+  }
+}
+
 // sum over b of complex frames [B][n], float64, in 64-slot chunks
 __global__ __launch_bounds__(256) void k_sum_frames_partial(const cf* __restrict__ frames, int B, int n,
                                                             double* __restrict__ part) {
@@ -748,6 +840,40 @@ int launch_probe_com(cf* probe, int S, hipStream_t stream) {
   return check_launch("k_probe_com");
 }
 
+int launch_position_correction(const PosArgs& a, int S, hipStream_t stream) {
+  if (a.B <= 0) return PTY_OK;
+  if (S == 64) hipLaunchKernelGGL(k_position_correction<64>, dim3(a.B), dim3(Frame<64>::T), 0, stream, a);
+  else hipLaunchKernelGGL(k_position_correction<128>, dim3(a.B), dim3(Frame<128>::T), 0, stream, a);
+  return check_launch("k_position_correction");
+}
+
+int launch_positions_com(float* pos, int N, const float* com, hipStream_t stream) {
+  hipLaunchKernelGGL(k_positions_com, dim3(1), dim3(1024), 0, stream, pos, N, com[0], com[1]);
+  return check_launch("k_positions_com");
+}
+
+int validate_position_params(const pty_position_params* pp, const float* positions) {
+  if (!pp) return PTY_OK;
+  if (!pp->positions || pp->positions != positions)
+    return fail(PTY_E_ARG, "position params must update the problem's positions buffer");
+  if (!pp->positions_initial) return fail(PTY_E_ARG, "positions_initial is NULL");
+  if (!std::isfinite(pp->step_size)) return fail(PTY_E_ARG, "positions step size is not finite");
+  return PTY_OK;
+}
+
+PosArgs position_args(const pty_position_params& pp, const cf* ps, const cf* far, const cf* obj, int Px, int Py,
+                      const float* amps, int N, const int* order, int B) {
+  PosArgs a{};
+  a.ps = ps; a.far = far; a.object = obj; a.amps = amps;
+  a.positions = pp.positions; a.positions_initial = pp.positions_initial; a.order = order;
+  a.N = N; a.B = B; a.Px = Px; a.Py = Py;
+  a.step = pp.step_size;
+  a.clip_update = pp.clip_update; a.max_update = pp.max_update;
+  a.clip_total = pp.clip_total; a.max_total = pp.max_total;
+  a.sums = nullptr; a.singular = pp.singular;
+  return a;
+}
+
 // near-equal batch sizes, larger ones first (phase/utils.py:539-579)
 std::vector<int> subdivide(int n, int max_batch) {
   const int nb = (n + ((max_batch - n % max_batch) % max_batch)) / max_batch;
@@ -758,7 +884,7 @@ std::vector<int> subdivide(int n, int max_batch) {
 }
 
 struct WsLayout {
-  size_t phat, ps, chi, err, grad, spsum, tilemax, ppart, cpart, run, total;
+  size_t phat, ps, chi, err, grad, spsum, tilemax, ppart, cpart, run, far, total;
 };
 
 // positions per fused launch: a multiple of 64 (probe-partial chunk) or the
@@ -769,10 +895,14 @@ int stage_size(const pty_gd_params& q) {
   return s < q.max_batch ? s : q.max_batch;
 }
 
-WsLayout layout(const pty_problem& p, const pty_gd_params& q) {
+// With position correction every launch covers the whole mini-batch: the
+// correction reads the batch's shifted probes and far fields after the update.
+int stage_for(const pty_gd_params& q, bool positions) { return positions ? q.max_batch : stage_size(q); }
+
+WsLayout layout(const pty_problem& p, const pty_gd_params& q, bool positions = false) {
   WsLayout L{};
   const size_t S2 = (size_t)p.S * p.S;
-  const size_t B = (size_t)stage_size(q);
+  const size_t B = (size_t)stage_for(q, positions);
   const size_t npix = (size_t)p.Px * p.Py;
   size_t o = 0;
   L.phat = o; o = align256(o + S2 * 8);
@@ -785,6 +915,7 @@ WsLayout layout(const pty_problem& p, const pty_gd_params& q) {
   L.ppart = o; o = align256(o + (q.fix_probe ? 0 : probe_ws_bytes(p.S, q.max_batch)));
   L.cpart = o; o = align256(o + (size_t)CONSTRAINT_BLOCKS * 16);
   L.run = o; o = align256(o + ((int)B < q.max_batch ? npix * 24 : 0));
+  L.far = o; o = align256(o + (positions ? B * S2 * 8 : 0));
   L.total = o;
   return L;
 }
@@ -808,6 +939,107 @@ int validate_problem(const pty_problem* p) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace {
+
+// num_iter iterations of the GD loop; pp (optional) adds the per-batch
+// position correction and the per-iteration centre-of-mass step
+int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_position_params* pp,
+           const int* orders, double* errors, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  int rc = validate_problem(problem);
+  if (rc) return rc;
+  if (!params) return fail(PTY_E_ARG, "params is NULL");
+  const pty_problem& p = *problem;
+  const pty_gd_params& q = *params;
+  if (q.num_iter < 0 || q.max_batch <= 0 || q.max_batch > p.N)
+    return fail(PTY_E_ARG, "bad num_iter=%d / max_batch=%d for N=%d", q.num_iter, q.max_batch, p.N);
+  if (!orders || !errors || !workspace) return fail(PTY_E_ARG, "null orders/errors/workspace");
+  if ((rc = validate_position_params(pp, p.positions))) return rc;
+  const bool correct = pp != nullptr;
+  const WsLayout L = layout(p, q, correct);
+  if (workspace_bytes < L.total)
+    return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.total);
+  char* ws = (char*)workspace;
+  cf* phat = (cf*)(ws + L.phat);
+  cf* ps = (cf*)(ws + L.ps);
+  cf* chi = (cf*)(ws + L.chi);
+  double* err = (double*)(ws + L.err);
+  cf* grad = (cf*)(ws + L.grad);
+  float* spsum = (float*)(ws + L.spsum);
+  float* tilemax = (float*)(ws + L.tilemax);
+  double* ppart = (double*)(ws + L.ppart);
+  double* cpart = (double*)(ws + L.cpart);
+  double* runbuf = (double*)(ws + L.run);
+  cf* far = correct ? (cf*)(ws + L.far) : nullptr;
+  const int stage = stage_for(q, correct);
+  cf* obj = (cf*)p.object;
+  cf* probe = (cf*)p.probe;
+  const std::vector<int> batches = subdivide(p.N, q.max_batch);
+  const int ntiles = num_tiles(p.Px, p.Py);
+  const double norm = (double)(float)(p.mean_intensity * (float)p.N);
+  bool phat_valid = false;
+  for (int it = 0; it < q.num_iter; ++it) {
+    const int* order_it = orders + (size_t)it * p.N;
+    int start = 0;
+    for (int nb : batches) {
+      const int* ob = order_it + start;
+      if (!phat_valid) {
+        if ((rc = launch_fft(probe, phat, 1, p.S, false, 0, stream))) return rc;
+        phat_valid = q.fix_probe != 0;
+      }
+      for (int s0 = 0; s0 < nb; s0 += stage) {
+        const int ns = (nb - s0) < stage ? (nb - s0) : stage;
+        FusedArgs a{};
+        a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
+        a.order = ob + s0; a.N = p.N; a.B = ns; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
+        a.ps_out = ps; a.chi_out = chi; a.err = err + start + s0; a.fwd_out = far;
+        if ((rc = launch_fused(a, p.S, stream))) return rc;
+        if (!q.fix_probe) {
+          ProbePartialArgs pa{chi, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
+                              ppart + (size_t)(s0 / PROBE_CHUNK) * p.S * p.S * 3};
+          hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(ns)), dim3(256), 0,
+                             stream, pa);
+          if ((rc = check_launch("k_probe_partial"))) return rc;
+        }
+        GatherArgs g{};
+        g.ps = ps; g.chi = chi; g.positions = p.positions; g.order = ob + s0; g.N = p.N; g.B = ns; g.S = p.S;
+        g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
+        g.tilemax = tilemax;
+        if (ns < nb) {
+          g.run = runbuf;
+          g.run_first = s0 == 0;
+          g.run_last = s0 + ns == nb;
+        }
+        if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+      }
+      if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
+                                     q.normalization_min, stream)))
+        return rc;
+      if (!q.fix_probe) {
+        if ((rc = launch_probe_apply(probe, ppart, p.S, nb, q.step_size, q.normalization_min, stream))) return rc;
+      }
+      if (correct) {  // singleslice_ptychography.py:913-926, after the adjoint
+        const PosArgs pa = position_args(*pp, ps, far, obj, p.Px, p.Py, p.amplitudes, p.N, ob, nb);
+        if ((rc = launch_position_correction(pa, p.S, stream))) return rc;
+      }
+      start += nb;
+    }
+    hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
+    if ((rc = check_launch("k_error_reduce"))) return rc;
+    if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
+                                 stream)))
+      return rc;
+    if (!q.fix_probe && q.fix_probe_com) {
+      if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
+    }
+    if (correct && pp->center_of_mass) {
+      if ((rc = launch_positions_com(pp->positions, p.N, pp->initial_com, stream))) return rc;
+    }
+  }
+  return PTY_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pty_version(void) { return 10000; }
@@ -1038,89 +1270,40 @@ size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* p
 }
 
 int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const int* orders, double* errors,
-               void* workspace, size_t workspace_bytes, pty_stream_t stream_) {
-  int rc = validate_problem(problem);
+               void* workspace, size_t workspace_bytes, pty_stream_t stream) {
+  return gd_run(problem, params, nullptr, orders, errors, workspace, workspace_bytes, st(stream));
+}
+
+size_t pty_gd_positions_workspace_bytes(const pty_problem* problem, const pty_gd_params* params) {
+  if (!problem || !params || params->max_batch <= 0) return 0;
+  return layout(*problem, *params, true).total;
+}
+
+int pty_gd_run_positions(const pty_problem* problem, const pty_gd_params* params,
+                         const pty_position_params* positions, const int* orders, double* errors,
+                         void* workspace, size_t workspace_bytes, pty_stream_t stream) {
+  if (!positions) return fail(PTY_E_ARG, "position params are NULL");
+  return gd_run(problem, params, positions, orders, errors, workspace, workspace_bytes, st(stream));
+}
+
+int pty_position_correction_batch(const pty_position_params* pp, const void* ps, const void* far,
+                                  const void* object, int Px, int Py, const float* amplitudes, int N,
+                                  const int* order, int B, int S, double* sums, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (Px < S || Py < S) return fail(PTY_E_ARG, "object %dx%d smaller than frame %d", Px, Py, S);
+  if (N <= 0 || B < 0 || B > N) return fail(PTY_E_ARG, "bad N=%d or B=%d", N, B);
+  if (!pp || !ps || !far || !object || !amplitudes || !order)
+    return fail(PTY_E_ARG, "null buffer in position_correction_batch");
+  int rc = validate_position_params(pp, pp->positions);
   if (rc) return rc;
-  if (!params) return fail(PTY_E_ARG, "params is NULL");
-  const pty_problem& p = *problem;
-  const pty_gd_params& q = *params;
-  if (q.num_iter < 0 || q.max_batch <= 0 || q.max_batch > p.N)
-    return fail(PTY_E_ARG, "bad num_iter=%d / max_batch=%d for N=%d", q.num_iter, q.max_batch, p.N);
-  if (!orders || !errors || !workspace) return fail(PTY_E_ARG, "null orders/errors/workspace");
-  const WsLayout L = layout(p, q);
-  if (workspace_bytes < L.total)
-    return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.total);
-  hipStream_t stream = st(stream_);
-  char* ws = (char*)workspace;
-  cf* phat = (cf*)(ws + L.phat);
-  cf* ps = (cf*)(ws + L.ps);
-  cf* chi = (cf*)(ws + L.chi);
-  double* err = (double*)(ws + L.err);
-  cf* grad = (cf*)(ws + L.grad);
-  float* spsum = (float*)(ws + L.spsum);
-  float* tilemax = (float*)(ws + L.tilemax);
-  double* ppart = (double*)(ws + L.ppart);
-  double* cpart = (double*)(ws + L.cpart);
-  double* runbuf = (double*)(ws + L.run);
-  const int stage = stage_size(q);
-  cf* obj = (cf*)p.object;
-  cf* probe = (cf*)p.probe;
-  const std::vector<int> batches = subdivide(p.N, q.max_batch);
-  const int ntiles = num_tiles(p.Px, p.Py);
-  const double norm = (double)(float)(p.mean_intensity * (float)p.N);
-  bool phat_valid = false;
-  for (int it = 0; it < q.num_iter; ++it) {
-    const int* order_it = orders + (size_t)it * p.N;
-    int start = 0;
-    for (int nb : batches) {
-      const int* ob = order_it + start;
-      if (!phat_valid) {
-        if ((rc = launch_fft(probe, phat, 1, p.S, false, 0, stream))) return rc;
-        phat_valid = q.fix_probe != 0;
-      }
-      for (int s0 = 0; s0 < nb; s0 += stage) {
-        const int ns = (nb - s0) < stage ? (nb - s0) : stage;
-        FusedArgs a{};
-        a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
-        a.order = ob + s0; a.N = p.N; a.B = ns; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
-        a.ps_out = ps; a.chi_out = chi; a.err = err + start + s0; a.fwd_out = nullptr;
-        if ((rc = launch_fused(a, p.S, stream))) return rc;
-        if (!q.fix_probe) {
-          ProbePartialArgs pa{chi, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
-                              ppart + (size_t)(s0 / PROBE_CHUNK) * p.S * p.S * 3};
-          hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(ns)), dim3(256), 0,
-                             stream, pa);
-          if ((rc = check_launch("k_probe_partial"))) return rc;
-        }
-        GatherArgs g{};
-        g.ps = ps; g.chi = chi; g.positions = p.positions; g.order = ob + s0; g.N = p.N; g.B = ns; g.S = p.S;
-        g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
-        g.tilemax = tilemax;
-        if (ns < nb) {
-          g.run = runbuf;
-          g.run_first = s0 == 0;
-          g.run_last = s0 + ns == nb;
-        }
-        if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
-      }
-      if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
-                                     q.normalization_min, stream)))
-        return rc;
-      if (!q.fix_probe) {
-        if ((rc = launch_probe_apply(probe, ppart, p.S, nb, q.step_size, q.normalization_min, stream))) return rc;
-      }
-      start += nb;
-    }
-    hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
-    if ((rc = check_launch("k_error_reduce"))) return rc;
-    if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
-                                 stream)))
-      return rc;
-    if (!q.fix_probe && q.fix_probe_com) {
-      if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
-    }
-  }
-  return PTY_OK;
+  PosArgs a = position_args(*pp, (const cf*)ps, (const cf*)far, (const cf*)object, Px, Py, amplitudes, N, order, B);
+  a.sums = sums;
+  return launch_position_correction(a, S, st(stream));
+}
+
+int pty_positions_com(float* positions, int N, const float* initial_com, pty_stream_t stream) {
+  if (!positions || !initial_com || N <= 0) return fail(PTY_E_ARG, "bad positions_com arguments");
+  return launch_positions_com(positions, N, initial_com, st(stream));
 }
 
 }  // extern "C"

@@ -171,10 +171,61 @@ def gd_run(problem, params, orders, errors, workspace):
               ptr(workspace), workspace.numel(), stream_handle())
 
 
+def position_params(positions, positions_initial, step_size, max_update_px=None,
+                    max_total_px=None, center_of_mass=False, initial_com=(0.0, 0.0),
+                    singular=None):
+    """pty_position_params for device positions [N, 2] (updated in place);
+    limits in object pixels, None for no limit."""
+    from ._lib import PtyPositionParams
+    pp = PtyPositionParams()
+    pp.positions = c_void_p(positions.data_ptr())
+    pp.positions_initial = c_void_p(positions_initial.data_ptr())
+    pp.step_size = float(step_size)
+    pp.clip_update = int(max_update_px is not None)
+    pp.max_update = float(max_update_px) if max_update_px is not None else 0.0
+    pp.clip_total = int(max_total_px is not None)
+    pp.max_total = float(max_total_px) if max_total_px is not None else 0.0
+    pp.center_of_mass = int(bool(center_of_mass))
+    pp.initial_com[0] = float(initial_com[0])
+    pp.initial_com[1] = float(initial_com[1])
+    pp.singular = c_void_p(singular.data_ptr()) if singular is not None else None
+    return pp
+
+
+def gd_positions_workspace_bytes(problem, params):
+    import ctypes
+    return int(_lib.load().pty_gd_positions_workspace_bytes(ctypes.byref(problem),
+                                                            ctypes.byref(params)))
+
+
+def gd_run_positions(problem, params, position_params, orders, errors, workspace):
+    import ctypes
+    _lib.call("pty_gd_run_positions", ctypes.byref(problem), ctypes.byref(params),
+              ctypes.byref(position_params), ptr(orders), ptr(errors), ptr(workspace),
+              workspace.numel(), stream_handle())
+
+
+def position_correction(position_params, ps, far, obj, amplitudes_k, order, sums=None):
+    """One batch of position correction (pty_position_correction_batch)."""
+    import ctypes
+    S = ps.shape[-1]
+    Px, Py = obj.shape
+    _lib.call("pty_position_correction_batch", ctypes.byref(position_params), ptr(ps), ptr(far),
+              ptr(obj), Px, Py, ptr(amplitudes_k), amplitudes_k.shape[0], ptr(order),
+              order.numel(), S, ptr(sums), stream_handle())
+
+
+def positions_com(positions, initial_com):
+    import ctypes
+    com = (ctypes.c_float * 2)(float(initial_com[0]), float(initial_com[1]))
+    _lib.call("pty_positions_com", ptr(positions), positions.shape[0], com, stream_handle())
+
+
 __all__ = ["fft2", "to_kernel_layout", "probe_spectrum", "forward_adjoint", "scatter",
            "object_update", "probe_update", "object_constraints", "probe_com_constraint",
            "make_problem", "gd_workspace_bytes", "gd_run", "PtyProblem", "PtyGdParams",
-           "kernel_index", "num_tiles"]
+           "kernel_index", "num_tiles", "position_params", "gd_positions_workspace_bytes",
+           "gd_run_positions", "position_correction", "positions_com"]
 
 
 def pattern_com(frames, mask=None):

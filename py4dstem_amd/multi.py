@@ -38,7 +38,12 @@ class MultiPtychography:
         runs = []
         for p, kw in zip(self.items, kwargs):
             kw = dict(kw, num_iter=num_iter, progress_bar=False, store_iterations=False)
-            runs.append(p._prepare_run(_allocate=False, **kw))
+            run = p._prepare_run(_allocate=False, **kw)
+            if run["position_params"] is not None:
+                raise NotImplementedError("position correction (fix_positions=False) runs through "
+                                          "SingleslicePtychography.reconstruct(), not the batched "
+                                          "runner")
+            runs.append(run)
         return runs
 
     def reconstruct(self, num_iter, reconstruct_kwargs=None, use_graph=True):

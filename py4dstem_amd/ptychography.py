@@ -334,6 +334,7 @@ class SingleslicePtychography:
 
         # device-resident positions for the loop (amplitudes are already there)
         self._positions_dev = torch.from_numpy(self._positions_px.copy()).to(dev)
+        self._positions_initial_dev = torch.from_numpy(self._positions_px_initial.copy()).to(dev)
 
         # probe-overlap field and FOV mask (singleslice_ptychography.py:504-537)
         if object_fov_mask is None or plot_probe_overlaps:
@@ -474,6 +475,8 @@ class SingleslicePtychography:
             self._object = self._object_initial.copy()
             self._probe = self._probe_initial.copy()
             self._positions_px = self._positions_px_initial.copy()
+            if hasattr(self, "_positions_initial_dev"):
+                self._positions_dev.copy_(self._positions_initial_dev)
             self._object_type = self._object_type_initial
             self._exit_waves = None
             for attr in ("_object_dev", "_probe_dev"):
@@ -577,7 +580,6 @@ class SingleslicePtychography:
                 "'SUPERFLIP' (or 'charge-flipping'), "
                 f"or 'GD' (or 'gradient-descent'), not  {reconstruction_method}.")
         unsupported = {
-            "fix_positions=False": not fix_positions,
             "detector_fourier_mask": detector_fourier_mask is not None,
             "gaussian_filter_sigma": gaussian_filter and gaussian_filter_sigma is not None,
             "q_lowpass/q_highpass": butterworth_filter and (q_lowpass is not None
@@ -600,7 +602,10 @@ class SingleslicePtychography:
             warnings.warn(first + (f", in batches of max {max_batch_size} measurements."
                                    if max_batch_size is not None else "."), UserWarning)
 
-        # batch order: identical RNG use to the reference loop (:835-857)
+        # batch order: identical RNG use to the reference loop (:835-857).
+        # The affine position constraint draws from the same generator after
+        # every iteration, so its orders are drawn one iteration at a time.
+        affine = (not fix_positions) and bool(global_affine_transformation)
         shuffled = np.arange(N)
         if max_batch_size is not None:
             np.random.seed(seed_random)
@@ -609,7 +614,7 @@ class SingleslicePtychography:
             B = N
         B = max(1, min(B, N))
         orders = np.empty((max(num_iter, 1), N), dtype=np.int32)
-        for it in range(num_iter):
+        for it in range(0 if affine else num_iter):
             np.random.shuffle(shuffled)
             orders[it] = shuffled
 
@@ -625,19 +630,39 @@ class SingleslicePtychography:
                     if use_mask else None)
         problem = K.make_problem(self._object_dev, self._probe_dev, self._positions_dev,
                                  self._amplitudes_dev, mask_dev, self._mean_diffraction_intensity)
-        per_call = 1 if (store_iterations or progress_bar) else max(num_iter, 1)
+        per_call = 1 if (store_iterations or progress_bar or affine) else max(num_iter, 1)
         params = PtyGdParams(per_call, B, int(bool(fix_probe)), int(bool(fix_probe_com)),
                              int(bool(pure_phase_object)), float(step_size),
                              float(normalization_min), float(shrinkage_rad),
                              int(self._stage_for(B)))
-        ws = (torch.empty(K.gd_workspace_bytes(problem, params), dtype=torch.uint8, device=dev)
-              if _allocate else None)
+        pos_params = singular = None
+        if not fix_positions:
+            # position correction (singleslice_ptychography.py:913-926) and the
+            # centre-of-mass constraint, which the reference applies when its
+            # own flag arrives False (fix_positions_com and not fix_positions
+            # at :956, tested as `if not fix_positions_com` in
+            # ptychographic_constraints.py:1367)
+            norm = np.sqrt(self.sampling[0] ** 2 + self.sampling[1] ** 2)
+            singular = torch.zeros(1, dtype=torch.int32, device=dev)
+            pos_params = K.position_params(
+                self._positions_dev, self._positions_initial_dev, positions_step_size,
+                None if max_position_update_distance is None
+                else max_position_update_distance / norm,
+                None if max_position_total_distance is None
+                else max_position_total_distance / norm,
+                center_of_mass=not fix_positions_com,
+                initial_com=self._positions_px_initial.mean(0), singular=singular)
+            nbytes = K.gd_positions_workspace_bytes(problem, params)
+        else:
+            nbytes = K.gd_workspace_bytes(problem, params)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev) if _allocate else None
         orders_dev = torch.from_numpy(orders).to(dev)
         errors_dev = torch.zeros(max(num_iter, 1), dtype=torch.float64, device=dev)
         self._last_run = dict(problem=problem, params=params, workspace=ws, orders=orders_dev,
                               errors=errors_dev, mask=mask_dev, num_iter=num_iter,
                               per_call=per_call, store_iterations=store_iterations,
-                              progress_bar=progress_bar)
+                              progress_bar=progress_bar, position_params=pos_params,
+                              singular=singular, affine=affine, shuffled=shuffled)
         return self._last_run
 
     def _execute_run(self, run):
@@ -652,10 +677,20 @@ class SingleslicePtychography:
                 bar = tqdm(total=num_iter, desc="Reconstructing object and probe", unit=" iter")
             except ImportError:
                 bar = None
+        pos_params = run["position_params"]
         while it < num_iter:
             n = min(per_call, num_iter - it)
             params.num_iter = n
-            K.gd_run(problem, params, orders_dev[it:], errors_dev[it:], ws)
+            if run["affine"]:
+                np.random.shuffle(run["shuffled"])
+                orders_dev[it].copy_(torch.from_numpy(run["shuffled"].astype(np.int32)))
+            if pos_params is None:
+                K.gd_run(problem, params, orders_dev[it:], errors_dev[it:], ws)
+            else:
+                K.gd_run_positions(problem, params, pos_params, orders_dev[it:], errors_dev[it:],
+                                   ws)
+            if run["affine"]:
+                self._affine_positions_constraint()
             it += n
             if run["store_iterations"]:
                 self.object_iterations.append(self._object_dev.cpu().numpy().copy())
@@ -666,8 +701,28 @@ class SingleslicePtychography:
         if bar is not None:
             bar.close()
 
+    def _affine_positions_constraint(self):
+        """ptychographic_constraints.py:1313-1353: positions replaced by the
+        RANSAC affine map of the initial scan (host; N x 2 floats)."""
+        from .positions import estimate_global_transformation_ransac
+        initial = self._positions_px_initial
+        current = self._positions_dev.cpu().numpy()
+        origin = initial.mean(0)
+        tf, _ = estimate_global_transformation_ransac(initial, current, origin=origin,
+                                                      translation_allowed=True,
+                                                      min_sample=initial.shape[0] // 10)
+        self._tf = tf
+        new = np.asarray(tf(initial, origin=origin), dtype=np.float32)
+        self._positions_dev.copy_(torch.from_numpy(np.ascontiguousarray(new)))
+
     def _finish_run(self, run):
         num_iter = run["num_iter"]
+        if run.get("position_params") is not None:
+            self._positions_px = self._positions_dev.cpu().numpy()
+            if int(run["singular"].item()):
+                raise np.linalg.LinAlgError(
+                    "Singular matrix in position correction "
+                    f"({int(run['singular'].item())} position(s) got no update)")
         errors = run["errors"][:num_iter].cpu().numpy()
         self.error_iterations.extend(float(e) for e in errors)
         self._object = self._object_dev.cpu().numpy()

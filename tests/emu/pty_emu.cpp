@@ -53,12 +53,12 @@ struct LdsCheck {
   }
 };
 
-template <int S> struct EmuRunner {
+template <int S, class RegsT = typename Fused<S>::Regs> struct EmuRunner {
   using K = Fused<S>;
   static constexpr int T = Frame<S>::T;
   std::vector<cf> mem;
   std::vector<int> wt, rd, ww, rw;
-  std::vector<typename K::Regs> regs;
+  std::vector<RegsT> regs;
   std::vector<cf> tw;
   int cur = -1;
   const FusedArgs* a;
@@ -99,6 +99,14 @@ template <int S> struct EmuRunner {
     double s = 0.0;
     for (int t = 0; t < T; ++t) s += regs[t].err;
     a->err[b] = s;
+    barrier();
+  }
+  void reduce_sums(double* tot) {
+    for (int k = 0; k < POS_SUMS; ++k) {
+      double s = 0.0;
+      for (int t = 0; t < T; ++t) s += regs[t].s[k];
+      tot[k] = s;
+    }
     barrier();
   }
 };
@@ -301,6 +309,31 @@ int emu_simulate(int S, const float* probe_hat, const float* object, int Px, int
   if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < N; ++b) simulate_position<64>(run, a, b); }
   else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < N; ++b) simulate_position<128>(run, a, b); }
   else return -1;
+  return 0;
+}
+
+// position correction of one batch: same arguments as
+// pty_position_correction_batch (positions updated in place)
+int emu_position_correction(int S, const float* ps, const float* far, const float* object, int Px, int Py,
+                            const float* amps, float* positions, const float* positions_initial, int N,
+                            const int* order, int B, float step, int clip_update, double max_update,
+                            int clip_total, double max_total, double* sums, int* singular) {
+  PosArgs a{};
+  a.ps = (const cf*)ps; a.far = (const cf*)far; a.object = (const cf*)object; a.amps = amps;
+  a.positions = positions; a.positions_initial = positions_initial; a.order = order;
+  a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.step = step;
+  a.clip_update = clip_update; a.max_update = max_update;
+  a.clip_total = clip_total; a.max_total = max_total;
+  a.sums = sums; a.singular = singular;
+  if (S == 64) {
+    EmuRunner<64, PosCorr<64>::Regs> run(nullptr);
+    for (int b = 0; b < B; ++b) position_correction_position<64>(run, a, b);
+  } else if (S == 128) {
+    EmuRunner<128, PosCorr<128>::Regs> run(nullptr);
+    for (int b = 0; b < B; ++b) position_correction_position<128>(run, a, b);
+  } else {
+    return -1;
+  }
   return 0;
 }
 

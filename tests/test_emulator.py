@@ -2,6 +2,8 @@
 run, executed by the emulator (tests/emu/pty_emu.cpp) with LDS race and
 bounds checking, compared with the oracle."""
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -241,3 +243,94 @@ def test_scatter_of_32px_stitching_windows(emu):
     assert rel(grad, ref) < 1e-6
     inner = (slice(16, 80), slice(16, 80))
     assert rel(grad[inner] / wsum[inner], obj[inner]) < 1e-6
+
+
+def _position_batch(emu, o, seed=4, B=12):
+    """Fused pass of a batch, the oracle's update of the object, and the
+    inputs of the position-correction phases."""
+    S = 64
+    rng = np.random.default_rng(seed)
+    obj = (o._object * np.exp(1j * 0.3 * rng.standard_normal(o._object.shape))).astype(np.complex64)
+    probe = o._probe.astype(np.complex64)
+    order = rng.permutation(o._num_diffraction_patterns)[:B].astype(np.int32)
+    pos = o._positions_px.copy()
+    k = kidx(emu, S)
+    phat = np.ascontiguousarray(np.fft.fft2(probe)[np.ix_(k, k)].astype(np.complex64))
+    amps = np.ascontiguousarray(o._amplitudes[:, k][:, :, k])
+    ps = np.zeros((B, S, S), np.complex64)
+    chi = np.zeros_like(ps)
+    far = np.zeros_like(ps)
+    err = np.zeros(B)
+    emu.emu_fused(S, cptr(phat), cptr(obj), obj.shape[0], obj.shape[1], cptr(pos), len(pos),
+                  cptr(amps), cptr(order), B, cptr(ps), cptr(chi), cptr(err), cptr(far))
+    sp, op, ew, _ = o.forward(obj, probe, pos[order], o._amplitudes[order])
+    new_obj, _ = o.adjoint(obj.copy(), probe.copy(), op, sp, pos[order], ew, 0.5, 1, True)
+    return dict(S=S, order=order, pos=pos, amps=amps, ps=ps, far=far, sp=sp, op=op,
+                obj=np.ascontiguousarray(new_obj.astype(np.complex64)))
+
+
+def _emu_positions(emu, d, clip_update=0, max_update=0.0, clip_total=0, max_total=0.0,
+                   initial=None, step=0.5):
+    pos = d["pos"].copy()
+    init = d["pos"].copy() if initial is None else initial
+    B = len(d["order"])
+    sums = np.zeros((B, 5))
+    singular = np.zeros(1, np.int32)
+    emu.emu_reset()
+    rc = emu.emu_position_correction(
+        d["S"], cptr(d["ps"]), cptr(d["far"]), cptr(d["obj"]), d["obj"].shape[0], d["obj"].shape[1],
+        cptr(d["amps"]), cptr(pos), cptr(init), len(pos), cptr(d["order"]), B,
+        ctypes.c_float(step), clip_update, ctypes.c_double(max_update), clip_total,
+        ctypes.c_double(max_total), cptr(sums), cptr(singular))
+    assert rc == 0
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    return pos, sums, int(singular[0])
+
+
+def test_position_correction_phases_match_oracle(emu, small_case):
+    """Position-correction phases (two shifted-object FFTs, float64 sums,
+    2x2 solve) against the oracle's _position_correction on one batch."""
+    _, o = small_case
+    d = _position_batch(emu, o)
+    order = d["order"]
+    pos, sums, singular = _emu_positions(emu, d)
+    assert singular == 0
+    diff, pdx, pdy = o.position_partials(d["obj"], d["pos"][order], d["sp"], d["sp"] * d["op"],
+                                         o._amplitudes[order])
+    pdx, pdy, diff = (x.astype(np.float64) for x in (pdx, pdy, diff))
+    ref_sums = np.stack([np.sum(pdx * pdx, 1), np.sum(pdx * pdy, 1), np.sum(pdy * pdy, 1),
+                         np.sum(pdx * diff, 1), np.sum(pdy * diff, 1)], 1)
+    assert rel(sums, ref_sums) < 1e-5
+    ref = o.position_correction(d["obj"], d["pos"][order].copy(), d["sp"], d["sp"] * d["op"],
+                                o._amplitudes[order], d["pos"][order], 0.5)
+    upd, ref_upd = d["pos"][order] - pos[order], d["pos"][order] - ref
+    assert np.max(np.abs(upd)) > 1e-3  # the step is not trivially zero
+    assert rel(upd, ref_upd) < 1e-4
+    untouched = np.setdiff1d(np.arange(len(pos)), order)
+    np.testing.assert_array_equal(pos[untouched], d["pos"][untouched])
+
+
+def test_position_correction_clips_match_oracle(emu, small_case):
+    _, o = small_case
+    d = _position_batch(emu, o, seed=7)
+    order = d["order"]
+    norm = float(np.hypot(*o.sampling))
+    free, _, _ = _emu_positions(emu, d)
+    steps = np.linalg.norm(d["pos"][order] - free[order], axis=1)
+    lim = float(np.median(steps))  # about half the positions get clipped
+    pos, _, _ = _emu_positions(emu, d, clip_update=1, max_update=lim)
+    ref = o.position_correction(d["obj"], d["pos"][order].copy(), d["sp"], d["sp"] * d["op"],
+                                o._amplitudes[order], d["pos"][order], 0.5,
+                                max_position_update_distance=lim * norm)
+    np.testing.assert_allclose(pos[order], ref, rtol=0, atol=1e-4 * max(lim, 1e-3))
+    # step lengths read back from float32 positions near 100 px: +-2e-5 px
+    assert np.max(np.linalg.norm(d["pos"][order] - pos[order], axis=1)) <= lim + 2e-5
+    # total-distance limit: half of the batch starts 1 px from its initial position
+    init = d["pos"].copy()
+    init[order[::2], 0] -= 1.0
+    pos, _, _ = _emu_positions(emu, d, clip_total=1, max_total=0.5, initial=init)
+    ref = o.position_correction(d["obj"], d["pos"][order].copy(), d["sp"], d["sp"] * d["op"],
+                                o._amplitudes[order], init[order], 0.5,
+                                max_position_total_distance=0.5 * norm)
+    np.testing.assert_array_equal(pos[order[::2]], d["pos"][order[::2]])
+    np.testing.assert_allclose(pos[order], ref, rtol=0, atol=1e-4)

@@ -198,3 +198,37 @@ def test_short_training_lowers_loss(dev):
     losses = [l for _, l in hist]
     assert all(np.isfinite(losses))
     assert losses[-1] < losses[0]
+
+
+def test_judge_keeps_flat_start_when_learned_is_worse(dev):
+    """The solve judges the learned start: a network whose amplitude output is
+    0.5 (exit wave halved, data error up) must fall back to the flat start;
+    the shipped model must be kept on an object of its training distribution."""
+    from py4dstem_amd import SingleslicePtychography, _lib
+    from py4dstem_amd._lib import ptr, stream_handle
+    from py4dstem_amd.learned_init import ARCH, ConvNet, conv_desc, evaluation_workload
+    say("initialiser judge / flat fallback")
+    w = evaluation_workload(3001, scan=32, object_px=256)
+    bad = ConvNet(dev, seed=1)
+    last = bad.layers[-1]
+    last["w"].zero_()
+    last["b"].zero_()
+    last["b"][0] = 0.5
+    _lib.call("pty_conv_pack_weights", conv_desc(len(ARCH) - 1, 1), ptr(last["w"]), ptr(last["wf"]),
+              ptr(last["wb"]), stream_handle())
+    p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+    p.preprocess(**w["preprocess"])
+    flat = p._object_initial.copy()
+    p.initialize_object_learned(bad)
+    info = p.learned_init_info
+    say(f"bad network: {info}")
+    assert not info["used_learned"] and info["error_learned"] > info["error_flat"]
+    np.testing.assert_array_equal(p._object_initial, flat)
+    q = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+    q.preprocess(**w["preprocess"])
+    q.initialize_object_learned()
+    say(f"shipped model: {q.learned_init_info}")
+    assert q.learned_init_info["used_learned"]
+    assert not np.array_equal(q._object_initial, flat)
+    q.reconstruct(num_iter=1, reset=True, seed_random=0, **w["reconstruct"])
+    assert np.isfinite(q.error)

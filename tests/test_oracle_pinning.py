@@ -314,3 +314,189 @@ def test_probe_gradient_true_adjoint_fp64_finite_differences(tiny):
     sp, op, chi, _ = o.forward(obj64, probe64, ipos, amps)
     np.testing.assert_allclose(g_int, -2 * 256 * np.sum(np.conj(op) * chi, axis=0), rtol=1e-10,
                                atol=1e-10 * np.abs(g_int).max())
+
+
+# ---------------------------------------------------------------------------
+# position correction and position constraints
+# ---------------------------------------------------------------------------
+def _pos_problem(mode="c64", S=16, seed=21):
+    """Tiny problem with calibrations set: S px frames at 0.4 A on a 48^2
+    object, smooth phase object with a little amplitude contrast."""
+    rng = np.random.default_rng(seed)
+    o = po.OracleSingleslicePtychography(energy=300e3, mode=mode)
+    o._region_of_interest_shape = np.array([S, S])
+    o._amplitudes_shape = np.array([S, S])
+    lam = po.electron_wavelength_angstrom(300e3)
+    dk = lam * 1e3 / (0.4 * S)  # mrad per pixel giving 0.4 A real-space sampling
+    o._angular_sampling = (dk, dk)
+    probe = po.ComplexProbeOracle(300e3, (S, S), (0.4, 0.4), 20.0, 2.0,
+                                  parameters={"defocus": 40.0}).build()._array * 30
+    phase = scipy.ndimage.gaussian_filter(rng.standard_normal((48, 48)), 1.5)
+    amp = 1 - 0.1 * scipy.ndimage.gaussian_filter(rng.random((48, 48)), 2)
+    obj = (amp * np.exp(1j * phase / phase.std())).astype(np.complex64)
+    pos = rng.uniform(12, 36, (10, 2)).astype(np.float32)
+    return o, probe.astype(np.complex64), obj, pos
+
+
+def _dft2_loops(x):
+    """Plain DFT matrices in float64 (no FFT library)."""
+    S = x.shape[-1]
+    k = np.arange(S)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / S)
+    return W @ x.astype(np.complex128) @ W.T
+
+
+def test_position_partials_brute_force_dft_and_rolled_object():
+    o, probe, obj, pos = _pos_problem(mode="c128", S=8)
+    S = 8
+    sp = po.fft_shift(probe[:S, :S] if probe.shape[0] != S else probe, pos - np.round(pos), "c128")
+    amps = np.random.default_rng(3).random((len(pos), S, S)).astype(np.float32)
+    rows, cols = _brute_indices(pos, S, obj.shape)
+    overlap = np.stack([sp[b] * obj[np.ix_(rows[b], cols[b])] for b in range(len(pos))])
+    d, pdx, pdy = o.position_partials(obj, pos, sp, overlap, amps)
+    down = np.roll(obj, -1, axis=0)   # down[r, c] = obj[r + 1, c]
+    right = np.roll(obj, -1, axis=1)  # right[r, c] = obj[r, c + 1]
+    for b in range(len(pos)):
+        F = _dft2_loops(overlap[b])
+        Fx = _dft2_loops(sp[b] * down[np.ix_(rows[b], cols[b])])
+        Fy = _dft2_loops(sp[b] * right[np.ix_(rows[b], cols[b])])
+        ref_d = amps[b].astype(np.float64) ** 2 - np.abs(F) ** 2
+        ref_x = 2 * np.real((F - Fx) * np.conj(F))
+        ref_y = 2 * np.real((F - Fy) * np.conj(F))
+        assert rel(d[b], ref_d.ravel()) < 1e-6
+        assert rel(pdx[b], ref_x.ravel()) < 1e-6
+        assert rel(pdy[b], ref_y.ravel()) < 1e-6
+        # the two axes are not interchangeable on a random object
+        assert rel(pdx[b], ref_y.ravel()) > 1e-2
+
+
+def test_position_correction_recovers_planted_least_squares_step():
+    """Measured intensities |F|^2 + alpha p_x + beta p_y: the least-squares
+    step is exactly (alpha, beta) per position, scaled by the step size and
+    subtracted from the positions."""
+    o, probe, obj, pos = _pos_problem(mode="c128")
+    S = 16
+    sp = po.fft_shift(probe, pos - np.round(pos), "c128")
+    rows, cols = po.patch_indices(pos, (S, S), obj.shape)
+    overlap = sp * obj[rows, cols]
+    _, pdx, pdy = o.position_partials(obj, pos, sp, overlap, np.zeros((len(pos), S, S)))
+    rng = np.random.default_rng(4)
+    ab = rng.uniform(-1, 1, (len(pos), 2)) * 1e-3
+    F = np.fft.fft2(overlap.astype(np.complex128))
+    I = np.abs(F).reshape(len(pos), -1) ** 2 + ab[:, :1] * pdx + ab[:, 1:] * pdy
+    assert I.min() > 0
+    amps = np.sqrt(I).reshape(len(pos), S, S)  # float64 amplitudes keep the plant exact
+    out = o.position_correction(obj, pos.astype(np.float64), sp, overlap, amps, pos, 0.5)
+    np.testing.assert_allclose(out, pos - 0.5 * ab, rtol=0, atol=1e-9)
+
+
+def test_position_correction_clipping_closed_forms():
+    o, probe, obj, pos = _pos_problem(mode="c128")
+    S = 16
+    sp = po.fft_shift(probe, pos - np.round(pos), "c128")
+    rows, cols = po.patch_indices(pos, (S, S), obj.shape)
+    overlap = sp * obj[rows, cols]
+    _, pdx, pdy = o.position_partials(obj, pos, sp, overlap, np.zeros((len(pos), S, S)))
+    ab = np.tile([[3e-3, -4e-3]], (len(pos), 1))  # |step| = 0.5 * 5e-3
+    F = np.fft.fft2(overlap.astype(np.complex128))
+    I = np.abs(F).reshape(len(pos), -1) ** 2 + ab[:, :1] * pdx + ab[:, 1:] * pdy
+    I = np.maximum(I, 0)
+    amps = np.sqrt(I).reshape(len(pos), S, S)
+    ref = pos.astype(np.float64) - 0.5 * ab
+    norm = np.hypot(*o.sampling)
+    # update clip: max distance in A divided by |sampling| (pixels)
+    out = o.position_correction(obj, pos.astype(np.float64), sp, overlap, amps, pos, 0.5,
+                                max_position_update_distance=1e-3 * norm)
+    step = pos - out
+    np.testing.assert_allclose(np.linalg.norm(step, axis=1), 1e-3, rtol=1e-5)
+    np.testing.assert_allclose(step / np.linalg.norm(step, axis=1)[:, None],
+                               ab / np.linalg.norm(ab, axis=1)[:, None], atol=1e-3)
+    # a generous clip changes nothing
+    out = o.position_correction(obj, pos.astype(np.float64), sp, overlap, amps, pos, 0.5,
+                                max_position_update_distance=1.0 * norm)
+    np.testing.assert_allclose(out, ref, atol=1e-6)
+    # total-distance limit: positions already 0.5 px from their initial
+    # values move no further; those at the initial positions do move
+    init = pos.astype(np.float64).copy()
+    init[::2, 0] -= 0.5
+    out = o.position_correction(obj, pos.astype(np.float64), sp, overlap, amps, init, 0.5,
+                                max_position_total_distance=0.1 * norm)
+    np.testing.assert_array_equal(out[::2], pos[::2].astype(np.float64))
+    np.testing.assert_allclose(out[1::2], ref[1::2], atol=1e-6)
+
+
+def test_position_correction_moves_displaced_positions_towards_truth():
+    o, probe, obj, pos = _pos_problem(mode="c128", seed=8)
+    S = 16
+    rng = np.random.default_rng(9)
+    true = pos.astype(np.float32)
+    rows, cols = po.patch_indices(true, (S, S), obj.shape)
+    amps = np.abs(np.fft.fft2(po.fft_shift(probe, true - np.round(true), "c128")
+                              * obj[rows, cols])).astype(np.float32)
+    shift = rng.choice([-0.3, 0.3], (len(pos), 2)).astype(np.float32)
+    cur = true + shift
+    sp = po.fft_shift(probe, cur - np.round(cur), "c128")
+    rows, cols = po.patch_indices(cur, (S, S), obj.shape)
+    out = o.position_correction(obj, cur.copy(), sp, sp * obj[rows, cols], amps, cur, 1.0)
+    before = np.abs(cur - true)
+    after = np.abs(out - true)
+    assert np.mean(after) < 0.5 * np.mean(before)
+    assert np.mean(np.sign(out - cur) == -np.sign(shift)) > 0.9
+
+
+def test_positions_centre_of_mass_constraint():
+    rng = np.random.default_rng(2)
+    init = rng.uniform(10, 50, (40, 2)).astype(np.float32)
+    cur = init + rng.normal(0, 0.5, init.shape).astype(np.float32) + np.float32(1.5)
+    # flag as passed by the loop: False applies the centre-of-mass step
+    out = po.OracleSingleslicePtychography.positions_constraints(cur.copy(), init, False, False,
+                                                                 False)
+    np.testing.assert_allclose(out.mean(0), init.mean(0), atol=1e-4)
+    np.testing.assert_allclose(out - out.mean(0), cur - cur.mean(0), atol=1e-4)
+    same = po.OracleSingleslicePtychography.positions_constraints(cur.copy(), init, False, True,
+                                                                  False)
+    np.testing.assert_array_equal(same, cur)
+    fixed = po.OracleSingleslicePtychography.positions_constraints(cur.copy(), init, True, False,
+                                                                   True)
+    np.testing.assert_array_equal(fixed, cur)
+
+
+def _planted_affine():
+    g = np.arange(20, dtype=np.float32) * 4
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    p0 = np.stack([gx.ravel(), gy.ravel()], 1).astype(np.float32) + 30
+    tf = (1.02, 0.97, 0.015, 0.03, np.float32(0.6), np.float32(-0.4))
+    origin = p0.mean(0)
+    return p0, tf, origin
+
+
+def test_affine_fit_recovers_planted_map_exactly():
+    p0, tf, origin = _planted_affine()
+    p1 = po.affine_apply(tf, p0, origin)
+    # hand-derived matrix rows: s0 (cos, -sin); s1 (h cos + sin, cos - h sin)
+    c, s = np.cos(0.03), np.sin(0.03)
+    np.testing.assert_allclose(po.affine_matrix(tf),
+                               [[1.02 * c, -1.02 * s],
+                                [0.97 * (0.015 * c + s), 0.97 * (c - 0.015 * s)]], rtol=1e-6)
+    est, err = po.affine_fit(p0, p1, origin)
+    np.testing.assert_allclose(est[:4], tf[:4], atol=2e-5)
+    np.testing.assert_allclose([est[4], est[5]], [0.6, -0.4], atol=2e-3)
+    assert err < 0.05
+    # degenerate input: identity (the reference catches the singular solve)
+    est, _ = po.affine_fit(p0[:1], p1[:1], origin)
+    assert est == po.AFFINE_IDENTITY
+
+
+def test_affine_ransac_ignores_outliers():
+    p0, tf, origin = _planted_affine()
+    p1 = po.affine_apply(tf, p0, origin).astype(np.float32)
+    rng = np.random.default_rng(6)
+    bad = rng.choice(len(p0), len(p0) // 10, replace=False)
+    p1[bad] += rng.uniform(30, 60, (len(bad), 2)).astype(np.float32)
+    np.random.seed(0)
+    est, fitness = po.affine_ransac(p0, p1, origin, min_sample=len(p0) // 10)
+    np.testing.assert_allclose(est[:4], tf[:4], atol=1e-4)
+    assert np.isfinite(fitness)
+    # the plain least-squares fit is pulled off by the outliers
+    est_ls, _ = po.affine_fit(p0, p1, origin)
+    assert abs(est_ls[0] - tf[0]) > 1e-3 or abs(est_ls[4] - tf[4]) > 1.0
