@@ -14,8 +14,12 @@ autotuner) advanced together by MultiPtychography (one launch per stage for
 all problems, one graph replay per iteration) against the same solves run one
 after another with reconstruct(); prints aggregate position-updates/s.
 
+``--workload positions`` times the production workload with position
+correction on (reconstruct(fix_positions=False)).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds T]
-                       [--workload production|many|kernels|learned-init] [--problems P]
+                       [--workload production|many|kernels|learned-init|positions]
+                       [--problems P]
 """
 
 from __future__ import annotations
@@ -282,6 +286,122 @@ def bench_learned_init(args):
     print(json.dumps(out, default=float), flush=True)
 
 
+def bench_positions(args):
+    """Production workload with position correction on (fix_positions=False):
+    GD iterations with the per-batch position step, and the position kernel
+    timed live on one mini-batch."""
+    import torch
+
+    from py4dstem_amd import kernels as K
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import production_workload
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    torch.cuda.set_device(0)
+    w = production_workload(seed=0, scan=(args.scan, args.scan))
+    p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+    p.preprocess(**w["preprocess"])
+    S = int(p._region_of_interest_shape[0])
+    Px, Py = p._object_shape
+    N = p._num_diffraction_patterns
+    B = w["reconstruct"]["max_batch_size"]
+    recon = dict(w["reconstruct"], progress_bar=False, fix_positions=False,
+                 positions_step_size=0.5)
+    if args.warmup:
+        p.reconstruct(**dict(recon, num_iter=args.warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.reconstruct(**dict(recon, num_iter=args.steps, reset=False))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pos_per_s = N * args.steps / dt
+    log(f"{args.steps} iterations with position correction in {dt:.3f} s -> "
+        f"{pos_per_s:,.0f} pos-updates/s")
+
+    dev = torch.device("cuda", 0)
+    order = torch.from_numpy(np.random.default_rng(1).permutation(N)[:B].astype(np.int32)).to(dev)
+    phat = K.probe_spectrum(p._probe_dev)
+    ps, chi, err, far = K.forward_adjoint(phat, p._object_dev, p._positions_dev,
+                                          p._amplitudes_dev, order, with_far=True)
+    positions = p._positions_dev.clone()
+    initial = p._positions_initial_dev
+    pp = K.position_params(positions, initial, 0.5)
+    stream = torch.cuda.current_stream()
+    K.position_correction(pp, ps, far, p._object_dev, p._amplitudes_dev, order)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.kernel_reps):
+        K.position_correction(pp, ps, far, p._object_dev, p._amplitudes_dev, order)
+    e1.record(stream)
+    e1.synchronize()
+    k_avg = e0.elapsed_time(e1) * 1e-3 / args.kernel_reps
+    # shifted probe + far field + amplitudes + two shifted object patches
+    per_pos = 36 * S * S
+    achieved = per_pos * B / k_avg / 1e9
+    log(f"k_position_correction avg {k_avg * 1e6:.1f} us per launch of {B} positions")
+
+    cpu_rate = sample = None
+    if args.cpu_seconds > 0:
+        from oracle import OracleSingleslicePtychography
+        o = OracleSingleslicePtychography(energy=p._energy)
+        o._region_of_interest_shape = p._region_of_interest_shape
+        o._amplitudes_shape = p._amplitudes_shape
+        o._angular_sampling = p._angular_sampling
+        obj = p._object_initial.copy()
+        probe = p._probe_initial.copy()
+        pos_h = p._positions_px.copy()
+        rng = np.random.default_rng(0)
+        done, b = 0, 256
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            idx = rng.permutation(N)[:b]
+            pos = pos_h[idx]
+            sp, op, ew, _ = o.forward(obj, probe, pos, p._amplitudes[idx])
+            o.adjoint(obj, probe, op, sp, pos, ew, 0.5, 1, False)
+            pos_h[idx] = o.position_correction(obj, pos, sp, sp * op, p._amplitudes[idx],
+                                               p._positions_px_initial[idx], 0.5)
+            done += b
+        cdt = time.perf_counter() - t0
+        cpu_rate = done / cdt
+        sample = (f"{done // b} mini-batches of {b} positions (forward, adjoint with probe "
+                  f"update, position correction), NumPy complex64 oracle, {cdt:.1f} s, one "
+                  f"process; host {cpu_model()}")
+        log(f"cpu baseline {cpu_rate:,.1f} pos-updates/s")
+
+    out = {
+        "metric": "position-updates/s",
+        "value": round(pos_per_s, 1),
+        "unit": "position-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (complex64)",
+        "data": "synthetic (seeded Gaussian-atom object, simulated Poisson patterns)",
+        "config": {
+            "workload": f"single-slice GD ptychography with position correction, "
+                        f"{args.scan}x{args.scan} scan, {S}x{S} px patterns, {Px}x{Py} object, "
+                        f"mini-batch {B}, probe update",
+            "scan_positions": N, "frame_px": S, "object_px": [Px, Py], "mini_batch": B,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_position_correction", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "kernel_avg_us": round(k_avg * 1e6, 2), "bytes_per_update": per_pos,
+        },
+        "cpu_baseline": {
+            "value": round(cpu_rate, 2) if cpu_rate is not None else None,
+            "unit": "position-updates/s", "cores": 1, "kind": "port", "sample": sample,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,8 +410,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--scan", type=int, default=128)
-    ap.add_argument("--workload", choices=("production", "many", "kernels", "learned-init"),
-                    default="production")
+    ap.add_argument("--workload", choices=("production", "many", "kernels", "learned-init",
+                                           "positions"), default="production")
     ap.add_argument("--problems", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -304,6 +424,8 @@ def main():
         return bench_kernels(args)
     if args.workload == "learned-init":
         return bench_learned_init(args)
+    if args.workload == "positions":
+        return bench_positions(args)
 
     import torch
 

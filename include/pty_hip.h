@@ -63,7 +63,11 @@ int pty_probe_spectrum(const void* probe, void* probe_hat, int S, pty_stream_t s
  *   ps[b]  = P_shift = ifft2(probe_hat * ramp(frac))    (natural order)
  *   chi[b] = ifft2(A e^{i angle F} - F), F = fft2(P_shift * O_patch)
  *   err[b] = sum (A - |F|)^2                             (float64)
- *   far[b] = F in kernel layout                          (skipped if NULL)
+ *   far[b] = F in the far-field staging layout           (skipped if NULL):
+ *            with N1 = 8 (S=64) or 16 (S=128) and T = S*S/N1, element
+ *            far[b][j*T + t] holds the kernel-layout entry at row t/(S/N1),
+ *            column N1*(t % (S/N1)) + j (consecutive threads write
+ *            consecutive words; only the position-correction pass reads it)
  * Patches are S x S, corner-centred. */
 int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px, int Py,
                               const float* positions, int N, const float* amplitudes,
@@ -225,7 +229,8 @@ int pty_gd_run_positions(const pty_problem* problem, const pty_gd_params* params
 
 /* One batch of _position_correction (phase/ptychographic_methods.py:2071-2225):
  * ps = the batch's shifted probes and far = fft2(P_shift * O_patch) of the
- * forward pass (pty_forward_adjoint_batch's outputs), object = the object
+ * forward pass (pty_forward_adjoint_batch's outputs, far in its staging
+ * layout), object = the object
  * after the update; updates pp->positions[order[b]].  sums (optional, device,
  * [B][5] float64): sum p_x^2, p_x p_y, p_y^2, p_x d, p_y d per slot. */
 int pty_position_correction_batch(const pty_position_params* pp, const void* ps, const void* far,

@@ -164,6 +164,10 @@ template <int S> struct Frame {
   static PTY_HD int row_q(int tid) { return tid % N2; }
   static PTY_HD int col_line(int tid) { return tid % S; }
   static PTY_HD int col_q(int tid) { return tid / S; }
+  // far-field staging layout: element j of row-pass thread t sits at j*T + t
+  // (consecutive lanes write consecutive words); it holds the kernel-layout
+  // position (row_line(t), N1 * row_q(t) + j)
+  static PTY_HD int far_index(int t, int j) { return j * T + t; }
 };
 
 PTY_HD int wrapi(int a, int n) {
@@ -226,7 +230,8 @@ struct FusedArgs {
   cf* ps_out;              // [B][S][S] shifted probe P_shift      (natural)
   cf* chi_out;             // [B][S][S] chi = ifft2(A e^{i angle F} - F) (natural)
   double* err;             // [B] sum (A - |F|)^2 per position
-  cf* fwd_out;             // optional [B][S][S] far field F (permuted), or null
+  cf* fwd_out;             // optional [B][N1][T] far field F, element j of thread t of the
+                           // row pass at [j][t] (coalesced; see far_index), or null
   const cf* chi_in;        // probe-gradient pass: [B][S][S] chi (natural)
   cf* pgrad_out;           // probe-gradient pass: [B][S][S] (kernel layout)
   float* sim_out;          // simulation pass: [B][S][S] |F|^2 (natural)
@@ -400,12 +405,12 @@ template <int S> struct Fused {
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
-    cf* far = a.fwd_out ? a.fwd_out + (size_t)b * S * S + rr * S + N1 * t : nullptr;
+    cf* far = a.fwd_out ? a.fwd_out + (size_t)b * S * S + tid : nullptr;
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
       const cf f = r.v[j];
-      if (far) far[j] = f;
+      if (far) far[(size_t)j * T] = f;
       const float amp = r.amp[j];
       const float m2 = cabs2(f);
       cf g;
@@ -573,7 +578,7 @@ template <int S, class Run> PTY_HD void probe_grad_position(Run& run, const Fuse
 // ---------------------------------------------------------------------------
 struct PosArgs {
   const cf* ps;                    // [B][S][S] shifted probes of the batch (natural)
-  const cf* far;                   // [B][S][S] F of the fused pass (permuted)
+  const cf* far;                   // [B][N1][T] F of the fused pass (staging layout)
   const cf* object;                // [Px][Py] object after the batch's update
   const float* amps;               // [N][S][S] measured amplitudes (permuted)
   float* positions;                // [N][2] slot b updates its own entry only
@@ -641,6 +646,9 @@ template <int S> struct PosCorr {
   static constexpr int N1 = F::N1, N2 = F::N2, T = F::T;
   static constexpr int LDS_TOTAL = F::LDS_WORDS;
 
+  // Register budget (1024 threads: 128 VGPRs): F is loaded at set-up and
+  // stays live; the amplitudes are loaded only for the second transform's
+  // last row passes and the sums exist only in the final phase.
   struct Regs {
     cf v[N1];
     cf f[N1];      // F at this thread's row-pass-B positions
@@ -650,7 +658,7 @@ template <int S> struct PosCorr {
     int x0, y0, idx;
   };
 
-  // anchor of slot b; F and the amplitudes stay in flight across the FFT
+  // anchor of slot b; the loads of F stay in flight across the first transform
   static PTY_HD void p_setup(int tid, const PosArgs& a, int b, Regs& r) {
     int idx = a.order ? a.order[b] : b;
     PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
@@ -658,16 +666,19 @@ template <int S> struct PosCorr {
     r.idx = idx;
     r.x0 = wrapi((int)rintf(a.positions[2 * idx]), a.Px);
     r.y0 = wrapi((int)rintf(a.positions[2 * idx + 1]), a.Py);
-    const size_t o = F::row_line(tid) * S + N1 * F::row_q(tid);
-    const float* A = a.amps + (size_t)idx * S * S + o;
-    const cf* Fb = a.far + (size_t)b * S * S + o;
+    const cf* Fb = a.far + (size_t)b * S * S + tid;
 #pragma unroll
-    for (int j = 0; j < N1; ++j) {
-      r.amp[j] = A[j];
-      r.f[j] = Fb[j];
-    }
+    for (int j = 0; j < N1; ++j) r.f[j] = Fb[(size_t)j * T];
+  }
+
+  // row pass A of the second transform; the amplitude loads issued first
+  // complete behind it
+  template <class L> static PTY_HD void p_rowA_amps(int tid, const PosArgs& a, Regs& r, L lds,
+                                                    const cf* tw) {
+    const float* A = a.amps + (size_t)r.idx * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
 #pragma unroll
-    for (int k = 0; k < POS_SUMS; ++k) r.s[k] = 0.0;
+    for (int j = 0; j < N1; ++j) r.amp[j] = A[j];
+    Fused<S>::template p_rowA<false>(tid, r, lds, tw);
   }
 
   // column pass A of fft2(P_shift * O[rows + DX, cols + DY]) from global memory
@@ -720,17 +731,19 @@ template <int S> struct PosCorr {
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
       const double px = r.px[j];
       const double py = partial(r.f[j], r.v[j]);
       const double d = r.amp[j] * r.amp[j] - cabs2(r.f[j]);  // float32 difference
-      r.s[0] += px * px;
-      r.s[1] += px * py;
-      r.s[2] += py * py;
-      r.s[3] += px * d;
-      r.s[4] += py * d;
+      s0 += px * px;
+      s1 += px * py;
+      s2 += py * py;
+      s3 += px * d;
+      s4 += py * d;
     }
+    r.s[0] = s0; r.s[1] = s1; r.s[2] = s2; r.s[3] = s3; r.s[4] = s4;
   }
 };
 
@@ -755,7 +768,7 @@ template <int S, class Run> PTY_HD void position_correction_position(Run& run, c
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { FK::template p_colB<false>(tid, r, lds, tw); });
   run.barrier();
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { FK::template p_rowA<false>(tid, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_rowA_amps(tid, a, r, lds, tw); });
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_partial_y_sums(tid, r, lds, tw); });
   double tot[POS_SUMS];

@@ -76,7 +76,8 @@ def probe_spectrum(probe):
 
 def forward_adjoint(probe_hat, obj, positions, amplitudes_k, order, with_far=False):
     """Fused forward pass for the slots of ``order``; returns
-    (P_shift, chi, err[, far]) with P_shift/chi natural (B, S, S)."""
+    (P_shift, chi, err[, far]) with P_shift/chi natural (B, S, S) and far
+    in the staging layout (far_to_kernel_layout converts it)."""
     S = probe_hat.shape[-1]
     check_frame(S)
     _req(probe_hat, torch.complex64, (S, S), "probe_hat")
@@ -169,6 +170,16 @@ def gd_run(problem, params, orders, errors, workspace):
     import ctypes
     _lib.call("pty_gd_run", ctypes.byref(problem), ctypes.byref(params), ptr(orders), ptr(errors),
               ptr(workspace), workspace.numel(), stream_handle())
+
+
+def far_to_kernel_layout(far):
+    """(B, S, S) far field from the staging layout of forward_adjoint to the
+    kernel layout (element j of row-pass thread t -> row t // (S/N1),
+    column N1 * (t % (S/N1)) + j)."""
+    S = far.shape[-1]
+    N1 = 16 if S == 128 else 8
+    x = far.reshape(far.shape[0], N1, S, S // N1)  # [b][j][row][q], t = row * (S/N1) + q
+    return x.permute(0, 2, 3, 1).reshape(far.shape[0], S, S).contiguous()
 
 
 def position_params(positions, positions_initial, step_size, max_update_px=None,

@@ -52,3 +52,12 @@ def small_case():
     o = OracleSingleslicePtychography(datacube=w["datacube"], **init)
     o.preprocess(**w["preprocess"])
     return w, o
+
+
+def far_staging_to_kernel(far):
+    """(B, S, S) far field from the fused pass's staging layout (element j of
+    row-pass thread t at [j][t]) to the kernel layout (NumPy)."""
+    S = far.shape[-1]
+    N1 = 16 if S == 128 else 8
+    x = far.reshape(far.shape[0], N1, S, S // N1)
+    return np.ascontiguousarray(x.transpose(0, 2, 3, 1).reshape(far.shape[0], S, S))

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle.ptycho_oracle as po
-from tests.conftest import cptr, rel
+from tests.conftest import cptr, far_staging_to_kernel, rel
 
 
 def kidx(emu, S):
@@ -59,7 +59,7 @@ def test_fused_position_matches_oracle_forward(emu, small_case):
     assert abs(err.sum() / e - 1) < 1e-6
     F = np.fft.fft2(sp * op)
     farn = np.empty_like(far)
-    farn[:, k[:, None], k[None, :]] = far
+    farn[:, k[:, None], k[None, :]] = far_staging_to_kernel(far)
     assert rel(farn, F) < 1e-6
 
 

@@ -88,6 +88,10 @@ def test_fused_forward_matches_oracle(dev, small_case):
     assert rel(ps.cpu().numpy(), sp) < 1e-6
     assert rel(chi.cpu().numpy(), ew) < 2e-5
     assert abs(err.sum().item() / e - 1) < 1e-6
+    # far field from the staging layout, in kernel layout, against fft2
+    k = K().kernel_index(S)
+    F = np.fft.fft2((sp * op).astype(np.complex128))[:, k][:, :, k]
+    assert rel(K().far_to_kernel_layout(far).cpu().numpy(), F) < 1e-6
 
 
 def test_fused_forward_matches_oracle_128(dev):
