@@ -15,10 +15,11 @@ all problems, one graph replay per iteration) against the same solves run one
 after another with reconstruct(); prints aggregate position-updates/s.
 
 ``--workload positions`` times the production workload with position
-correction on (reconstruct(fix_positions=False)).
+correction on (reconstruct(fix_positions=False)); ``--workload projection``
+with the DM_AP(0.5) projection-set method (all positions per update).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds T]
-                       [--workload production|many|kernels|learned-init|positions]
+                       [--workload production|many|kernels|learned-init|positions|projection]
                        [--problems P]
 """
 
@@ -402,6 +403,88 @@ def bench_positions(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_projection(args):
+    """Production workload with a projection-set method (DM_AP, alpha 0.5;
+    all positions per update, exit waves resident in HBM); k_projection timed
+    live on 1024 positions."""
+    import torch
+
+    from py4dstem_amd import kernels as K
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import production_workload
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    torch.cuda.set_device(0)
+    w = production_workload(seed=0, scan=(args.scan, args.scan))
+    p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+    p.preprocess(**w["preprocess"])
+    S = int(p._region_of_interest_shape[0])
+    Px, Py = p._object_shape
+    N = p._num_diffraction_patterns
+    recon = dict(w["reconstruct"], progress_bar=False, max_batch_size=None,
+                 reconstruction_method="DM_AP", reconstruction_parameter=0.5)
+    if args.warmup:
+        p.reconstruct(**dict(recon, num_iter=args.warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.reconstruct(**dict(recon, num_iter=args.steps, reset=False))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pos_per_s = N * args.steps / dt
+    log(f"{args.steps} DM_AP iterations in {dt:.3f} s -> {pos_per_s:,.0f} pos-updates/s; "
+        f"errors {p.error_iterations[-3:]}")
+
+    dev = torch.device("cuda", 0)
+    B = 1024
+    order = torch.arange(B, dtype=torch.int32, device=dev)
+    exits = p._exit_waves[:B].clone()
+    jp = K.projection_params(exits, -0.5, 1, 1.5, exit_valid=True)
+    phat = K.probe_spectrum(p._probe_dev)
+    K.projection_batch(phat, p._object_dev, p._positions_dev, p._amplitudes_dev, order, jp)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.kernel_reps):
+        K.projection_batch(phat, p._object_dev, p._positions_dev, p._amplitudes_dev, order, jp)
+    e1.record(stream)
+    e1.synchronize()
+    k_avg = e0.elapsed_time(e1) * 1e-3 / args.kernel_reps
+    # amplitudes 4, object patch 8, exit wave read 8 + partial write 8 + read 8 + write 8,
+    # P_shift write 8 (x S^2 bytes)
+    per_pos = 52 * S * S
+    achieved = per_pos * B / k_avg / 1e9
+    log(f"k_projection avg {k_avg * 1e6:.1f} us per launch of {B} positions")
+    out = {
+        "metric": "position-updates/s",
+        "value": round(pos_per_s, 1),
+        "unit": "position-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (complex64)",
+        "data": "synthetic (seeded Gaussian-atom object, simulated Poisson patterns)",
+        "config": {
+            "workload": f"single-slice ptychography, DM_AP(0.5) projection sets, "
+                        f"{args.scan}x{args.scan} scan, {S}x{S} px patterns, {Px}x{Py} object, "
+                        f"all positions per update, probe update",
+            "scan_positions": N, "frame_px": S, "object_px": [Px, Py],
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_projection", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "kernel_avg_us": round(k_avg * 1e6, 2), "bytes_per_update": per_pos,
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,7 +494,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--scan", type=int, default=128)
     ap.add_argument("--workload", choices=("production", "many", "kernels", "learned-init",
-                                           "positions"), default="production")
+                                           "positions", "projection"), default="production")
     ap.add_argument("--problems", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -426,6 +509,8 @@ def main():
         return bench_learned_init(args)
     if args.workload == "positions":
         return bench_positions(args)
+    if args.workload == "projection":
+        return bench_projection(args)
 
     import torch
 

@@ -241,6 +241,41 @@ int pty_position_correction_batch(const pty_position_params* pp, const void* ps,
  * (phase/ptychographic_constraints.py:1292-1311); initial_com is a host pointer. */
 int pty_positions_com(float* positions, int N, const float* initial_com, pty_stream_t stream);
 
+/* ---- projection sets ---------------------------------------------------------
+ * reconstruct(reconstruction_method='DM_AP' | 'RAAR' | 'RRR' | 'SUPERFLIP' |
+ * 'generalized-projections'): phase/ptychographic_methods.py:1637-1734
+ * (_projection_sets_fourier_projection) and :1910-1998
+ * (_projection_sets_adjoint), with the (a, b, c) of
+ * phase/phase_base_class.py:2005-2108.  Per position with overlap
+ * psi = P_shift * O_patch and previous exit wave e (psi when !exit_valid):
+ *   F = fft2(c psi + (1 - c) e),  err = sum (A - |F|)^2,
+ *   e <- (1 - a - b) e + a psi + b ifft2(A e^{i angle F});
+ * then O = scatter(conj(P_shift) e) / norm, P = sum conj(O_patch) e / norm
+ * (replacements, no step size). */
+typedef struct pty_projection_params {
+  double a, b, c;    /* projection_a, projection_b, projection_c */
+  void* exit_waves;  /* complex64 [N][S][S], natural order; slot k pairs with order[k];
+                        persistent across calls (the reference's _exit_waves) */
+  int exit_valid;    /* 0: the first iteration starts from e = psi */
+} pty_projection_params;
+
+size_t pty_projection_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);
+
+/* num_iter iterations with one batch of all N positions (params->max_batch
+ * must be N; step_size unused; normalization_min, fix_probe, fix_probe_com,
+ * pure_phase_object, shrinkage_rad and stage as for pty_gd_run).  `order` is
+ * int32 [N] (the reference uses 0..N-1). */
+int pty_projection_run(const pty_problem* problem, const pty_gd_params* params,
+                       const pty_projection_params* projection, const int* order, double* errors,
+                       void* workspace, size_t workspace_bytes, pty_stream_t stream);
+
+/* The exit-wave step of one batch (slots 0..B-1 of order; exit_waves[k] is
+ * slot k): ps[b] = P_shift (natural), err[b], exit waves advanced in place. */
+int pty_projection_batch(const void* probe_hat, const void* object, int Px, int Py,
+                         const float* positions, int N, const float* amplitudes, const int* order,
+                         int B, int S, const pty_projection_params* projection, void* ps,
+                         double* err, pty_stream_t stream);
+
 /* ---- many reconstructions together --------------------------------------- */
 
 /* One entry per independent reconstruction; params.num_iter is ignored. */

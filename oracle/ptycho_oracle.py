@@ -46,6 +46,10 @@ PINNING = {
     "positions_constraints": "scan centre of mass restored exactly; exact affine maps "
                              "recovered (with outliers) by the RANSAC fit",
     "affine_ransac": "exact recovery of planted affine maps with 10% outliers",
+    "method_parameters": "hand-derived (a, b, c) table and range errors",
+    "projection_forward": "AP output meets the modulus constraint; DM = e + P_M(2 psi - e) - psi; "
+                          "a consistent exit wave is a fixed point for every (a, b, c)",
+    "projection_adjoint": "noise-free truth with normalization_min=0 is reproduced",
     "reconstruct": "composition of the above; c64 vs c128 sensitivity measured",
 }
 
@@ -778,6 +782,88 @@ class OracleSingleslicePtychography:
         g_probe = -2.0 * S2 * np.sum(back, axis=0)
         return err, g_obj, g_probe
 
+    # -- projection sets ----------------------------------------------------
+    @staticmethod
+    def method_parameters(reconstruction_method, reconstruction_parameter=1.0,
+                          reconstruction_parameter_a=None, reconstruction_parameter_b=None,
+                          reconstruction_parameter_c=None):
+        """phase_base_class.py:2005-2108: (use_projection_scheme, a, b, c)."""
+        m = reconstruction_method
+        r = reconstruction_parameter
+        if m == "generalized-projections":
+            if (reconstruction_parameter_a is None or reconstruction_parameter_b is None
+                    or reconstruction_parameter_c is None):
+                raise ValueError("reconstruction_parameter_a/b/c must all be specified "
+                                 "when using reconstruction_method='generalized-projections'.")
+            return (True, reconstruction_parameter_a, reconstruction_parameter_b,
+                    reconstruction_parameter_c)
+        if m in ("DM_AP", "difference-map_alternating-projections"):
+            if r < 0.0 or r > 1.0:
+                raise ValueError("reconstruction_parameter must be between 0-1.")
+            return True, -r, 1, 1 + r
+        if m in ("RAAR", "relaxed-averaged-alternating-reflections"):
+            if r < 0.0 or r > 1.0:
+                raise ValueError("reconstruction_parameter must be between 0-1.")
+            return True, 1 - 2 * r, r, 2
+        if m in ("RRR", "relax-reflect-reflect"):
+            if r < 0.0 or r > 2.0:
+                raise ValueError("reconstruction_parameter must be between 0-2.")
+            return True, -r, r, 2
+        if m in ("SUPERFLIP", "charge-flipping"):
+            return True, 0, 1, 2
+        if m in ("GD", "gradient-descent"):
+            return False, None, None, None
+        raise ValueError(f"reconstruction_method {m!r} not recognized")
+
+    def projection_forward(self, obj, probe, positions_px, amplitudes, exit_waves, a, b, c):
+        """``_forward`` with use_projection_scheme (ptychographic_methods.py:
+        1736-1816 -> 1637-1734).  Returns (shifted_probes, object_patches,
+        exit_waves, error)."""
+        mode = self._mode
+        frac = positions_px - np.round(positions_px)
+        rows, cols = patch_indices(positions_px, self._region_of_interest_shape, obj.shape)
+        shifted_probes = fft_shift(probe, frac, mode)
+        object_patches = obj[rows, cols]
+        overlap = shifted_probes * object_patches
+        projection_x = 1 - a - b
+        projection_y = 1 - c
+        if exit_waves is None:
+            exit_waves = overlap.copy()
+        factor_to_be_projected = c * overlap + projection_y * exit_waves
+        fourier_projected_factor = _fft2(factor_to_be_projected, mode)
+        farfield_amplitudes = np.abs(fourier_projected_factor)
+        error = np.sum(np.abs(amplitudes - farfield_amplitudes) ** 2)
+        fourier_projected_factor = amplitudes * np.exp(1j * np.angle(fourier_projected_factor))
+        projected_factor = _ifft2(fourier_projected_factor, mode)
+        exit_waves = projection_x * exit_waves + a * overlap + b * projected_factor
+        return shifted_probes, object_patches, exit_waves, error
+
+    def projection_adjoint(self, obj, probe, object_patches, shifted_probes, positions_px,
+                           exit_waves, normalization_min, fix_probe):
+        """``_projection_sets_adjoint`` for a complex object
+        (ptychographic_methods.py:1910-1998): object and probe replaced by
+        their normalised back-projections."""
+        roi = self._region_of_interest_shape
+        shape = obj.shape
+        probe_normalization = scatter_bincount(np.abs(shifted_probes) ** 2, positions_px, roi,
+                                               shape)
+        probe_normalization = 1 / np.sqrt(
+            1e-16
+            + ((1 - normalization_min) * probe_normalization) ** 2
+            + (normalization_min * np.max(probe_normalization)) ** 2
+        )
+        obj = (scatter_bincount(np.conj(shifted_probes) * exit_waves, positions_px, roi, shape)
+               * probe_normalization)
+        if not fix_probe:
+            object_normalization = np.sum(np.abs(object_patches) ** 2, axis=0)
+            object_normalization = 1 / np.sqrt(
+                1e-16
+                + ((1 - normalization_min) * object_normalization) ** 2
+                + (normalization_min * np.max(object_normalization)) ** 2
+            )
+            probe = np.sum(np.conj(object_patches) * exit_waves, axis=0) * object_normalization
+        return obj, probe
+
     # -- position correction ------------------------------------------------
     def position_partials(self, obj, positions_px, shifted_probes, overlap, amplitudes):
         """Intensity residual and the two finite-difference intensity partials
@@ -880,8 +966,18 @@ class OracleSingleslicePtychography:
                     store_iterations=False, reset=None, positions_step_size=0.5,
                     fix_positions=True, fix_positions_com=True,
                     max_position_update_distance=None, max_position_total_distance=None,
-                    global_affine_transformation=False, **kwargs):
-        """phase/singleslice_ptychography.py:606-996 for GD."""
+                    global_affine_transformation=False, reconstruction_method="gradient-descent",
+                    reconstruction_parameter=1.0, reconstruction_parameter_a=None,
+                    reconstruction_parameter_b=None, reconstruction_parameter_c=None, **kwargs):
+        """phase/singleslice_ptychography.py:606-996 (gradient descent and the
+        projection sets)."""
+        use_projection, pa, pb, pc = self.method_parameters(
+            reconstruction_method, reconstruction_parameter, reconstruction_parameter_a,
+            reconstruction_parameter_b, reconstruction_parameter_c)
+        if use_projection and max_batch_size is not None:
+            raise ValueError("Stochastic object/probe updating is inconsistent with 'DM_AP', "
+                             "'RAAR', 'RRR', and 'SUPERFLIP'. Use reconstruction_method='GD' or "
+                             "set max_batch_size=None.")
         # ptychographic_methods.py:252-294
         if store_iterations and (not hasattr(self, "object_iterations") or reset):
             self.object_iterations = []
@@ -891,15 +987,20 @@ class OracleSingleslicePtychography:
             self._object = self._object_initial.copy()
             self._probe = self._probe_initial.copy()
             self._positions_px = self._positions_px_initial.copy()
+            self._exit_waves = None
         elif reset is None:
             if hasattr(self, "error"):
                 warnings.warn("Continuing reconstruction from previous result. "
                               "Use reset=True for a fresh start.", UserWarning)
             else:
                 self.error_iterations = []
+                self._exit_waves = None
         else:
             if not hasattr(self, "error"):
                 self.error_iterations = []
+                self._exit_waves = None
+        if not hasattr(self, "_exit_waves"):
+            self._exit_waves = None
 
         N = self._num_diffraction_patterns
         shuffled_indices = np.arange(N)
@@ -913,15 +1014,25 @@ class OracleSingleslicePtychography:
         error = 0.0
         for _ in range(num_iter):
             error = 0.0
-            np.random.shuffle(shuffled_indices)
+            if not use_projection:
+                np.random.shuffle(shuffled_indices)
             for start, end in generate_batches(N, max_batch=max_batch_size):
                 idx = shuffled_indices[start:end]
                 pos = self._positions_px[idx]
-                sp, op, ew, err = self.forward(self._object, self._probe, pos,
-                                               self._amplitudes[idx])
-                self._object, self._probe = self.adjoint(
-                    self._object, self._probe, op, sp, pos, ew, step_size,
-                    normalization_min, fix_probe)
+                if use_projection:
+                    sp, op, self._exit_waves, err = self.projection_forward(
+                        self._object, self._probe, pos, self._amplitudes[idx], self._exit_waves,
+                        pa, pb, pc)
+                    self._object, self._probe = self.projection_adjoint(
+                        self._object, self._probe, op, sp, pos, self._exit_waves,
+                        normalization_min, fix_probe)
+                    ew = None
+                else:
+                    sp, op, ew, err = self.forward(self._object, self._probe, pos,
+                                                   self._amplitudes[idx])
+                    self._object, self._probe = self.adjoint(
+                        self._object, self._probe, op, sp, pos, ew, step_size,
+                        normalization_min, fix_probe)
                 if not fix_positions:  # singleslice_ptychography.py:913-926
                     self._positions_px[idx] = self.position_correction(
                         self._object, pos, sp, sp * op, self._amplitudes[idx],
@@ -943,4 +1054,6 @@ class OracleSingleslicePtychography:
         self.object = np.asarray(self._object)
         self.probe = np.fft.fftshift(self._probe, axes=(-2, -1))
         self.error = float(error)
+        if not use_projection:  # singleslice_ptychography.py:990-992
+            self._exit_waves = None
         return self

@@ -41,6 +41,11 @@ class PtyPositionParams(ctypes.Structure):
                 ("initial_com", c_float * 2), ("singular", c_void_p)]
 
 
+class PtyProjectionParams(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double),
+                ("exit_waves", c_void_p), ("exit_valid", c_int)]
+
+
 class PtyConvDesc(ctypes.Structure):
     _fields_ = [("n", c_int), ("Hi", c_int), ("Wi", c_int), ("Ci", c_int), ("Ho", c_int),
                 ("Wo", c_int), ("Co", c_int), ("stride", c_int), ("dil", c_int)]
@@ -116,6 +121,14 @@ _SIGNATURES = {
                                               c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
                                               c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "pty_positions_com": (c_int, [c_void_p, c_int, ctypes.POINTER(c_float), c_void_p]),
+    "pty_projection_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem),
+                                                  ctypes.POINTER(PtyGdParams)]),
+    "pty_projection_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams),
+                                   ctypes.POINTER(PtyProjectionParams), c_void_p, c_void_p,
+                                   c_void_p, c_size_t, c_void_p]),
+    "pty_projection_batch": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p,
+                                     c_void_p, c_int, c_int, ctypes.POINTER(PtyProjectionParams),
+                                     c_void_p, c_void_p, c_void_p]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -235,6 +235,12 @@ struct FusedArgs {
   const cf* chi_in;        // probe-gradient pass: [B][S][S] chi (natural)
   cf* pgrad_out;           // probe-gradient pass: [B][S][S] (kernel layout)
   float* sim_out;          // simulation pass: [B][S][S] |F|^2 (natural)
+  // projection sets: coefficients (float32 roundings of the host's float64
+  // a, b, c, x = 1 - a - b, y = 1 - c); the exit waves of the previous
+  // iteration are read from chi_in and the new ones written to chi_out
+  // (slot-indexed, may alias); exit_valid = 0 takes the overlap instead
+  float proj_a, proj_b, proj_c, proj_x, proj_y;
+  int exit_valid;
 };
 
 template <int S> struct Fused {
@@ -399,8 +405,10 @@ template <int S> struct Fused {
   }
 
   // P8: row pass B (completing fft2), amplitude projection, row pass B'
-  template <class L> static PTY_HD void p_project(int tid, const FusedArgs& a, int b, Regs& r,
-                                                  L lds, const cf* tw) {
+  // PROJ = false: g = A e^{i angle F} - F (gradient descent);
+  // PROJ = true:  g = A e^{i angle F}     (projection sets)
+  template <class L, bool PROJ = false>
+  static PTY_HD void p_project(int tid, const FusedArgs& a, int b, Regs& r, L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
@@ -418,10 +426,11 @@ template <int S> struct Fused {
       if (m2 > 0.f) {
         const float inv = frsq(m2);
         mag = m2 * inv;
-        g = mkcf(amp * (f.x * inv) - f.x, amp * (f.y * inv) - f.y);
+        g = PROJ ? mkcf(amp * (f.x * inv), amp * (f.y * inv))
+                 : mkcf(amp * (f.x * inv) - f.x, amp * (f.y * inv) - f.y);
       } else {
         mag = 0.f;
-        g = mkcf(amp - f.x, -f.y);
+        g = PROJ ? mkcf(amp, 0.f) : mkcf(amp - f.x, -f.y);
       }
       const float d = amp - mag;
       acc += (double)(d * d);
@@ -431,6 +440,53 @@ template <int S> struct Fused {
     P::template B<true>(t, r.v, tw);
 #pragma unroll
     for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
+  }
+
+  // projection sets (ptychographic_methods.py:1637-1734): shifted probe out,
+  // overlap psi = P * O_patch, previous exit wave e (psi on the first
+  // iteration), c psi + y e into the forward column pass, and the partial
+  // new exit wave w = x e + a psi (the reference's left-to-right float32
+  // evaluation order) written over e: the same thread reads it back in the
+  // last phase (keeping it in registers through the transforms would spill)
+  template <class L> static PTY_HD void p_proj_colA(int tid, const FusedArgs& a, int b, Regs& r,
+                                                    L lds, const cf* tw) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    p_colAinv_to_regs(tid, r, lds, tw);
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+    cf* ps_b = a.ps_out ? a.ps_out + (size_t)b * S * S : nullptr;
+    const cf* ex_b = a.chi_in + (size_t)b * S * S;
+    cf* w_b = a.chi_out + (size_t)b * S * S;
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int i = n2 + N2 * m;
+      if (ps_b) ps_b[i * S + cc] = r.v[m];
+      const cf psi = cmul(r.v[m], a.object[object_offset(a, r, i, cc)]);
+      const cf e = a.exit_valid ? ex_b[i * S + cc] : psi;
+      r.v[m] = cadd(cscale(psi, a.proj_c), cscale(e, a.proj_y));
+      w_b[i * S + cc] = cadd(cscale(e, a.proj_x), cscale(psi, a.proj_a));
+    }
+    P::template A<false>(n2, r.v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) lds.st(F::at(n2 + N2 * m, cc), r.v[m]);
+  }
+
+  // projection sets, last phase: new exit wave w + b * ifft2(A e^{i angle F})
+  // (w from the first phase, same thread and elements)
+  template <class L> static PTY_HD void p_proj_out(int tid, const FusedArgs& a, int b, Regs& r,
+                                                   L lds, const cf* tw) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    p_colAinv_to_regs(tid, r, lds, tw);
+    const int cc = F::col_line(tid), n2 = F::col_q(tid);
+    cf* ex_b = a.chi_out + (size_t)b * S * S;
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int o = (n2 + N2 * m) * S + cc;
+      ex_b[o] = cadd(ex_b[o], cscale(r.v[m], a.proj_b));
+    }
   }
 
   // probe gradient, first phase: g = conj(O_patch) * chi from global memory,
@@ -522,6 +578,37 @@ template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_outputs(tid, a, b, r, lds, tw); });
+  run.reduce_err(b);  // ends with a barrier
+}
+
+// Projection-set version of fused_position (same barriers): the exit waves
+// of chi_in are advanced in place into chi_out.
+template <int S, class Run> PTY_HD void projection_position(Run& run, const FusedArgs& a, int b) {
+  using K = Fused<S>;
+  using Regs = typename K::Regs;
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, false, true, true); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
+  run.wave_barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_proj_colA(tid, a, b, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
+  run.wave_barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::template p_project<decltype(lds), true>(tid, a, b, r, lds, tw);
+  });
+  run.wave_barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+  run.barrier();
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_proj_out(tid, a, b, r, lds, tw); });
   run.reduce_err(b);  // ends with a barrier
 }
 

@@ -119,6 +119,19 @@ __global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
   fused_body<S>(a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
 }
 
+// projection-set exit-wave update, one work group per position
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_projection(FusedArgs a) {
+  __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
+  __shared__ cf tw[S];
+  __shared__ double red[Frame<S>::T / 64];
+  const int tid = threadIdx.x;
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
+  typename Fused<S>::Regs r;
+  GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
+  projection_position<S>(run, a, a.b0 + (int)blockIdx.x);
+}
+
 // backward of the probe path, one work group per position
 template <int S>
 __global__ __launch_bounds__(Frame<S>::T) void k_probe_grad(FusedArgs a) {
@@ -436,25 +449,33 @@ __device__ __forceinline__ cf step_add(cf o, cf g, float nrm, float step) {
   return mkcf(o.x + ux, o.y + uy);
 }
 
+// o + step * g * nrm (gradient descent), or g * nrm when `replace` (the
+// projection-set adjoint, ptychographic_methods.py:1910-1998)
+__device__ __forceinline__ cf update_value(cf o, cf g, float nrm, float step, int replace) {
+#pragma clang fp contract(off)
+  return replace ? mkcf(g.x * nrm, g.y * nrm) : step_add(o, g, nrm, step);
+}
+
 __device__ __forceinline__ void object_update_body(cf* __restrict__ obj, const cf* __restrict__ grad,
                                                    const float* __restrict__ spsum,
                                                    const float* __restrict__ tilemax, int ntiles,
                                                    long long npix, float step, float c1, float c2, int blk,
-                                                   int nblk) {
+                                                   int nblk, int replace = 0) {
   __shared__ float sh[256];
   float m = 0.f;
   for (int i = threadIdx.x; i < ntiles; i += blockDim.x) m = fmaxf(m, tilemax[i]);
   m = block_max(m, sh);
   for (long long p = (long long)blk * blockDim.x + threadIdx.x; p < npix; p += (long long)nblk * blockDim.x) {
-    obj[p] = step_add(obj[p], grad[p], overlap_norm(spsum[p], m, c1, c2), step);
+    obj[p] = update_value(obj[p], grad[p], overlap_norm(spsum[p], m, c1, c2), step, replace);
   }
 }
 
 __global__ __launch_bounds__(256) void k_object_update(cf* __restrict__ obj, const cf* __restrict__ grad,
                                                        const float* __restrict__ spsum,
                                                        const float* __restrict__ tilemax, int ntiles,
-                                                       long long npix, float step, float c1, float c2) {
-  object_update_body(obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, blockIdx.x, gridDim.x);
+                                                       long long npix, float step, float c1, float c2,
+                                                       int replace) {
+  object_update_body(obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, blockIdx.x, gridDim.x, replace);
 }
 
 // ---------------------------------------------------------------------------
@@ -535,20 +556,21 @@ __global__ __launch_bounds__(256) void k_probe_reduce(const double* __restrict__
 
 __device__ __forceinline__ void probe_apply_body(cf* __restrict__ probe, const cf* __restrict__ gsum,
                                                  const float* __restrict__ osum, const float* __restrict__ bmax,
-                                                 int nblk, int npix, float step, float c1, float c2, int blk) {
+                                                 int nblk, int npix, float step, float c1, float c2, int blk,
+                                                 int replace = 0) {
   __shared__ float sh[256];
   float m = 0.f;
   for (int i = threadIdx.x; i < nblk; i += blockDim.x) m = fmaxf(m, bmax[i]);
   m = block_max(m, sh);
   const int pix = blk * blockDim.x + threadIdx.x;
-  if (pix < npix) probe[pix] = step_add(probe[pix], gsum[pix], overlap_norm(osum[pix], m, c1, c2), step);
+  if (pix < npix) probe[pix] = update_value(probe[pix], gsum[pix], overlap_norm(osum[pix], m, c1, c2), step, replace);
 }
 
 __global__ __launch_bounds__(256) void k_probe_apply(cf* __restrict__ probe, const cf* __restrict__ gsum,
                                                      const float* __restrict__ osum,
                                                      const float* __restrict__ bmax, int nblk, int npix,
-                                                     float step, float c1, float c2) {
-  probe_apply_body(probe, gsum, osum, bmax, nblk, npix, step, c1, c2, blockIdx.x);
+                                                     float step, float c1, float c2, int replace) {
+  probe_apply_body(probe, gsum, osum, bmax, nblk, npix, step, c1, c2, blockIdx.x, replace);
 }
 
 // ---------------------------------------------------------------------------
@@ -749,6 +771,14 @@ int launch_fused(const FusedArgs& a, int S, hipStream_t stream) {
   return check_launch("k_fused");
 }
 
+int launch_projection(const FusedArgs& a, int S, hipStream_t stream) {
+  const int grid = a.B - a.b0;
+  if (grid <= 0) return PTY_OK;
+  if (S == 64) hipLaunchKernelGGL(k_projection<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+  else hipLaunchKernelGGL(k_projection<128>, dim3(grid), dim3(Frame<128>::T), 0, stream, a);
+  return check_launch("k_projection");
+}
+
 int launch_fft(const cf* in, cf* out, int batch, int S, bool inv, int natural, hipStream_t stream) {
   if (batch <= 0) return PTY_OK;
   if (S == 64) {
@@ -774,11 +804,11 @@ int grid_for(long long n, int threads, int cap = 4096) {
 }
 
 int launch_object_update(cf* obj, const cf* grad, const float* spsum, const float* tilemax, int ntiles,
-                         int Px, int Py, float step, float nmin, hipStream_t stream) {
+                         int Px, int Py, float step, float nmin, hipStream_t stream, int replace = 0) {
   const long long npix = (long long)Px * Py;
   const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
   hipLaunchKernelGGL(k_object_update, dim3(grid_for(npix, 256)), dim3(256), 0, stream, obj, grad, spsum,
-                     tilemax, ntiles, npix, step, c1, c2);
+                     tilemax, ntiles, npix, step, c1, c2, replace);
   return check_launch("k_object_update");
 }
 
@@ -791,7 +821,8 @@ size_t probe_ws_bytes(int S, int B) {
          align256((S2 / 256 + 1) * 4);
 }
 
-int launch_probe_apply(cf* probe, double* ws, int S, int B, float step, float nmin, hipStream_t stream) {
+int launch_probe_apply(cf* probe, double* ws, int S, int B, float step, float nmin, hipStream_t stream,
+                       int replace = 0) {
   const size_t S2 = (size_t)S * S;
   char* base = (char*)ws;
   double* part = ws;
@@ -805,7 +836,7 @@ int launch_probe_apply(cf* probe, double* ws, int S, int B, float step, float nm
   if (rc) return rc;
   const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
   hipLaunchKernelGGL(k_probe_apply, dim3(nblk), dim3(256), 0, stream, probe, gsum, osum, bmax, nblk, npix, step,
-                     c1, c2);
+                     c1, c2, replace);
   return check_launch("k_probe_apply");
 }
 
@@ -899,7 +930,9 @@ int stage_size(const pty_gd_params& q) {
 // correction reads the batch's shifted probes and far fields after the update.
 int stage_for(const pty_gd_params& q, bool positions) { return positions ? q.max_batch : stage_size(q); }
 
-WsLayout layout(const pty_problem& p, const pty_gd_params& q, bool positions = false) {
+// projection sets stage no exit-wave differences (chi): the exit waves
+// themselves are the caller's persistent buffer
+WsLayout layout(const pty_problem& p, const pty_gd_params& q, bool positions = false, bool no_chi = false) {
   WsLayout L{};
   const size_t S2 = (size_t)p.S * p.S;
   const size_t B = (size_t)stage_for(q, positions);
@@ -907,7 +940,7 @@ WsLayout layout(const pty_problem& p, const pty_gd_params& q, bool positions = f
   size_t o = 0;
   L.phat = o; o = align256(o + S2 * 8);
   L.ps = o; o = align256(o + B * S2 * 8);
-  L.chi = o; o = align256(o + B * S2 * 8);
+  L.chi = o; o = align256(o + (no_chi ? 0 : B * S2 * 8));
   L.err = o; o = align256(o + (size_t)p.N * 8);
   L.grad = o; o = align256(o + npix * 8);
   L.spsum = o; o = align256(o + npix * 4);
@@ -1033,6 +1066,98 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
     }
     if (correct && pp->center_of_mass) {
       if ((rc = launch_positions_com(pp->positions, p.N, pp->initial_com, stream))) return rc;
+    }
+  }
+  return PTY_OK;
+}
+
+// num_iter projection-set iterations (singleslice_ptychography.py:847-983
+// with use_projection_scheme): one batch of all N positions in `order`, the
+// exit waves advanced in place, object and probe replaced by their normalised
+// adjoints, then the same constraints as gradient descent
+int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_projection_params* jp,
+             const int* order, double* errors, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  int rc = validate_problem(problem);
+  if (rc) return rc;
+  if (!params || !jp) return fail(PTY_E_ARG, "params are NULL");
+  const pty_problem& p = *problem;
+  const pty_gd_params& q = *params;
+  if (q.num_iter < 0 || q.max_batch != p.N)
+    return fail(PTY_E_ARG, "projection sets update from all %d positions at once (max_batch=%d)", p.N, q.max_batch);
+  if (!order || !errors || !workspace || !jp->exit_waves) return fail(PTY_E_ARG, "null order/errors/workspace/exit waves");
+  if (!aligned8(jp->exit_waves)) return fail(PTY_E_ARG, "exit waves must be 8-byte aligned");
+  const WsLayout L = layout(p, q, false, true);
+  if (workspace_bytes < L.total)
+    return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.total);
+  char* ws = (char*)workspace;
+  cf* phat = (cf*)(ws + L.phat);
+  cf* ps = (cf*)(ws + L.ps);
+  double* err = (double*)(ws + L.err);
+  cf* grad = (cf*)(ws + L.grad);
+  float* spsum = (float*)(ws + L.spsum);
+  float* tilemax = (float*)(ws + L.tilemax);
+  double* ppart = (double*)(ws + L.ppart);
+  double* cpart = (double*)(ws + L.cpart);
+  double* runbuf = (double*)(ws + L.run);
+  cf* exits = (cf*)jp->exit_waves;
+  const int stage = stage_size(q);
+  cf* obj = (cf*)p.object;
+  cf* probe = (cf*)p.probe;
+  const int ntiles = num_tiles(p.Px, p.Py);
+  const double norm = (double)(float)(p.mean_intensity * (float)p.N);
+  const size_t S2 = (size_t)p.S * p.S;
+  // the reference's Python-float coefficients, rounded to float32 where they
+  // multiply complex64 arrays
+  const double x = 1.0 - jp->a - jp->b, y = 1.0 - jp->c;
+  int valid = jp->exit_valid != 0;
+  bool phat_valid = false;
+  for (int it = 0; it < q.num_iter; ++it) {
+    if (!phat_valid) {
+      if ((rc = launch_fft(probe, phat, 1, p.S, false, 0, stream))) return rc;
+      phat_valid = q.fix_probe != 0;
+    }
+    for (int s0 = 0; s0 < p.N; s0 += stage) {
+      const int ns = (p.N - s0) < stage ? (p.N - s0) : stage;
+      cf* ex = exits + (size_t)s0 * S2;
+      FusedArgs a{};
+      a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
+      a.order = order + s0; a.N = p.N; a.B = ns; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
+      a.ps_out = ps; a.chi_in = ex; a.chi_out = ex; a.err = err + s0; a.fwd_out = nullptr;
+      a.proj_a = (float)jp->a; a.proj_b = (float)jp->b; a.proj_c = (float)jp->c;
+      a.proj_x = (float)x; a.proj_y = (float)y; a.exit_valid = valid;
+      if ((rc = launch_projection(a, p.S, stream))) return rc;
+      if (!q.fix_probe) {
+        ProbePartialArgs pa{ex, obj, p.positions, order + s0, p.N, ns, p.S, p.Px, p.Py,
+                            ppart + (size_t)(s0 / PROBE_CHUNK) * S2 * 3};
+        hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(ns)), dim3(256), 0,
+                           stream, pa);
+        if ((rc = check_launch("k_probe_partial"))) return rc;
+      }
+      GatherArgs g{};
+      g.ps = ps; g.chi = ex; g.positions = p.positions; g.order = order + s0; g.N = p.N; g.B = ns; g.S = p.S;
+      g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
+      g.tilemax = tilemax;
+      if (ns < p.N) {
+        g.run = runbuf;
+        g.run_first = s0 == 0;
+        g.run_last = s0 + ns == p.N;
+      }
+      if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+    }
+    valid = 1;
+    if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, 0.f, q.normalization_min, stream,
+                                   1)))
+      return rc;
+    if (!q.fix_probe) {
+      if ((rc = launch_probe_apply(probe, ppart, p.S, p.N, 0.f, q.normalization_min, stream, 1))) return rc;
+    }
+    hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
+    if ((rc = check_launch("k_error_reduce"))) return rc;
+    if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
+                                 stream)))
+      return rc;
+    if (!q.fix_probe && q.fix_probe_com) {
+      if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
     }
   }
   return PTY_OK;
@@ -1299,6 +1424,37 @@ int pty_position_correction_batch(const pty_position_params* pp, const void* ps,
   PosArgs a = position_args(*pp, (const cf*)ps, (const cf*)far, (const cf*)object, Px, Py, amplitudes, N, order, B);
   a.sums = sums;
   return launch_position_correction(a, S, st(stream));
+}
+
+size_t pty_projection_workspace_bytes(const pty_problem* problem, const pty_gd_params* params) {
+  if (!problem || !params || params->max_batch <= 0) return 0;
+  return layout(*problem, *params, false, true).total;
+}
+
+int pty_projection_run(const pty_problem* problem, const pty_gd_params* params,
+                       const pty_projection_params* projection, const int* order, double* errors,
+                       void* workspace, size_t workspace_bytes, pty_stream_t stream) {
+  return proj_run(problem, params, projection, order, errors, workspace, workspace_bytes, st(stream));
+}
+
+int pty_projection_batch(const void* probe_hat, const void* object, int Px, int Py, const float* positions,
+                         int N, const float* amplitudes, const int* order, int B, int S,
+                         const pty_projection_params* projection, void* ps, double* err, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (Px < S || Py < S) return fail(PTY_E_ARG, "object %dx%d smaller than frame %d", Px, Py, S);
+  if (N <= 0 || B < 0) return fail(PTY_E_ARG, "bad N=%d or B=%d", N, B);
+  if (!probe_hat || !object || !positions || !amplitudes || !order || !projection || !projection->exit_waves ||
+      !err)
+    return fail(PTY_E_ARG, "null buffer in projection_batch");
+  const double x = 1.0 - projection->a - projection->b, y = 1.0 - projection->c;
+  FusedArgs a{};
+  a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions;
+  a.amps = amplitudes; a.order = order; a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0;
+  a.ps_out = (cf*)ps; a.chi_in = (const cf*)projection->exit_waves; a.chi_out = (cf*)projection->exit_waves;
+  a.err = err;
+  a.proj_a = (float)projection->a; a.proj_b = (float)projection->b; a.proj_c = (float)projection->c;
+  a.proj_x = (float)x; a.proj_y = (float)y; a.exit_valid = projection->exit_valid != 0;
+  return launch_projection(a, S, st(stream));
 }
 
 int pty_positions_com(float* positions, int N, const float* initial_com, pty_stream_t stream) {

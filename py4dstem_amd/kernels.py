@@ -172,6 +172,43 @@ def gd_run(problem, params, orders, errors, workspace):
               ptr(workspace), workspace.numel(), stream_handle())
 
 
+def projection_params(exit_waves, a, b, c, exit_valid=False):
+    """pty_projection_params for device exit waves [N, S, S] (complex64)."""
+    from ._lib import PtyProjectionParams
+    _req(exit_waves, torch.complex64, name="exit_waves")
+    return PtyProjectionParams(float(a), float(b), float(c), c_void_p(exit_waves.data_ptr()),
+                               int(bool(exit_valid)))
+
+
+def projection_workspace_bytes(problem, params):
+    import ctypes
+    return int(_lib.load().pty_projection_workspace_bytes(ctypes.byref(problem),
+                                                          ctypes.byref(params)))
+
+
+def projection_run(problem, params, proj_params, order, errors, workspace):
+    import ctypes
+    _lib.call("pty_projection_run", ctypes.byref(problem), ctypes.byref(params),
+              ctypes.byref(proj_params), ptr(order), ptr(errors), ptr(workspace),
+              workspace.numel(), stream_handle())
+
+
+def projection_batch(probe_hat, obj, positions, amplitudes_k, order, proj_params):
+    """Exit-wave step of the projection sets for the slots of ``order``
+    (exit waves of proj_params advanced in place); returns (P_shift, err)."""
+    import ctypes
+    S = probe_hat.shape[-1]
+    check_frame(S)
+    B = order.numel()
+    Px, Py = obj.shape
+    ps = torch.empty((B, S, S), dtype=torch.complex64, device=obj.device)
+    err = torch.empty(B, dtype=torch.float64, device=obj.device)
+    _lib.call("pty_projection_batch", ptr(probe_hat), ptr(obj), Px, Py, ptr(positions),
+              positions.shape[0], ptr(amplitudes_k), ptr(order), B, S, ctypes.byref(proj_params),
+              ptr(ps), ptr(err), stream_handle())
+    return ps, err
+
+
 def far_to_kernel_layout(far):
     """(B, S, S) far field from the staging layout of forward_adjoint to the
     kernel layout (element j of row-pass thread t -> row t // (S/N1),
@@ -236,7 +273,9 @@ __all__ = ["fft2", "to_kernel_layout", "probe_spectrum", "forward_adjoint", "sca
            "object_update", "probe_update", "object_constraints", "probe_com_constraint",
            "make_problem", "gd_workspace_bytes", "gd_run", "PtyProblem", "PtyGdParams",
            "kernel_index", "num_tiles", "position_params", "gd_positions_workspace_bytes",
-           "gd_run_positions", "position_correction", "positions_com"]
+           "gd_run_positions", "position_correction", "positions_com", "projection_params",
+           "projection_workspace_bytes", "projection_run", "projection_batch",
+           "far_to_kernel_layout"]
 
 
 def pattern_com(frames, mask=None):

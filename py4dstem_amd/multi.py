@@ -39,6 +39,10 @@ class MultiPtychography:
         for p, kw in zip(self.items, kwargs):
             kw = dict(kw, num_iter=num_iter, progress_bar=False, store_iterations=False)
             run = p._prepare_run(_allocate=False, **kw)
+            if run["projection_params"] is not None:
+                raise NotImplementedError("projection-set methods run through "
+                                          "SingleslicePtychography.reconstruct(), not the batched "
+                                          "runner")
             if run["position_params"] is not None:
                 raise NotImplementedError("position correction (fix_positions=False) runs through "
                                           "SingleslicePtychography.reconstruct(), not the batched "

@@ -40,9 +40,40 @@ from .probe import POLAR_ALIASES, POLAR_SYMBOLS, ComplexProbe, polar_parameters_
 warnings.simplefilter("always", UserWarning)
 
 _GD_NAMES = ("GD", "gradient-descent")
-_PROJECTION_NAMES = ("generalized-projections", "DM_AP", "difference-map_alternating-projections",
-                     "RAAR", "relaxed-averaged-alternating-reflections", "RRR",
-                     "relax-reflect-reflect", "SUPERFLIP", "charge-flipping")
+
+
+def _method_parameters(method, parameter, a, b, c):
+    """(projection_a, projection_b, projection_c) of a projection-set method,
+    or None for gradient descent (phase_base_class.py:2005-2108, same
+    ValueErrors)."""
+    if method == "generalized-projections":
+        if a is None or b is None or c is None:
+            raise ValueError("reconstruction_parameter_a/b/c must all be specified "
+                             "when using reconstruction_method='generalized-projections'.")
+        return (a, b, c)
+    if method in ("DM_AP", "difference-map_alternating-projections"):
+        if parameter < 0.0 or parameter > 1.0:
+            raise ValueError("reconstruction_parameter must be between 0-1.")
+        return (-parameter, 1, 1 + parameter)
+    if method in ("RAAR", "relaxed-averaged-alternating-reflections"):
+        if parameter < 0.0 or parameter > 1.0:
+            raise ValueError("reconstruction_parameter must be between 0-1.")
+        return (1 - 2 * parameter, parameter, 2)
+    if method in ("RRR", "relax-reflect-reflect"):
+        if parameter < 0.0 or parameter > 2.0:
+            raise ValueError("reconstruction_parameter must be between 0-2.")
+        return (-parameter, parameter, 2)
+    if method in ("SUPERFLIP", "charge-flipping"):
+        return (0, 1, 2)
+    if method in _GD_NAMES:
+        return None
+    raise ValueError(
+        "reconstruction_method must be one of 'generalized-projections', "
+        "'DM_AP' (or 'difference-map_alternating-projections'), "
+        "'RAAR' (or 'relaxed-averaged-alternating-reflections'), "
+        "'RRR' (or 'relax-reflect-reflect'), "
+        "'SUPERFLIP' (or 'charge-flipping'), "
+        f"or 'GD' (or 'gradient-descent'), not  {method}.")
 
 
 def _warn(msg):
@@ -558,27 +589,22 @@ class SingleslicePtychography:
             raise NotImplementedError("switching object_type is not supported")
 
         # method parameters (phase_base_class.py:2005-2108)
-        if reconstruction_method in _PROJECTION_NAMES:
-            if reconstruction_method == "generalized-projections" and (
-                    reconstruction_parameter_a is None or reconstruction_parameter_b is None
-                    or reconstruction_parameter_c is None):
-                raise ValueError("reconstruction_parameter_a/b/c must all be specified "
-                                 "when using reconstruction_method='generalized-projections'.")
+        projection = _method_parameters(reconstruction_method, reconstruction_parameter,
+                                        reconstruction_parameter_a, reconstruction_parameter_b,
+                                        reconstruction_parameter_c)
+        if projection is not None:
+            # the reference raises this from its verbose summary
+            # (phase_base_class.py:2130-2137); it is raised here whatever the
+            # verbosity, since mini-batches would mix exit waves of different
+            # positions
             if max_batch_size is not None:
                 raise ValueError(
                     "Stochastic object/probe updating is inconsistent with 'DM_AP', 'RAAR', "
                     "'RRR', and 'SUPERFLIP'. Use reconstruction_method='GD' or set "
                     "max_batch_size=None.")
-            raise NotImplementedError(f"reconstruction_method={reconstruction_method!r}: only "
-                                      "gradient descent is implemented on MI355X")
-        if reconstruction_method not in _GD_NAMES:
-            raise ValueError(
-                "reconstruction_method must be one of 'generalized-projections', "
-                "'DM_AP' (or 'difference-map_alternating-projections'), "
-                "'RAAR' (or 'relaxed-averaged-alternating-reflections'), "
-                "'RRR' (or 'relax-reflect-reflect'), "
-                "'SUPERFLIP' (or 'charge-flipping'), "
-                f"or 'GD' (or 'gradient-descent'), not  {reconstruction_method}.")
+            if not fix_positions:
+                raise NotImplementedError("position correction with projection-set methods is "
+                                          "not supported on this path")
         unsupported = {
             "detector_fourier_mask": detector_fourier_mask is not None,
             "gaussian_filter_sigma": gaussian_filter and gaussian_filter_sigma is not None,
@@ -598,9 +624,16 @@ class SingleslicePtychography:
         if self._verbose:
             first = (f"Performing {num_iter} iterations using a {self._object_type} object type, "
                      f"with the {reconstruction_method} algorithm, with normalization_min: "
-                     f"{normalization_min} and step _size: {step_size}")
-            warnings.warn(first + (f", in batches of max {max_batch_size} measurements."
-                                   if max_batch_size is not None else "."), UserWarning)
+                     f"{normalization_min}")
+            if projection is None:
+                warnings.warn(first + f" and step _size: {step_size}" + (
+                    f", in batches of max {max_batch_size} measurements."
+                    if max_batch_size is not None else "."), UserWarning)
+            elif reconstruction_method in ("generalized-projections", "SUPERFLIP",
+                                           "charge-flipping"):
+                warnings.warn(first + f" and (a,b,c): {projection}.", UserWarning)
+            else:
+                warnings.warn(first + f" and α: {reconstruction_parameter}.", UserWarning)
 
         # batch order: identical RNG use to the reference loop (:835-857).
         # The affine position constraint draws from the same generator after
@@ -613,8 +646,13 @@ class SingleslicePtychography:
         else:
             B = N
         B = max(1, min(B, N))
-        orders = np.empty((max(num_iter, 1), N), dtype=np.int32)
-        for it in range(0 if affine else num_iter):
+        if projection is not None:
+            # projection sets: one batch of all positions in scan order, no
+            # shuffle (singleslice_ptychography.py:853-855)
+            orders = np.arange(N, dtype=np.int32)[None, :]
+        else:
+            orders = np.empty((max(num_iter, 1), N), dtype=np.int32)
+        for it in range(0 if (affine or projection is not None) else num_iter):
             np.random.shuffle(shuffled)
             orders[it] = shuffled
 
@@ -635,8 +673,17 @@ class SingleslicePtychography:
                              int(bool(pure_phase_object)), float(step_size),
                              float(normalization_min), float(shrinkage_rad),
                              int(self._stage_for(B)))
-        pos_params = singular = None
-        if not fix_positions:
+        pos_params = singular = proj_params = None
+        if projection is not None:
+            if getattr(self, "_exit_waves", None) is None:
+                # the reference starts from exit waves = overlap (None here)
+                self._exit_waves = torch.empty((N,) + tuple(self._amplitudes_shape),
+                                               dtype=torch.complex64, device=dev)
+                self._exit_valid = False
+            proj_params = K.projection_params(self._exit_waves, *projection,
+                                              exit_valid=self._exit_valid)
+            nbytes = K.projection_workspace_bytes(problem, params)
+        elif not fix_positions:
             # position correction (singleslice_ptychography.py:913-926) and the
             # centre-of-mass constraint, which the reference applies when its
             # own flag arrives False (fix_positions_com and not fix_positions
@@ -662,7 +709,8 @@ class SingleslicePtychography:
                               errors=errors_dev, mask=mask_dev, num_iter=num_iter,
                               per_call=per_call, store_iterations=store_iterations,
                               progress_bar=progress_bar, position_params=pos_params,
-                              singular=singular, affine=affine, shuffled=shuffled)
+                              singular=singular, affine=affine, shuffled=shuffled,
+                              projection_params=proj_params)
         return self._last_run
 
     def _execute_run(self, run):
@@ -684,7 +732,12 @@ class SingleslicePtychography:
             if run["affine"]:
                 np.random.shuffle(run["shuffled"])
                 orders_dev[it].copy_(torch.from_numpy(run["shuffled"].astype(np.int32)))
-            if pos_params is None:
+            if run["projection_params"] is not None:
+                K.projection_run(problem, params, run["projection_params"], orders_dev[0],
+                                 errors_dev[it:], ws)
+                run["projection_params"].exit_valid = 1
+                self._exit_valid = True
+            elif pos_params is None:
                 K.gd_run(problem, params, orders_dev[it:], errors_dev[it:], ws)
             else:
                 K.gd_run_positions(problem, params, pos_params, orders_dev[it:], errors_dev[it:],
@@ -733,5 +786,6 @@ class SingleslicePtychography:
             self.error = float(errors[-1])
         elif not hasattr(self, "error"):
             self.error = 0.0
-        self._exit_waves = None
+        if run.get("projection_params") is None:
+            self._exit_waves = None  # singleslice_ptychography.py:990-992
         return self

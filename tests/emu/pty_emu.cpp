@@ -312,6 +312,23 @@ int emu_simulate(int S, const float* probe_hat, const float* object, int Px, int
   return 0;
 }
 
+// projection-set exit-wave step of one batch: same arguments as
+// pty_projection_batch (exit waves advanced in place)
+int emu_projection(int S, const float* probe_hat, const float* object, int Px, int Py, const float* positions,
+                   int N, const float* amps, const int* order, int B, float* exits, int exit_valid, double pa,
+                   double pb, double pc, float* ps, double* err) {
+  FusedArgs a{};
+  a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions; a.amps = amps;
+  a.order = order; a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0;
+  a.ps_out = (cf*)ps; a.chi_in = (const cf*)exits; a.chi_out = (cf*)exits; a.err = err;
+  a.proj_a = (float)pa; a.proj_b = (float)pb; a.proj_c = (float)pc;
+  a.proj_x = (float)(1.0 - pa - pb); a.proj_y = (float)(1.0 - pc); a.exit_valid = exit_valid;
+  if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) projection_position<64>(run, a, b); }
+  else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < B; ++b) projection_position<128>(run, a, b); }
+  else return -1;
+  return 0;
+}
+
 // position correction of one batch: same arguments as
 // pty_position_correction_batch (positions updated in place)
 int emu_position_correction(int S, const float* ps, const float* far, const float* object, int Px, int Py,

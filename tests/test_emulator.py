@@ -334,3 +334,37 @@ def test_position_correction_clips_match_oracle(emu, small_case):
                                 max_position_total_distance=0.5 * norm)
     np.testing.assert_array_equal(pos[order[::2]], d["pos"][order[::2]])
     np.testing.assert_allclose(pos[order], ref, rtol=0, atol=1e-4)
+
+
+@pytest.mark.parametrize("abc,valid", [((-1.0, 1, 2.0), 1), ((1 - 2 * 0.6, 0.6, 2), 1),
+                                       ((0, 1, 2), 0)])
+def test_projection_phases_match_oracle(emu, small_case, abc, valid):
+    """Projection-set exit-wave step (DM, RAAR, SUPERFLIP's first iteration)
+    against the oracle's _projection_sets_fourier_projection, race-free."""
+    _, o = small_case
+    S = 64
+    rng = np.random.default_rng(21)
+    obj = (o._object * np.exp(1j * 0.2 * rng.standard_normal(o._object.shape))).astype(np.complex64)
+    probe = o._probe.astype(np.complex64)
+    B = 10
+    order = np.arange(B, dtype=np.int32) * 7
+    pos = o._positions_px
+    k = kidx(emu, S)
+    phat = np.ascontiguousarray(np.fft.fft2(probe)[np.ix_(k, k)].astype(np.complex64))
+    amps = np.ascontiguousarray(o._amplitudes[:, k][:, :, k])
+    sp, op, _, _ = o.forward(obj, probe, pos[order], o._amplitudes[order])
+    e0 = (sp * op + 0.1 * np.abs(sp * op).mean()
+          * (rng.standard_normal(sp.shape) + 1j * rng.standard_normal(sp.shape))).astype(np.complex64)
+    exits = e0.copy()
+    ps = np.zeros_like(exits)
+    err = np.zeros(B)
+    emu.emu_reset()
+    emu.emu_projection(S, cptr(phat), cptr(obj), obj.shape[0], obj.shape[1], cptr(pos), len(pos),
+                       cptr(amps), cptr(order), B, cptr(exits), valid, ctypes.c_double(abc[0]),
+                       ctypes.c_double(abc[1]), ctypes.c_double(abc[2]), cptr(ps), cptr(err))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    rsp, _, ref, e = o.projection_forward(obj, probe, pos[order], o._amplitudes[order],
+                                          e0.copy() if valid else None, *abc)
+    assert rel(ps, rsp) < 1e-6
+    assert rel(exits, ref) < 2e-5
+    assert abs(err.sum() / e - 1) < 1e-5

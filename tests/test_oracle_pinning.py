@@ -500,3 +500,97 @@ def test_affine_ransac_ignores_outliers():
     # the plain least-squares fit is pulled off by the outliers
     est_ls, _ = po.affine_fit(p0, p1, origin)
     assert abs(est_ls[0] - tf[0]) > 1e-3 or abs(est_ls[4] - tf[4]) > 1.0
+
+
+# ---------------------------------------------------------------------------
+# projection sets
+# ---------------------------------------------------------------------------
+def test_projection_method_parameters_table():
+    mp = po.OracleSingleslicePtychography.method_parameters
+    assert mp("DM_AP", 0.3) == (True, -0.3, 1, 1.3)
+    assert mp("difference-map_alternating-projections", 0.0) == (True, -0.0, 1, 1.0)
+    assert mp("RAAR", 0.4) == (True, 1 - 0.8, 0.4, 2)
+    assert mp("RRR", 1.5) == (True, -1.5, 1.5, 2)
+    assert mp("SUPERFLIP") == (True, 0, 1, 2)
+    assert mp("generalized-projections", None, 0.1, 0.2, 0.3) == (True, 0.1, 0.2, 0.3)
+    assert mp("GD") == (False, None, None, None)
+    for bad in (("DM_AP", 1.2), ("RAAR", -0.1), ("RRR", 2.5)):
+        with pytest.raises(ValueError):
+            mp(*bad)
+    with pytest.raises(ValueError):
+        mp("generalized-projections", None, 0.1, None, 0.3)
+
+
+def _projector(psi, A):
+    """Modulus projector with plain DFT matrices (float64): ifft2(A F/|F|)."""
+    F = _dft2_loops(psi)
+    S = psi.shape[-1]
+    k = np.arange(S)
+    Winv = np.exp(2j * np.pi * np.outer(k, k) / S) / S
+    G = A * np.exp(1j * np.angle(F))
+    return Winv @ G @ Winv.T
+
+
+def _proj_case(tiny):
+    o, probe, obj, pos = tiny
+    rng = np.random.default_rng(12)
+    rows, cols = po.patch_indices(pos, (16, 16), obj.shape)
+    psi = po.fft_shift(probe, pos - np.round(pos)) * obj[rows, cols]
+    A = (np.abs(np.fft.fft2(psi)) * (1 + 0.2 * rng.random(psi.shape))).astype(np.float32)
+    e = (psi + 0.3 * (rng.standard_normal(psi.shape) + 1j * rng.standard_normal(psi.shape))
+         * np.abs(psi).mean()).astype(np.complex64)
+    return o, probe, obj, pos, psi, A, e
+
+
+def test_projection_ap_meets_the_modulus_constraint(tiny):
+    o, probe, obj, pos, psi, A, e = _proj_case(tiny)
+    _, _, ew, err = o.projection_forward(obj, probe, pos, A, e, 0, 1, 1)  # DM_AP(0) = AP
+    np.testing.assert_allclose(np.abs(np.fft.fft2(ew.astype(np.complex128))), A,
+                               rtol=2e-5, atol=2e-5 * A.max())
+    for b in range(len(pos)):
+        np.testing.assert_allclose(ew[b], _projector(psi[b], A[b]), rtol=0,
+                                   atol=1e-5 * np.abs(psi).max())
+    assert err == pytest.approx(np.sum((A - np.abs(np.fft.fft2(psi.astype(np.complex128)))) ** 2),
+                                rel=1e-4)
+
+
+def test_projection_dm_is_the_difference_map(tiny):
+    o, probe, obj, pos, psi, A, e = _proj_case(tiny)
+    _, _, ew, _ = o.projection_forward(obj, probe, pos, A, e, -1.0, 1, 2.0)  # DM_AP(1)
+    for b in range(len(pos)):
+        ref = e[b] - psi[b] + _projector(2 * psi[b] - e[b], A[b])
+        np.testing.assert_allclose(ew[b], ref, rtol=0, atol=2e-5 * np.abs(psi).max())
+    # first iteration (no exit waves yet): e = psi
+    _, _, ew0, _ = o.projection_forward(obj, probe, pos, A, None, -1.0, 1, 2.0)
+    for b in range(len(pos)):
+        np.testing.assert_allclose(ew0[b], _projector(psi[b], A[b]), rtol=0,
+                                   atol=2e-5 * np.abs(psi).max())
+
+
+@pytest.mark.parametrize("abc", [(1 - 2 * 0.7, 0.7, 2), (-1.2, 1.2, 2), (0.3, 0.6, 1.7), (0, 1, 2)])
+def test_projection_consistent_exit_wave_is_fixed_point(tiny, abc):
+    o, probe, obj, pos, psi, _, _ = _proj_case(tiny)
+    A = np.abs(np.fft.fft2(psi.astype(np.complex128))).astype(np.float32)
+    _, _, ew, err = o.projection_forward(obj, probe, pos, A, psi.copy(), *abc)
+    np.testing.assert_allclose(ew, psi, rtol=0, atol=1e-5 * np.abs(psi).max())
+    assert err < 1e-8 * np.sum(A.astype(np.float64) ** 2)
+
+
+def test_projection_adjoint_reproduces_consistent_truth(tiny):
+    o, probe, obj, pos = tiny
+    rows, cols = po.patch_indices(pos, (16, 16), obj.shape)
+    sp = po.fft_shift(probe, pos - np.round(pos))
+    op = obj[rows, cols]
+    new_obj, new_probe = o.projection_adjoint(obj.copy(), probe.copy(), op, sp, pos, sp * op, 0.0,
+                                              False)
+    cover = po.scatter_bincount(np.abs(sp) ** 2, pos, (16, 16), obj.shape) > 1e-3
+    np.testing.assert_allclose(new_obj[cover], obj[cover], rtol=2e-5, atol=2e-5)
+    # the probe term sums conj(O_b) O_b P_shift_b over positions; with all
+    # fractions zero it is the probe itself
+    pos0 = np.round(pos)
+    sp0 = po.fft_shift(probe, pos0 - np.round(pos0))
+    rows, cols = po.patch_indices(pos0, (16, 16), obj.shape)
+    op0 = obj[rows, cols]
+    _, new_probe = o.projection_adjoint(obj.copy(), probe.copy(), op0, sp0, pos0, sp0 * op0, 0.0,
+                                        False)
+    np.testing.assert_allclose(new_probe, probe, rtol=0, atol=2e-5 * np.abs(probe).max())
