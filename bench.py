@@ -16,10 +16,13 @@ after another with reconstruct(); prints aggregate position-updates/s.
 
 ``--workload positions`` times the production workload with position
 correction on (reconstruct(fix_positions=False)); ``--workload projection``
-with the DM_AP(0.5) projection-set method (all positions per update).
+with the DM_AP(0.5) projection-set method (all positions per update);
+``--workload mixed`` as a mixed-state reconstruction with ``--probe-modes``
+probe modes.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds T]
-                       [--workload production|many|kernels|learned-init|positions|projection]
+                       [--workload production|many|kernels|learned-init|positions|projection|
+                                   mixed] [--probe-modes K]
                        [--problems P]
 """
 
@@ -485,6 +488,54 @@ def bench_projection(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_mixed(args):
+    """Production workload as a mixed-state reconstruction with
+    ``--probe-modes`` incoherent probe modes (gradient descent, mini-batch
+    1024, probe update and mode orthogonalisation on)."""
+    import torch
+
+    from py4dstem_amd import MixedstatePtychography
+    from py4dstem_amd.synthetic import production_workload
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    torch.cuda.set_device(0)
+    w = production_workload(seed=0, scan=(args.scan, args.scan))
+    np.random.seed(0)
+    p = MixedstatePtychography(datacube=w["datacube"], num_probes=args.probe_modes, **w["init"])
+    p.preprocess(**w["preprocess"])
+    S = int(p._region_of_interest_shape[0])
+    Px, Py = p._object_shape
+    N = p._num_diffraction_patterns
+    B = w["reconstruct"]["max_batch_size"]
+    recon = dict(w["reconstruct"], progress_bar=False)
+    if args.warmup:
+        p.reconstruct(**dict(recon, num_iter=args.warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.reconstruct(**dict(recon, num_iter=args.steps, reset=False))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pos_per_s = N * args.steps / dt
+    log(f"{args.steps} mixed-state iterations (K={args.probe_modes}) in {dt:.3f} s -> "
+        f"{pos_per_s:,.0f} pos-updates/s; errors {p.error_iterations[-3:]}")
+    out = {
+        "metric": "position-updates/s", "value": round(pos_per_s, 1), "unit": "position-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (complex64)",
+        "data": "synthetic (seeded Gaussian-atom object, simulated Poisson patterns)",
+        "config": {
+            "workload": f"mixed-state GD ptychography, {args.probe_modes} probe modes, "
+                        f"{args.scan}x{args.scan} scan, {S}x{S} px patterns, {Px}x{Py} object, "
+                        f"mini-batch {B}, probe update, mode orthogonalisation",
+            "scan_positions": N, "frame_px": S, "object_px": [Px, Py], "mini_batch": B,
+            "probe_modes": args.probe_modes,
+        },
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -494,7 +545,9 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--scan", type=int, default=128)
     ap.add_argument("--workload", choices=("production", "many", "kernels", "learned-init",
-                                           "positions", "projection"), default="production")
+                                           "positions", "projection", "mixed"),
+                    default="production")
+    ap.add_argument("--probe-modes", type=int, default=4)
     ap.add_argument("--problems", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -511,6 +564,8 @@ def main():
         return bench_positions(args)
     if args.workload == "projection":
         return bench_projection(args)
+    if args.workload == "mixed":
+        return bench_mixed(args)
 
     import torch
 

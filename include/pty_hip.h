@@ -276,6 +276,33 @@ int pty_projection_batch(const void* probe_hat, const void* object, int Px, int 
                          int B, int S, const pty_projection_params* projection, void* ps,
                          double* err, pty_stream_t stream);
 
+/* ---- mixed state (several incoherent probe modes) ---------------------------
+ * MixedstatePtychography (phase/mixedstate_ptychography.py:45-1027) on the GD
+ * path: probe [K][S][S]; per position and mode P_k = shifted probe,
+ * F_k = fft2(P_k O_patch); far = sqrt(sum_k |F_k|^2); chi_k = ifft2(A/far F_k - F_k)
+ * (ptychographic_methods.py:2696-2805); object += (sum_k step scatter(conj(P_k)
+ * chi_k)) norm(sum_k scatter(|P_k|^2)), probe_k += step sum_b conj(O_b) chi_bk
+ * norm(sum_b |O_b|^2) (:2906-3000); per iteration the object constraints and
+ * the recentring of every mode by mode 0's centre of mass
+ * (ptychographic_constraints.py:1156-1178).  The mode orthogonalisation
+ * (:1180-1221) is left to the host between calls. */
+size_t pty_mixed_workspace_bytes(const pty_problem* problem, const pty_gd_params* params, int num_probes);
+int pty_mixed_run(const pty_problem* problem, const pty_gd_params* params, int num_probes, const int* orders,
+                  double* errors, void* workspace, size_t workspace_bytes, pty_stream_t stream);
+
+/* One batch of the mixed-state forward/adjoint terms: probe_hat [K][S][S]
+ * (kernel layout); ps, chi [K][B][S][S] natural (mode-major); far [B][K][S*S]
+ * far fields in the staging layout of pty_forward_adjoint_batch; err[b]. */
+int pty_mixed_forward_adjoint_batch(const void* probe_hat, int num_probes, const void* object, int Px,
+                                    int Py, const float* positions, int N, const float* amplitudes,
+                                    const int* order, int B, int S, void* ps, void* chi, double* err,
+                                    void* far, pty_stream_t stream);
+
+/* Every mode shifted by the centre of mass of mode 0's intensity
+ * (ptychographic_constraints.py:1156-1178); `shift` is 2 floats of device
+ * scratch. */
+int pty_probe_com_modes(void* probe, int num_probes, int S, float* shift, pty_stream_t stream);
+
 /* ---- many reconstructions together --------------------------------------- */
 
 /* One entry per independent reconstruction; params.num_iter is ignored. */

@@ -16,6 +16,7 @@ fixes it) is listed in ``ptycho_oracle.PINNING``.
 from .ptycho_oracle import (  # noqa: F401
     PINNING,
     ComplexProbeOracle,
+    OracleMixedstatePtychography,
     OracleSingleslicePtychography,
     electron_wavelength_angstrom,
     fft_shift,

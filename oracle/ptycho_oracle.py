@@ -50,6 +50,12 @@ PINNING = {
     "projection_forward": "AP output meets the modulus constraint; DM = e + P_M(2 psi - e) - psi; "
                           "a consistent exit wave is a fixed point for every (a, b, c)",
     "projection_adjoint": "noise-free truth with normalization_min=0 is reproduced",
+    "mixed_probe_modes": "equal moduli; mode-to-mode phase is a linear ramp",
+    "probe_orthogonalization": "orthonormal-up-to-scale output, sorted, intensity and span "
+                               "preserved",
+    "OracleMixedstatePtychography.forward/adjoint": "K=1 equals the single-mode path; data "
+                                                    "from the mixed model is a fixed point; FP64 "
+                                                    "finite differences of the mixed loss",
     "reconstruct": "composition of the above; c64 vs c128 sensitivity measured",
 }
 
@@ -941,6 +947,10 @@ class OracleSingleslicePtychography:
         x0, y0 = get_CoM(intensity, corner_centered=True)
         return fft_shift(probe, -np.array([x0, y0]), self._mode)
 
+    def probe_extra_constraints(self, probe):
+        """Probe constraints of subclasses (none for a single probe)."""
+        return probe
+
     @staticmethod
     def positions_constraints(current_positions, initial_positions, fix_positions,
                               fix_positions_com, global_affine_transformation):
@@ -1044,6 +1054,7 @@ class OracleSingleslicePtychography:
                                                    shrinkage_rad, pure_phase_object)
             if fix_probe_com and not fix_probe:
                 self._probe = self.probe_com_constraint(self._probe)
+            self._probe = self.probe_extra_constraints(self._probe)
             self._positions_px = self.positions_constraints(
                 self._positions_px, self._positions_px_initial, fix_positions,
                 fix_positions_com and not fix_positions, global_affine_transformation)
@@ -1057,3 +1068,142 @@ class OracleSingleslicePtychography:
         if not use_projection:  # singleslice_ptychography.py:990-992
             self._exit_waves = None
         return self
+
+
+# ---------------------------------------------------------------------------
+# Mixed-state ptychography (several incoherent probe modes)
+# ---------------------------------------------------------------------------
+def probe_orthogonalization(probe):
+    """ptychographic_constraints.py:1180-1221: modes rotated onto the
+    eigenvectors of their Gram matrix (upper triangle, complex64), then sorted
+    by real-space intensity, largest first."""
+    n_probes = probe.shape[0]
+    pairwise_dot_product = np.empty((n_probes, n_probes), dtype=probe.dtype)
+    for i in range(n_probes):
+        for j in range(i, n_probes):
+            pairwise_dot_product[i, j] = np.sum(probe[i].conj() * probe[j])
+    _, evecs = np.linalg.eigh(pairwise_dot_product, UPLO="U")
+    probe = np.tensordot(evecs.T, probe, axes=1)
+    intensities = np.sum(np.abs(probe) ** 2, axis=(-2, -1))
+    intensities_order = np.argsort(intensities, axis=None)[::-1]
+    return probe[intensities_order]
+
+
+def mixed_probe_modes(probe, num_probes):
+    """ptychographic_methods.py:1416-1462: mode 0 is the single-mode probe,
+    mode i the previous one times random sub-pixel phase ramps drawn from the
+    global NumPy generator."""
+    sx, sy = probe.shape
+    probes = np.zeros((num_probes, sx, sy), dtype=np.complex64)
+    probes[0] = probe
+    for i_probe in range(1, num_probes):
+        shift_x = np.exp(-2j * np.pi * (np.random.rand() - 0.5) * np.fft.fftfreq(sx))
+        shift_y = np.exp(-2j * np.pi * (np.random.rand() - 0.5) * np.fft.fftfreq(sy))
+        probes[i_probe] = probes[i_probe - 1] * shift_x[:, None] * shift_y[None]
+    return probes
+
+
+class OracleMixedstatePtychography(OracleSingleslicePtychography):
+    """``MixedstatePtychography`` (mixedstate_ptychography.py:45-1027) for the
+    gradient-descent path: probe [K][S][S], the mixed-state overlap, Fourier
+    projection and adjoint (ptychographic_methods.py:2696-2805, 2906-3000) and
+    the mixed probe constraints (ptychographic_constraints.py:1150-1286)."""
+
+    def __init__(self, energy, datacube=None, num_probes=None, initial_probe_guess=None,
+                 **kwargs):
+        # mixedstate_ptychography.py:150-168
+        if initial_probe_guess is None:
+            if num_probes is None:
+                raise ValueError("If initial_probe_guess is None, or a ComplexProbe object, "
+                                 "num_probes must be specified.")
+        else:
+            if len(initial_probe_guess.shape) != 3:
+                raise ValueError("Specified initial_probe_guess must have dimensions (N,Sx,Sy).")
+            num_probes = initial_probe_guess.shape[0]
+        self._num_probes = num_probes
+        super().__init__(energy, datacube=datacube, initial_probe_guess=initial_probe_guess,
+                         **kwargs)
+
+    def preprocess(self, **kwargs):
+        # the FOV mask comes from the first mode (mixedstate_ptychography.py:548-563)
+        given = None if self._probe is None else np.asarray(self._probe, dtype=np.complex64)
+        if given is not None:
+            self._probe = given[0]
+        super().preprocess(**kwargs)
+        self._probe = (given.copy() if given is not None
+                       else mixed_probe_modes(self._probe, self._num_probes))
+        self._probe_initial = self._probe.copy()
+        return self
+
+    def forward(self, obj, probe, positions_px, amplitudes):
+        """Mixed-state _forward for GD: shifted probes [B][K], far field
+        sqrt(sum_k |F_k|^2), chi_k = ifft2(A/far F_k - F_k)."""
+        mode = self._mode
+        frac = positions_px - np.round(positions_px)
+        rows, cols = patch_indices(positions_px, self._region_of_interest_shape, obj.shape)
+        shifted_probes = fft_shift(probe, frac, mode)
+        object_patches = obj[rows, cols]
+        overlap = shifted_probes * np.expand_dims(object_patches, axis=1)
+        fourier_overlap = _fft2(overlap, mode)
+        farfield_amplitudes = np.sqrt(np.sum(np.abs(fourier_overlap) ** 2, axis=1))
+        error = np.sum(np.abs(amplitudes - farfield_amplitudes) ** 2)
+        farfield_amplitudes[farfield_amplitudes == 0.0] = np.inf
+        amplitude_modification = amplitudes / farfield_amplitudes
+        fourier_modified_overlap = amplitude_modification[:, None] * fourier_overlap
+        fourier_modified_overlap = fourier_modified_overlap - fourier_overlap
+        exit_waves = _ifft2(fourier_modified_overlap, mode)
+        return shifted_probes, object_patches, exit_waves, error
+
+    def adjoint(self, obj, probe, object_patches, shifted_probes, positions_px, exit_waves,
+                step_size, normalization_min, fix_probe):
+        """Mixed-state _gradient_descent_adjoint for a complex object: per-mode
+        scatters added in float32 (complex64 accumulators, as the reference
+        allocates them with zeros_like(object))."""
+        roi = self._region_of_interest_shape
+        shape = obj.shape
+        probe_normalization = np.zeros_like(obj)
+        object_update = np.zeros_like(obj)
+        for i_probe in range(probe.shape[0]):
+            probe_normalization += scatter_bincount(np.abs(shifted_probes[:, i_probe]) ** 2,
+                                                    positions_px, roi, shape)
+            object_update += step_size * scatter_bincount(
+                np.conj(shifted_probes[:, i_probe]) * exit_waves[:, i_probe], positions_px, roi,
+                shape)
+        probe_normalization = 1 / np.sqrt(
+            1e-16
+            + ((1 - normalization_min) * probe_normalization) ** 2
+            + (normalization_min * np.max(probe_normalization)) ** 2
+        )
+        obj += object_update * probe_normalization
+        if not fix_probe:
+            object_normalization = np.sum(np.abs(object_patches) ** 2, axis=0)
+            object_normalization = 1 / np.sqrt(
+                1e-16
+                + ((1 - normalization_min) * object_normalization) ** 2
+                + (normalization_min * np.max(object_normalization)) ** 2
+            )
+            probe += step_size * (
+                np.sum(np.expand_dims(np.conj(object_patches), axis=1) * exit_waves, axis=0)
+                * object_normalization[None]
+            )
+        return obj, probe
+
+    def probe_com_constraint(self, probe):
+        """ptychographic_constraints.py:1156-1178: CoM of mode 0, every mode
+        shifted by it."""
+        intensity = np.abs(probe[0]) ** 2
+        x0, y0 = get_CoM(intensity, corner_centered=True)
+        return fft_shift(probe, -np.array([x0, y0]), self._mode)
+
+    def probe_extra_constraints(self, probe):
+        if self._orthogonalize_probe:
+            return probe_orthogonalization(probe)
+        return probe
+
+    def reconstruct(self, *args, orthogonalize_probe=True, **kwargs):
+        """mixedstate_ptychography.py:631-1027 (GD)."""
+        method = kwargs.get("reconstruction_method", "gradient-descent")
+        if method not in ("GD", "gradient-descent"):
+            raise NotImplementedError("the mixed-state oracle restates gradient descent only")
+        self._orthogonalize_probe = orthogonalize_probe
+        return super().reconstruct(*args, **kwargs)

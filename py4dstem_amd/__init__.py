@@ -16,4 +16,7 @@ def __getattr__(name):
     if name == "SingleslicePtychography":
         from .ptychography import SingleslicePtychography
         return SingleslicePtychography
+    if name == "MixedstatePtychography":
+        from .mixedstate import MixedstatePtychography
+        return MixedstatePtychography
     raise AttributeError(name)

@@ -121,6 +121,14 @@ _SIGNATURES = {
                                               c_void_p, c_void_p, c_int, c_int, c_void_p, c_int,
                                               c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "pty_positions_com": (c_int, [c_void_p, c_int, ctypes.POINTER(c_float), c_void_p]),
+    "pty_mixed_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams),
+                                             c_int]),
+    "pty_mixed_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams), c_int,
+                              c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_mixed_forward_adjoint_batch": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p,
+                                                c_int, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pty_probe_com_modes": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "pty_projection_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem),
                                                   ctypes.POINTER(PtyGdParams)]),
     "pty_projection_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams),

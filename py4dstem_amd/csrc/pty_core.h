@@ -59,6 +59,36 @@ PTY_HD float frsq(float x) {
 #endif
 }
 
+// correctly rounded float32 square root: the hardware estimate r is moved to
+// a neighbour when x lies beyond the midpoint between them (midpoints squared
+// exactly in float64)
+PTY_HD float sqrt_rn(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (!(x > 0.f)) return x == 0.f ? x : 0.f;
+  const float r = __builtin_amdgcn_sqrtf(x);
+  const float lo = __uint_as_float(__float_as_uint(r) - 1u);
+  const float hi = __uint_as_float(__float_as_uint(r) + 1u);
+  const double m_lo = ((double)r + (double)lo) * 0.5, m_hi = ((double)r + (double)hi) * 0.5;
+  const double xd = (double)x;
+  if (xd < m_lo * m_lo) return lo;
+  if (xd > m_hi * m_hi) return hi;
+  return r;
+#else
+  return x > 0.f ? std::sqrt(x) : 0.f;
+#endif
+}
+
+// |x + iy| in float32 as sqrt of the float64 sum of the exact squares,
+// rounded once more to float32 (the libm hypotf construction)
+PTY_HD float hypot_f(float x, float y) {
+  const double s = (double)x * (double)x + (double)y * (double)y;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (float)__builtin_sqrt(s);
+#else
+  return (float)std::sqrt(s);
+#endif
+}
+
 // multiply by the quarter-turn of the transform direction: -i (forward), +i (inverse)
 template <bool INV> PTY_HD cf rot90(cf a) { return INV ? mkcf(-a.y, a.x) : mkcf(a.y, -a.x); }
 
@@ -235,6 +265,10 @@ struct FusedArgs {
   const cf* chi_in;        // probe-gradient pass: [B][S][S] chi (natural)
   cf* pgrad_out;           // probe-gradient pass: [B][S][S] (kernel layout)
   float* sim_out;          // simulation pass: [B][S][S] |F|^2 (natural)
+  // mixed state: K probe modes; probe_hat, ps_out and chi_out hold K slices
+  // (mode-major: mode k of slot b at (k * B + b) * S * S), fwd_out the
+  // [B][K] far fields in the staging layout
+  int K;
   // projection sets: coefficients (float32 roundings of the host's float64
   // a, b, c, x = 1 - a - b, y = 1 - c); the exit waves of the previous
   // iteration are read from chi_in and the new ones written to chi_out
@@ -489,6 +523,72 @@ template <int S> struct Fused {
     }
   }
 
+  // ---- mixed state (ptychographic_methods.py:2696-2805) -------------------
+  // mode k > 0: this thread's probe-spectrum values for the first row pass
+  static PTY_HD void p_load_probe(int tid, const FusedArgs& a, int k, Regs& r) {
+    const cf* ph = a.probe_hat + (size_t)k * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = ph[j];
+  }
+
+  // row pass B completes F_k; F_k to the staging buffer; sum_k |F_k|^2 in
+  // r.amp (float32, modes in order, |F| as the reference's abs then square)
+  template <class L> static PTY_HD void p_mixed_far(int tid, const FusedArgs& a, int b, int k,
+                                                    Regs& r, L lds, const cf* tw) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
+    P::template B<false>(t, r.v, tw);
+    cf* far = a.fwd_out + ((size_t)b * a.K + k) * S * S + tid;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      far[(size_t)j * T] = r.v[j];
+      const float m = hypot_f(r.v[j].x, r.v[j].y);
+      r.amp[j] = (k == 0 ? 0.f : r.amp[j]) + m * m;
+    }
+  }
+
+  // far = sqrt(sum_k |F_k|^2), err += (A - far)^2, factor A / far (0 where
+  // far = 0) kept in r.amp
+  static PTY_HD void p_mixed_factor(int tid, const FusedArgs& a, Regs& r) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const float* A = a.amps + (size_t)r.idx * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      const float amp = A[j];
+      const float far = sqrt_rn(r.amp[j]);
+      const float d = amp - far;
+      acc += (double)(d * d);
+      // float32 quotient, correctly rounded through float64 (53 >= 2 * 24 + 2)
+      r.amp[j] = far == 0.f ? 0.f : (float)((double)amp / (double)far);
+    }
+    r.err += acc;
+  }
+
+  // mode k: F_k from the staging buffer, factor F_k - F_k, row pass B'
+  template <class L> static PTY_HD void p_mixed_back(int tid, const FusedArgs& a, int b, int k,
+                                                     Regs& r, L lds, const cf* tw) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+    const cf* far = a.fwd_out + ((size_t)b * a.K + k) * S * S + tid;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      const cf f = far[(size_t)j * T];
+      r.v[j] = csub(cscale(f, r.amp[j]), f);
+    }
+    P::template B<true>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
+  }
+
   // probe gradient, first phase: g = conj(O_patch) * chi from global memory,
   // forward column pass A from registers
   template <class L> static PTY_HD void p_pgrad_colA(int tid, const FusedArgs& a, int b, Regs& r,
@@ -609,6 +709,54 @@ template <int S, class Run> PTY_HD void projection_position(Run& run, const Fuse
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_proj_out(tid, a, b, r, lds, tw); });
+  run.reduce_err(b);  // ends with a barrier
+}
+
+// Mixed-state version of fused_position: per mode the shifted probe, exit
+// wave and far field (staged), then the common amplitude factor, then per
+// mode the inverse transform of factor * F_k - F_k.  ps/chi slices of mode k
+// start at k * B * S * S.
+template <int S, class Run> PTY_HD void mixed_position(Run& run, const FusedArgs& a, int b) {
+  using K = Fused<S>;
+  using Regs = typename K::Regs;
+  for (int k = 0; k < a.K; ++k) {
+    FusedArgs ak = a;
+    ak.probe_hat = a.probe_hat + (size_t)k * S * S;
+    ak.ps_out = a.ps_out ? a.ps_out + (size_t)k * a.B * S * S : nullptr;
+    if (k == 0) {
+      run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, ak, b, r, lds, false, false, true); });
+    } else {
+      run.phase([&](int tid, Regs& r, auto, const cf*) { K::p_load_probe(tid, a, k, r); });
+    }
+    run.barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, ak, r, lds, tw); });
+    run.wave_barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+    run.barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+    run.barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_exitwave_colA(tid, ak, b, r, lds, tw); });
+    run.barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
+    run.barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
+    run.wave_barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_mixed_far(tid, a, b, k, r, lds, tw); });
+    run.barrier();
+  }
+  run.phase([&](int tid, Regs& r, auto, const cf*) { K::p_mixed_factor(tid, a, r); });
+  for (int k = 0; k < a.K; ++k) {
+    FusedArgs ak = a;
+    ak.chi_out = a.chi_out + (size_t)k * a.B * S * S;
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_mixed_back(tid, a, b, k, r, lds, tw); });
+    run.wave_barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+    run.barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+    run.barrier();
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_outputs(tid, ak, b, r, lds, tw); });
+    run.barrier();
+  }
   run.reduce_err(b);  // ends with a barrier
 }
 

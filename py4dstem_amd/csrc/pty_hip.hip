@@ -119,6 +119,19 @@ __global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
   fused_body<S>(a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
 }
 
+// mixed-state forward/adjoint terms, one work group per position
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_mixed(FusedArgs a) {
+  __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
+  __shared__ cf tw[S];
+  __shared__ double red[Frame<S>::T / 64];
+  const int tid = threadIdx.x;
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the first barrier
+  typename Fused<S>::Regs r;
+  GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
+  mixed_position<S>(run, a, a.b0 + (int)blockIdx.x);
+}
+
 // projection-set exit-wave update, one work group per position
 template <int S>
 __global__ __launch_bounds__(Frame<S>::T) void k_projection(FusedArgs a) {
@@ -419,17 +432,6 @@ __device__ float block_max(float v, float* sh) {
 // float neighbours have at most 25 significant bits, so their squares are
 // exact in float64; comparing x with them selects the correctly rounded
 // result (x, a float, can never equal such a square).  x is 0 or normal here.
-__device__ __forceinline__ float sqrt_rn(float x) {
-  if (!(x > 0.f)) return x == 0.f ? x : 0.f;
-  const float r = __builtin_amdgcn_sqrtf(x);
-  const float lo = __uint_as_float(__float_as_uint(r) - 1u);
-  const float hi = __uint_as_float(__float_as_uint(r) + 1u);
-  const double m_lo = ((double)r + (double)lo) * 0.5, m_hi = ((double)r + (double)hi) * 0.5;
-  const double xd = (double)x;
-  if (xd < m_lo * m_lo) return lo;
-  if (xd > m_hi * m_hi) return hi;
-  return r;
-}
 
 // 1 / sqrt(1e-16 + (c1 s)^2 + (c2 m)^2) in float32, every operation rounded
 // (contraction off, correctly rounded sqrt and division)
@@ -476,6 +478,47 @@ __global__ __launch_bounds__(256) void k_object_update(cf* __restrict__ obj, con
                                                        long long npix, float step, float c1, float c2,
                                                        int replace) {
   object_update_body(obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, blockIdx.x, gridDim.x, replace);
+}
+
+// mixed state (ptychographic_methods.py:2945-2975): the mode overlaps summed
+// in float32, modes in order, with a per-block maximum
+__global__ __launch_bounds__(256) void k_mixed_overlap(const float* __restrict__ spsum, int K, long long npix,
+                                                       float* __restrict__ total, float* __restrict__ bmax) {
+  __shared__ float sh[256];
+  float m = 0.f;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    float s = spsum[p];
+    for (int k = 1; k < K; ++k) s = s + spsum[(size_t)k * npix + p];
+    total[p] = s;
+    m = fmaxf(m, s);
+  }
+  m = block_max(m, sh);
+  if (threadIdx.x == 0) bmax[blockIdx.x] = m;
+}
+
+// object += (sum_k step * grad_k) * norm(total), the reference's float32 order
+__global__ __launch_bounds__(256) void k_mixed_object_update(cf* __restrict__ obj, const cf* __restrict__ grad,
+                                                             int K, const float* __restrict__ total,
+                                                             const float* __restrict__ bmax, int nblk,
+                                                             long long npix, float step, float c1, float c2) {
+#pragma clang fp contract(off)
+  __shared__ float sh[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) m = fmaxf(m, bmax[i]);
+  m = block_max(m, sh);
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    float ux = 0.f, uy = 0.f;
+    for (int k = 0; k < K; ++k) {
+      const cf g = grad[(size_t)k * npix + p];
+      ux = ux + step * g.x;
+      uy = uy + step * g.y;
+    }
+    const float n = overlap_norm(total[p], m, c1, c2);
+    const cf o = obj[p];
+    obj[p] = mkcf(o.x + ux * n, o.y + uy * n);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -738,6 +781,97 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_com(cf* __restrict__ prob
   probe_com_body<S>(probe);
 }
 
+// mixed state: the centre of mass of mode 0's intensity (one work group;
+// the same float64 sums as probe_com_body), then every mode shifted by it
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_probe_com_measure(const cf* __restrict__ probe,
+                                                                   float* __restrict__ shift) {
+  using F = Frame<S>;
+  constexpr int T = F::T;
+  __shared__ double red[3][T / 64];
+  const int tid = threadIdx.x;
+  const int cc = F::col_line(tid), n2 = F::col_q(tid);
+  double sI = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) {
+    const int i = n2 + F::N2 * m;
+    const cf v = probe[i * S + cc];
+    const float a = hypotf(v.x, v.y);
+    const double I = (double)(a * a);
+    sI += I;
+    sx += (double)F::fidx(i) * I;
+    sy += (double)F::fidx(cc) * I;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sI += __shfl_xor(sI, off);
+    sx += __shfl_xor(sx, off);
+    sy += __shfl_xor(sy, off);
+  }
+  if ((tid & 63) == 0) {
+    red[0][tid >> 6] = sI;
+    red[1][tid >> 6] = sx;
+    red[2][tid >> 6] = sy;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int w = 0; w < T / 64; ++w) { a += red[0][w]; b += red[1][w]; c += red[2][w]; }
+    shift[0] = (float)(-(b / a));
+    shift[1] = (float)(-(c / a));
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_probe_shift_modes(cf* __restrict__ probe,
+                                                                   const float* __restrict__ shift) {
+  using F = Frame<S>;
+  using K = Fused<S>;
+  __shared__ cf lds_mem[F::LDS_WORDS];
+  __shared__ cf tw[S];
+  const int tid = threadIdx.x;
+  K::twiddles(tid, LdsPtr{tw});
+  LdsPtr lds{lds_mem};
+  typename K::Regs r;
+  cf* pk = probe + (size_t)blockIdx.x * S * S;
+  const float s0 = shift[0], s1 = shift[1];
+  const int cc = F::col_line(tid), n2 = F::col_q(tid);
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) r.v[m] = pk[(n2 + F::N2 * m) * S + cc];
+  __syncthreads();  // twiddle table
+  Pass<S>::template A<false>(n2, r.v, tw);
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
+  __syncthreads();
+  K::template p_colB<false>(tid, r, lds, tw);
+  __syncthreads();
+  K::template p_rowA<false>(tid, r, lds, tw);
+  __syncthreads();
+  {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
+    Pass<S>::template B<false>(t, r.v, tw);
+    const cf ex = K::ramp1(F::kidx(rr), s0);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) {
+      const cf e = cmul(ex, K::ramp1(F::kidx(F::N1 * t + j), s1));
+      r.v[j] = cmul(r.v[j], e);
+    }
+    Pass<S>::template B<true>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
+  }
+  __syncthreads();
+  K::template p_rowA<true>(tid, r, lds, tw);
+  __syncthreads();
+  K::template p_colB<true>(tid, r, lds, tw);
+  __syncthreads();
+  K::p_colAinv_to_regs(tid, r, lds, tw);
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) pk[(n2 + F::N2 * m) * S + cc] = r.v[m];
+}
+
 // per-iteration error: fixed-order float64 sum, normalised
 __device__ __forceinline__ void error_reduce_body(const double* __restrict__ err, int n, double norm,
                                                   double* __restrict__ out) {
@@ -769,6 +903,14 @@ int launch_fused(const FusedArgs& a, int S, hipStream_t stream) {
   if (S == 64) hipLaunchKernelGGL(k_fused<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
   else hipLaunchKernelGGL(k_fused<128>, dim3(grid), dim3(Frame<128>::T), 0, stream, a);
   return check_launch("k_fused");
+}
+
+int launch_mixed(const FusedArgs& a, int S, hipStream_t stream) {
+  const int grid = a.B - a.b0;
+  if (grid <= 0) return PTY_OK;
+  if (S == 64) hipLaunchKernelGGL(k_mixed<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+  else hipLaunchKernelGGL(k_mixed<128>, dim3(grid), dim3(Frame<128>::T), 0, stream, a);
+  return check_launch("k_mixed");
 }
 
 int launch_projection(const FusedArgs& a, int S, hipStream_t stream) {
@@ -869,6 +1011,17 @@ int launch_probe_com(cf* probe, int S, hipStream_t stream) {
   if (S == 64) hipLaunchKernelGGL(k_probe_com<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe);
   else hipLaunchKernelGGL(k_probe_com<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe);
   return check_launch("k_probe_com");
+}
+
+// every one of K modes shifted by mode 0's centre of mass (shift: 2 floats of scratch)
+int launch_probe_com_modes(cf* probe, int K, int S, float* shift, hipStream_t stream) {
+  if (S == 64) hipLaunchKernelGGL(k_probe_com_measure<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe, shift);
+  else hipLaunchKernelGGL(k_probe_com_measure<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe, shift);
+  int rc = check_launch("k_probe_com_measure");
+  if (rc) return rc;
+  if (S == 64) hipLaunchKernelGGL(k_probe_shift_modes<64>, dim3(K), dim3(Frame<64>::T), 0, stream, probe, shift);
+  else hipLaunchKernelGGL(k_probe_shift_modes<128>, dim3(K), dim3(Frame<128>::T), 0, stream, probe, shift);
+  return check_launch("k_probe_shift_modes");
 }
 
 int launch_position_correction(const PosArgs& a, int S, hipStream_t stream) {
@@ -1158,6 +1311,131 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
       return rc;
     if (!q.fix_probe && q.fix_probe_com) {
       if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
+    }
+  }
+  return PTY_OK;
+}
+
+// workspace of the mixed-state loop: per-mode staging, far fields, scatter
+// outputs and probe partials for whole-batch launches
+struct MixedLayout {
+  size_t phat, ps, chi, far, err, grad, spsum, tilemax, total, bmax, ppart, pstride, cpart, shift, all;
+};
+constexpr int MIXED_BLOCKS = 1024;
+
+MixedLayout mixed_layout(const pty_problem& p, const pty_gd_params& q, int K) {
+  MixedLayout L{};
+  const size_t S2 = (size_t)p.S * p.S, B = (size_t)q.max_batch, npix = (size_t)p.Px * p.Py;
+  size_t o = 0;
+  L.phat = o; o = align256(o + K * S2 * 8);
+  L.ps = o; o = align256(o + K * B * S2 * 8);
+  L.chi = o; o = align256(o + K * B * S2 * 8);
+  L.far = o; o = align256(o + K * B * S2 * 8);
+  L.err = o; o = align256(o + (size_t)p.N * 8);
+  L.grad = o; o = align256(o + K * npix * 8);
+  L.spsum = o; o = align256(o + K * npix * 4);
+  L.tilemax = o; o = align256(o + (size_t)K * num_tiles(p.Px, p.Py) * 4);
+  L.total = o; o = align256(o + npix * 4);
+  L.bmax = o; o = align256(o + MIXED_BLOCKS * 4);
+  L.pstride = q.fix_probe ? 0 : align256(probe_ws_bytes(p.S, q.max_batch));
+  L.ppart = o; o = align256(o + K * L.pstride);
+  L.cpart = o; o = align256(o + (size_t)CONSTRAINT_BLOCKS * 16);
+  L.shift = o; o = align256(o + 8);
+  L.all = o;
+  return L;
+}
+
+// num_iter mixed-state GD iterations (mixedstate_ptychography.py:846-1027 with
+// ObjectNDProbeMixedMethodsMixin, ptychographic_methods.py:2696-3000); probe
+// [K][S][S]; the per-iteration probe constraint is the recentring by mode 0's
+// centre of mass (the orthogonalisation runs on the host between calls)
+int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, const int* orders, double* errors,
+              void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  int rc = validate_problem(problem);
+  if (rc) return rc;
+  if (!params) return fail(PTY_E_ARG, "params is NULL");
+  const pty_problem& p = *problem;
+  const pty_gd_params& q = *params;
+  if (K < 1 || K > 64) return fail(PTY_E_ARG, "num_probes=%d out of range 1..64", K);
+  if (q.num_iter < 0 || q.max_batch <= 0 || q.max_batch > p.N)
+    return fail(PTY_E_ARG, "bad num_iter=%d / max_batch=%d for N=%d", q.num_iter, q.max_batch, p.N);
+  if (!orders || !errors || !workspace) return fail(PTY_E_ARG, "null orders/errors/workspace");
+  const MixedLayout L = mixed_layout(p, q, K);
+  if (workspace_bytes < L.all)
+    return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.all);
+  char* ws = (char*)workspace;
+  cf* phat = (cf*)(ws + L.phat);
+  cf* ps = (cf*)(ws + L.ps);
+  cf* chi = (cf*)(ws + L.chi);
+  cf* far = (cf*)(ws + L.far);
+  double* err = (double*)(ws + L.err);
+  cf* grad = (cf*)(ws + L.grad);
+  float* spsum = (float*)(ws + L.spsum);
+  float* tilemax = (float*)(ws + L.tilemax);
+  float* total = (float*)(ws + L.total);
+  float* bmax = (float*)(ws + L.bmax);
+  double* cpart = (double*)(ws + L.cpart);
+  cf* obj = (cf*)p.object;
+  cf* probe = (cf*)p.probe;
+  const size_t S2 = (size_t)p.S * p.S;
+  const long long npix = (long long)p.Px * p.Py;
+  const int ntiles = num_tiles(p.Px, p.Py);
+  const int nblk = grid_for(npix, 256, MIXED_BLOCKS);
+  const std::vector<int> batches = subdivide(p.N, q.max_batch);
+  const double norm = (double)(float)(p.mean_intensity * (float)p.N);
+  const float c1 = (float)(1.0 - (double)q.normalization_min), c2 = q.normalization_min;
+  bool phat_valid = false;
+  for (int it = 0; it < q.num_iter; ++it) {
+    const int* order_it = orders + (size_t)it * p.N;
+    int start = 0;
+    for (int nb : batches) {
+      const int* ob = order_it + start;
+      if (!phat_valid) {
+        if ((rc = launch_fft(probe, phat, K, p.S, false, 0, stream))) return rc;
+        phat_valid = q.fix_probe != 0;
+      }
+      FusedArgs a{};
+      a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
+      a.order = ob; a.N = p.N; a.B = nb; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
+      a.ps_out = ps; a.chi_out = chi; a.err = err + start; a.fwd_out = far; a.K = K;
+      if ((rc = launch_mixed(a, p.S, stream))) return rc;
+      for (int k = 0; k < K; ++k) {
+        const cf* ps_k = ps + (size_t)k * nb * S2;
+        const cf* chi_k = chi + (size_t)k * nb * S2;
+        if (!q.fix_probe) {
+          ProbePartialArgs pa{chi_k, obj, p.positions, ob, p.N, nb, p.S, p.Px, p.Py,
+                              (double*)(ws + L.ppart + k * L.pstride)};
+          hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(nb)), dim3(256), 0,
+                             stream, pa);
+          if ((rc = check_launch("k_probe_partial"))) return rc;
+        }
+        GatherArgs g{};
+        g.ps = ps_k; g.chi = chi_k; g.positions = p.positions; g.order = ob; g.N = p.N; g.B = nb; g.S = p.S;
+        g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad + (size_t)k * npix;
+        g.spsum = spsum + (size_t)k * npix; g.tilemax = tilemax + (size_t)k * ntiles;
+        if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+      }
+      hipLaunchKernelGGL(k_mixed_overlap, dim3(nblk), dim3(256), 0, stream, spsum, K, npix, total, bmax);
+      if ((rc = check_launch("k_mixed_overlap"))) return rc;
+      hipLaunchKernelGGL(k_mixed_object_update, dim3(grid_for(npix, 256)), dim3(256), 0, stream, obj, grad, K,
+                         total, bmax, nblk, npix, q.step_size, c1, c2);
+      if ((rc = check_launch("k_mixed_object_update"))) return rc;
+      if (!q.fix_probe) {
+        for (int k = 0; k < K; ++k) {
+          if ((rc = launch_probe_apply(probe + (size_t)k * S2, (double*)(ws + L.ppart + k * L.pstride), p.S, nb,
+                                       q.step_size, q.normalization_min, stream)))
+            return rc;
+        }
+      }
+      start += nb;
+    }
+    hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
+    if ((rc = check_launch("k_error_reduce"))) return rc;
+    if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
+                                 stream)))
+      return rc;
+    if (!q.fix_probe && q.fix_probe_com) {
+      if ((rc = launch_probe_com_modes(probe, K, p.S, (float*)(ws + L.shift), stream))) return rc;
     }
   }
   return PTY_OK;
@@ -1455,6 +1733,38 @@ int pty_projection_batch(const void* probe_hat, const void* object, int Px, int 
   a.proj_a = (float)projection->a; a.proj_b = (float)projection->b; a.proj_c = (float)projection->c;
   a.proj_x = (float)x; a.proj_y = (float)y; a.exit_valid = projection->exit_valid != 0;
   return launch_projection(a, S, st(stream));
+}
+
+size_t pty_mixed_workspace_bytes(const pty_problem* problem, const pty_gd_params* params, int num_probes) {
+  if (!problem || !params || params->max_batch <= 0 || num_probes < 1) return 0;
+  return mixed_layout(*problem, *params, num_probes).all;
+}
+
+int pty_mixed_run(const pty_problem* problem, const pty_gd_params* params, int num_probes, const int* orders,
+                  double* errors, void* workspace, size_t workspace_bytes, pty_stream_t stream) {
+  return mixed_run(problem, params, num_probes, orders, errors, workspace, workspace_bytes, st(stream));
+}
+
+int pty_mixed_forward_adjoint_batch(const void* probe_hat, int num_probes, const void* object, int Px, int Py,
+                                    const float* positions, int N, const float* amplitudes, const int* order,
+                                    int B, int S, void* ps, void* chi, double* err, void* far,
+                                    pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (Px < S || Py < S) return fail(PTY_E_ARG, "object %dx%d smaller than frame %d", Px, Py, S);
+  if (N <= 0 || B < 0 || num_probes < 1 || num_probes > 64) return fail(PTY_E_ARG, "bad N, B or num_probes");
+  if (!probe_hat || !object || !positions || !amplitudes || !order || !ps || !chi || !err || !far)
+    return fail(PTY_E_ARG, "null buffer in mixed_forward_adjoint_batch");
+  FusedArgs a{};
+  a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions;
+  a.amps = amplitudes; a.order = order; a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0;
+  a.ps_out = (cf*)ps; a.chi_out = (cf*)chi; a.err = err; a.fwd_out = (cf*)far; a.K = num_probes;
+  return launch_mixed(a, S, st(stream));
+}
+
+int pty_probe_com_modes(void* probe, int num_probes, int S, float* shift, pty_stream_t stream) {
+  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!probe || !shift || num_probes < 1) return fail(PTY_E_ARG, "bad probe modes");
+  return launch_probe_com_modes((cf*)probe, num_probes, S, shift, st(stream));
 }
 
 int pty_positions_com(float* positions, int N, const float* initial_com, pty_stream_t stream) {

@@ -209,6 +209,45 @@ def projection_batch(probe_hat, obj, positions, amplitudes_k, order, proj_params
     return ps, err
 
 
+def mixed_workspace_bytes(problem, params, num_probes):
+    import ctypes
+    return int(_lib.load().pty_mixed_workspace_bytes(ctypes.byref(problem), ctypes.byref(params),
+                                                     int(num_probes)))
+
+
+def mixed_run(problem, params, num_probes, orders, errors, workspace):
+    import ctypes
+    _lib.call("pty_mixed_run", ctypes.byref(problem), ctypes.byref(params), int(num_probes),
+              ptr(orders), ptr(errors), ptr(workspace), workspace.numel(), stream_handle())
+
+
+def mixed_forward_adjoint(probe_hats, obj, positions, amplitudes_k, order):
+    """Mixed-state forward pass for the slots of ``order`` with K probe
+    spectra [K, S, S] (kernel layout); returns (P_shift [K, B, S, S],
+    chi [K, B, S, S], err [B], far [B, K, S, S] staging layout)."""
+    Kp, S = probe_hats.shape[0], probe_hats.shape[-1]
+    check_frame(S)
+    _req(probe_hats, torch.complex64, (Kp, S, S), "probe_hats")
+    B = order.numel()
+    Px, Py = obj.shape
+    ps = torch.empty((Kp, B, S, S), dtype=torch.complex64, device=obj.device)
+    chi = torch.empty_like(ps)
+    far = torch.empty((B, Kp, S, S), dtype=torch.complex64, device=obj.device)
+    err = torch.empty(B, dtype=torch.float64, device=obj.device)
+    _lib.call("pty_mixed_forward_adjoint_batch", ptr(probe_hats), Kp, ptr(obj), Px, Py,
+              ptr(positions), positions.shape[0], ptr(amplitudes_k), ptr(order), B, S, ptr(ps),
+              ptr(chi), ptr(err), ptr(far), stream_handle())
+    return ps, chi, err, far
+
+
+def probe_com_modes(probe):
+    """Shift every mode of probe [K, S, S] (in place) by mode 0's centre of mass."""
+    Kp, S = probe.shape[0], probe.shape[-1]
+    check_frame(S)
+    shift = torch.empty(2, dtype=torch.float32, device=probe.device)
+    _lib.call("pty_probe_com_modes", ptr(probe), Kp, S, ptr(shift), stream_handle())
+
+
 def far_to_kernel_layout(far):
     """(B, S, S) far field from the staging layout of forward_adjoint to the
     kernel layout (element j of row-pass thread t -> row t // (S/N1),
@@ -275,7 +314,8 @@ __all__ = ["fft2", "to_kernel_layout", "probe_spectrum", "forward_adjoint", "sca
            "kernel_index", "num_tiles", "position_params", "gd_positions_workspace_bytes",
            "gd_run_positions", "position_correction", "positions_com", "projection_params",
            "projection_workspace_bytes", "projection_run", "projection_batch",
-           "far_to_kernel_layout"]
+           "far_to_kernel_layout", "mixed_workspace_bytes", "mixed_run", "mixed_forward_adjoint",
+           "probe_com_modes"]
 
 
 def pattern_com(frames, mask=None):

@@ -37,6 +37,9 @@ class MultiPtychography:
     def _runs(self, num_iter, kwargs):
         runs = []
         for p, kw in zip(self.items, kwargs):
+            if getattr(p, "_num_probes", 1) > 1:
+                raise NotImplementedError("mixed-state reconstructions run through their own "
+                                          "reconstruct(), not the batched runner")
             kw = dict(kw, num_iter=num_iter, progress_bar=False, store_iterations=False)
             run = p._prepare_run(_allocate=False, **kw)
             if run["projection_params"] is not None:

@@ -312,6 +312,20 @@ int emu_simulate(int S, const float* probe_hat, const float* object, int Px, int
   return 0;
 }
 
+// mixed-state forward/adjoint terms of one batch: same arguments as
+// pty_mixed_forward_adjoint_batch
+int emu_mixed(int S, const float* probe_hat, int K, const float* object, int Px, int Py, const float* positions,
+              int N, const float* amps, const int* order, int B, float* ps, float* chi, double* err, float* far) {
+  FusedArgs a{};
+  a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions; a.amps = amps;
+  a.order = order; a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0; a.K = K;
+  a.ps_out = (cf*)ps; a.chi_out = (cf*)chi; a.err = err; a.fwd_out = (cf*)far;
+  if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) mixed_position<64>(run, a, b); }
+  else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < B; ++b) mixed_position<128>(run, a, b); }
+  else return -1;
+  return 0;
+}
+
 // projection-set exit-wave step of one batch: same arguments as
 // pty_projection_batch (exit waves advanced in place)
 int emu_projection(int S, const float* probe_hat, const float* object, int Px, int Py, const float* positions,

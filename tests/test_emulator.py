@@ -368,3 +368,36 @@ def test_projection_phases_match_oracle(emu, small_case, abc, valid):
     assert rel(ps, rsp) < 1e-6
     assert rel(exits, ref) < 2e-5
     assert abs(err.sum() / e - 1) < 1e-5
+
+
+@pytest.mark.parametrize("K", [1, 3])
+def test_mixed_state_phases_match_oracle(emu, small_case, K):
+    """Mixed-state phases (K probe modes, staged far fields, common amplitude
+    factor) against the oracle's mixed forward, race-free."""
+    _, o = small_case
+    S = 64
+    rng = np.random.default_rng(31 + K)
+    obj = (o._object * np.exp(1j * 0.2 * rng.standard_normal(o._object.shape))).astype(np.complex64)
+    np.random.seed(5)
+    probes = po.mixed_probe_modes(o._probe.astype(np.complex64), K)
+    probes[1:] *= 0.6
+    B = 8
+    order = (np.arange(B, dtype=np.int32) * 11) % o._num_diffraction_patterns
+    pos = o._positions_px
+    k = kidx(emu, S)
+    phat = np.ascontiguousarray(np.fft.fft2(probes)[:, k][:, :, k].astype(np.complex64))
+    amps = np.ascontiguousarray(o._amplitudes[:, k][:, :, k])
+    ps = np.zeros((K, B, S, S), np.complex64)
+    chi = np.zeros_like(ps)
+    far = np.zeros((B, K, S, S), np.complex64)
+    err = np.zeros(B)
+    emu.emu_reset()
+    emu.emu_mixed(S, cptr(phat), K, cptr(obj), obj.shape[0], obj.shape[1], cptr(pos), len(pos),
+                  cptr(amps), cptr(order), B, cptr(ps), cptr(chi), cptr(err), cptr(far))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    m = po.OracleMixedstatePtychography(energy=300e3, num_probes=K)
+    m._region_of_interest_shape = np.array([S, S])
+    sp, _, ref, e = m.forward(obj, probes, pos[order], o._amplitudes[order])
+    assert rel(ps.transpose(1, 0, 2, 3), sp) < 1e-6
+    assert rel(chi.transpose(1, 0, 2, 3), ref) < 2e-5
+    assert abs(err.sum() / e - 1) < 1e-5

@@ -1,0 +1,158 @@
+"""GPU mixed-state ptychography (MixedstatePtychography, gradient descent)
+against the oracle.
+
+Tolerances:
+  * one batch of the mixed-state kernel on the oracle's inputs: P_shift
+    <= 1e-6, chi <= 2e-5 relative L2, error sum <= 1e-5;
+  * mode recentring: <= 2e-6;
+  * whole reconstructions (K = 2, well-conditioned start, probe modes of
+    distinct intensity so that the orthogonalisation's eigenvectors are
+    well defined): object, probe modes and every per-iteration error <= 1e-4.
+"""
+
+import numpy as np
+import pytest
+import scipy.ndimage
+
+import oracle.ptycho_oracle as po
+from oracle import OracleMixedstatePtychography, OracleSingleslicePtychography
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    from py4dstem_amd import _lib
+    _lib.load()
+    return torch.device("cuda", 0)
+
+
+def K():
+    from py4dstem_amd import kernels
+    return kernels
+
+
+def _batch(o, dev, Kp, B, seed):
+    S = int(o._region_of_interest_shape[0])
+    rng = np.random.default_rng(seed)
+    obj = (o._object * np.exp(1j * 0.2 * rng.standard_normal(o._object.shape))).astype(np.complex64)
+    np.random.seed(seed)
+    probes = po.mixed_probe_modes(o._probe.astype(np.complex64), Kp)
+    probes[1:] *= 0.6
+    order = rng.permutation(o._num_diffraction_patterns)[:B].astype(np.int32)
+    pos = o._positions_px
+    amps = K().to_kernel_layout(torch.from_numpy(np.ascontiguousarray(o._amplitudes)).to(dev))
+    idx = K().kernel_index(S)
+    phat = torch.from_numpy(np.ascontiguousarray(
+        np.fft.fft2(probes)[:, idx][:, :, idx].astype(np.complex64))).to(dev)
+    ps, chi, err, _ = K().mixed_forward_adjoint(phat, torch.from_numpy(obj).to(dev),
+                                                torch.from_numpy(pos.copy()).to(dev), amps,
+                                                torch.from_numpy(order).to(dev))
+    m = OracleMixedstatePtychography(energy=300e3, num_probes=Kp)
+    m._region_of_interest_shape = np.array([S, S])
+    sp, _, ref, e = m.forward(obj, probes, pos[order], o._amplitudes[order])
+    return (ps.cpu().numpy().transpose(1, 0, 2, 3), chi.cpu().numpy().transpose(1, 0, 2, 3),
+            err.sum().item(), sp, ref, e)
+
+
+def test_mixed_kernel_matches_oracle_64(dev, small_case):
+    say("mixed batch S=64 K=2")
+    _, o = small_case
+    ps, chi, err, sp, ref, e = _batch(o, dev, 2, 120, 3)
+    say(f"ps {rel(ps, sp):.2e} chi {rel(chi, ref):.2e} err {abs(err / e - 1):.2e}")
+    assert rel(ps, sp) <= 1e-6
+    assert rel(chi, ref) <= 2e-5
+    assert abs(err / e - 1) <= 1e-5
+
+
+def test_mixed_kernel_matches_oracle_128(dev):
+    say("mixed batch S=128 K=3")
+    from py4dstem_amd.synthetic import production_workload
+    w = production_workload(seed=2, scan=(6, 7), object_px=200, num_iter=1)
+    o = OracleSingleslicePtychography(datacube=w["datacube"], **dict(
+        w["init"], initial_object_guess=np.ones((200, 200), np.complex64)))
+    o.preprocess(**w["preprocess"])
+    ps, chi, err, sp, ref, e = _batch(o, dev, 3, 42, 4)
+    say(f"ps {rel(ps, sp):.2e} chi {rel(chi, ref):.2e} err {abs(err / e - 1):.2e}")
+    assert rel(ps, sp) <= 1e-6
+    assert rel(chi, ref) <= 2e-5
+    assert abs(err / e - 1) <= 1e-5
+
+
+@pytest.mark.parametrize("S", [64, 128])
+def test_mode_recentring_matches_oracle(dev, S):
+    say(f"mode recentring S={S}")
+    p = po.ComplexProbeOracle(300e3, (S, S), (0.2, 0.2), 20.0, 2.0).build()._array
+    p = (po.fft_shift(p.astype(np.complex64), np.array([1.3, -0.7])) * 30).astype(np.complex64)
+    np.random.seed(2)
+    modes = po.mixed_probe_modes(p, 3)
+    modes[2] *= 0.5
+    t = torch.from_numpy(modes.copy()).to(dev)
+    K().probe_com_modes(t)
+    m = OracleMixedstatePtychography(energy=300e3, num_probes=3)
+    ref = m.probe_com_constraint(modes.copy())
+    assert rel(t.cpu().numpy(), ref) <= 2e-6
+
+
+def _init_object(shape, seed):
+    rng = np.random.default_rng(seed)
+    return np.exp(0.02j * scipy.ndimage.gaussian_filter(rng.standard_normal(shape), 3)
+                  / 0.05).astype(np.complex64)
+
+
+@pytest.mark.parametrize("orthogonalize", [True, False])
+def test_reconstruct_mixed_state_matches_oracle(dev, orthogonalize):
+    from py4dstem_amd import MixedstatePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say(f"reconstruct mixed state K=2 orthogonalize={orthogonalize}")
+    w = small_workload(seed=1)
+    recon = dict(w["reconstruct"], max_batch_size=256, fix_probe=False, num_iter=4,
+                 orthogonalize_probe=orthogonalize)
+    init_obj = _init_object((256, 256), 17)
+    out = {}
+    for kind, cls in (("gpu", MixedstatePtychography), ("oracle", OracleMixedstatePtychography)):
+        p = cls(datacube=w["datacube"], num_probes=2,
+                **dict(w["init"], initial_object_guess=init_obj.copy()))
+        np.random.seed(77)  # the probe modes' phase ramps
+        p.preprocess(**w["preprocess"])
+        p._probe[1] *= 0.5  # distinct mode intensities: a well-posed eigen decomposition
+        p._probe_initial = p._probe.copy()
+        if hasattr(p, "_probe_dev"):
+            del p._probe_dev
+        out[kind] = p.reconstruct(**recon)
+    g, o = out["gpu"], out["oracle"]
+    err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(np.asarray(g.probe), o.probe):.2e} "
+        f"error {err:.2e} errors {['%.4g' % e for e in o.error_iterations]}")
+    assert np.asarray(g.probe).shape == (2, 64, 64)
+    assert rel(g.object, o.object) <= 1e-4
+    assert rel(np.asarray(g.probe), o.probe) <= 1e-4
+    assert err <= 1e-4
+
+
+def test_mixed_state_argument_errors(dev):
+    from py4dstem_amd import MixedstatePtychography
+    from py4dstem_amd.synthetic import small_workload
+    w = small_workload(seed=1, scan=(8, 8), object_px=128)
+    with pytest.raises(ValueError):
+        MixedstatePtychography(datacube=w["datacube"], **w["init"])  # num_probes missing
+    with pytest.raises(ValueError):
+        MixedstatePtychography(datacube=w["datacube"],
+                               **dict(w["init"], initial_probe_guess=np.ones((64, 64), np.complex64)))
+    p = MixedstatePtychography(datacube=w["datacube"], num_probes=2, **w["init"])
+    p.preprocess(**w["preprocess"])
+    assert p._probe.shape == (2, 64, 64)
+    with pytest.raises(NotImplementedError):
+        p.reconstruct(num_iter=1, reconstruction_method="DM_AP")
+    with pytest.raises(NotImplementedError):
+        p.reconstruct(num_iter=1, fix_positions=False, max_batch_size=16)
+    p.reconstruct(num_iter=2, max_batch_size=16, progress_bar=False, reset=True)
+    assert np.isfinite(p.error) and len(p.error_iterations) == 2

@@ -594,3 +594,149 @@ def test_projection_adjoint_reproduces_consistent_truth(tiny):
     _, new_probe = o.projection_adjoint(obj.copy(), probe.copy(), op0, sp0, pos0, sp0 * op0, 0.0,
                                         False)
     np.testing.assert_allclose(new_probe, probe, rtol=0, atol=2e-5 * np.abs(probe).max())
+
+
+# ---------------------------------------------------------------------------
+# mixed-state probes
+# ---------------------------------------------------------------------------
+def test_mixed_probe_modes_are_phase_ramped_copies():
+    probe = po.ComplexProbeOracle(300e3, (16, 16), (0.4, 0.4), 20.0, 2.0,
+                                  parameters={"defocus": 50.0}).build()._array.astype(np.complex64)
+    np.random.seed(3)
+    modes = po.mixed_probe_modes(probe, 3)
+    np.random.seed(3)
+    r = [np.random.rand() for _ in range(4)]
+    np.testing.assert_array_equal(modes[0], probe)
+    for i in (1, 2):
+        np.testing.assert_allclose(np.abs(modes[i]), np.abs(probe), rtol=1e-5, atol=1e-7)
+        fx, fy = r[2 * (i - 1)] - 0.5, r[2 * (i - 1) + 1] - 0.5
+        f = np.fft.fftfreq(16)
+        ramp = np.exp(-2j * np.pi * (fx * f[:, None] + fy * f[None, :]))
+        np.testing.assert_allclose(modes[i], modes[i - 1] * ramp, rtol=0, atol=1e-5 * np.abs(probe).max())
+
+
+def test_probe_orthogonalization_invariants():
+    rng = np.random.default_rng(8)
+    p = (rng.standard_normal((3, 16, 16)) + 1j * rng.standard_normal((3, 16, 16))).astype(np.complex64)
+    p[1] += 0.8 * p[0]
+    p[2] *= 0.3
+    q = po.probe_orthogonalization(p)
+    G = np.einsum("ixy,jxy->ij", q.conj().astype(np.complex128), q.astype(np.complex128))
+    off = np.abs(G - np.diag(np.diag(G))).max()
+    assert off < 1e-4 * np.abs(np.diag(G)).max()
+    inten = np.real(np.diag(G))
+    assert np.all(np.diff(inten) <= 0)  # largest first
+    np.testing.assert_allclose(inten.sum(), np.sum(np.abs(p.astype(np.complex128)) ** 2), rtol=1e-5)
+    # same span: projecting the input onto the output modes loses nothing
+    Q = q.reshape(3, -1).astype(np.complex128).T
+    P = p.reshape(3, -1).astype(np.complex128).T
+    coef, *_ = np.linalg.lstsq(Q, P, rcond=None)
+    assert np.linalg.norm(Q @ coef - P) < 1e-5 * np.linalg.norm(P)
+
+
+@pytest.fixture(scope="module")
+def mixed_tiny(tiny):
+    o, probe, obj, pos = tiny
+    m = po.OracleMixedstatePtychography(energy=300e3, num_probes=2)
+    m._region_of_interest_shape = np.array([16, 16])
+    m._mode = "c64"
+    rng = np.random.default_rng(2)
+    second = (0.5 * np.roll(probe, 1, axis=0) * np.exp(1j * rng.uniform(0, 6.28))).astype(np.complex64)
+    return m, o, np.stack([probe, second]), obj, pos
+
+
+def test_mixed_forward_single_mode_equals_single_slice(mixed_tiny):
+    m, o, probes, obj, pos = mixed_tiny
+    rng = np.random.default_rng(4)
+    A = rng.uniform(0, 60, (len(pos), 16, 16)).astype(np.float32)
+    sp1, op1, chi1, e1 = o.forward(obj, probes[0], pos, A)
+    sp, op, chi, e = m.forward(obj, probes[:1], pos, A)
+    np.testing.assert_allclose(sp[:, 0], sp1, rtol=0, atol=1e-6 * np.abs(sp1).max())
+    np.testing.assert_allclose(chi[:, 0], chi1, rtol=0, atol=1e-5 * np.abs(chi1).max())
+    assert e == pytest.approx(e1, rel=1e-5)
+
+
+def test_mixed_forward_fixed_point_and_incoherent_sum(mixed_tiny):
+    m, _, probes, obj, pos = mixed_tiny
+    frac = pos - np.round(pos)
+    rows, cols = po.patch_indices(pos, (16, 16), obj.shape)
+    I = 0
+    for k in range(2):
+        I = I + np.abs(np.fft.fft2((po.fft_shift(probes[k], frac) * obj[rows, cols])
+                                   .astype(np.complex128))) ** 2
+    A = np.sqrt(I).astype(np.float32)
+    _, _, chi, e = m.forward(obj, probes, pos, A)
+    assert np.max(np.abs(chi)) < 1e-4 * np.max(np.abs(probes))
+    assert e < 1e-8 * np.sum(A.astype(np.float64) ** 2)
+    # one mode alone does not explain the data
+    _, _, chi1, e1 = m.forward(obj, probes[:1], pos, A)
+    assert e1 > 1e-3 * np.sum(A.astype(np.float64) ** 2)
+
+
+def test_mixed_object_term_is_loss_gradient_fp64(mixed_tiny):
+    """L = sum (sqrt(sum_k |F_k|^2) - A)^2: dL/dRe O + i dL/dIm O =
+    -2 S^2 sum_k scatter(conj(P_k) chi_k) (the un-normalised object term)."""
+    m, _, probes, obj, pos = mixed_tiny
+    m64 = po.OracleMixedstatePtychography(energy=300e3, num_probes=2, mode="c128")
+    m64._region_of_interest_shape = np.array([16, 16])
+    rng = np.random.default_rng(9)
+    A = rng.uniform(5, 40, (len(pos), 16, 16))
+    obj64 = obj.astype(np.complex128)
+    pr64 = probes.astype(np.complex128)
+    sp, op, chi, _ = m64.forward(obj64, pr64, pos, A)
+    g = sum(po.scatter_bincount(np.conj(sp[:, k]) * chi[:, k], pos, (16, 16), obj.shape)
+            for k in range(2))
+    g = -2 * 256 * g
+
+    def loss(o):
+        return m64.forward(o, pr64, pos, A)[3]
+    for (r, c) in [(20, 20), (18, 25), (24, 16)]:
+        h = 1e-6
+        d = np.zeros_like(obj64)
+        d[r, c] = h
+        gr = (loss(obj64 + d) - loss(obj64 - d)) / (2 * h)
+        gi = (loss(obj64 + 1j * d) - loss(obj64 - 1j * d)) / (2 * h)
+        assert abs(gr - g[r, c].real) < 1e-4 * np.abs(g).max()
+        assert abs(gi - g[r, c].imag) < 1e-4 * np.abs(g).max()
+
+
+def test_mixed_adjoint_keeps_consistent_truth(mixed_tiny):
+    m, _, probes, obj, pos = mixed_tiny
+    frac = pos - np.round(pos)
+    rows, cols = po.patch_indices(pos, (16, 16), obj.shape)
+    sp = po.fft_shift(probes, frac)
+    op = obj[rows, cols]
+    zero = np.zeros_like(sp)
+    o2, p2 = m.adjoint(obj.copy(), probes.copy(), op, sp, pos, zero, 0.5, 1, False)
+    np.testing.assert_array_equal(o2, obj)
+    np.testing.assert_array_equal(p2, probes)
+
+
+def test_mixed_adjoint_step_is_normalised_loss_gradient(mixed_tiny):
+    """With normalization_min = 0 the object step times the summed mode
+    overlap sum_k scatter(|P_k|^2) is -1/(2 S^2) times the FP64 finite
+    difference gradient of the mixed loss."""
+    _, _, probes, obj, pos = mixed_tiny
+    m64 = po.OracleMixedstatePtychography(energy=300e3, num_probes=2, mode="c128")
+    m64._region_of_interest_shape = np.array([16, 16])
+    rng = np.random.default_rng(10)
+    A = rng.uniform(5, 40, (len(pos), 16, 16))
+    obj64 = obj.astype(np.complex128)
+    pr64 = probes.astype(np.complex128)
+    sp, op, chi, _ = m64.forward(obj64, pr64, pos, A)
+    new, _ = m64.adjoint(obj64.copy(), pr64.copy(), op, sp, pos, chi, 1.0, 0.0, True)
+    overlap = (po.scatter_bincount(np.abs(sp[:, 0]) ** 2, pos, (16, 16), obj.shape)
+               + po.scatter_bincount(np.abs(sp[:, 1]) ** 2, pos, (16, 16), obj.shape))
+    step = (new - obj64) * np.sqrt(1e-16 + overlap.astype(np.float64) ** 2)
+
+    def loss(o):
+        return m64.forward(o, pr64, pos, A)[3]
+    scale = np.abs(step).max()
+    for (r, c) in [(20, 20), (18, 25), (24, 16)]:
+        h = 1e-6
+        d = np.zeros_like(obj64)
+        d[r, c] = h
+        gr = (loss(obj64 + d) - loss(obj64 - d)) / (2 * h)
+        gi = (loss(obj64 + 1j * d) - loss(obj64 - 1j * d)) / (2 * h)
+        assert abs(-gr / 512 - step[r, c].real) < 1e-4 * scale
+        assert abs(-gi / 512 - step[r, c].imag) < 1e-4 * scale
