@@ -172,6 +172,9 @@ typedef struct pty_problem {
   const float* amplitudes;               /* float32 [N][S][S], kernel layout */
   const unsigned char* fov_mask_inverse; /* uint8 [Px][Py] or NULL */
   float mean_intensity;
+  const float* fourier_mask;             /* float32 [S][S] kernel layout or NULL: the
+                                            detector_fourier_mask of the GD projection
+                                            (ptychographic_methods.py:1603-1616) */
 } pty_problem;
 
 typedef struct pty_gd_params {

@@ -35,7 +35,8 @@ PINNING = {
     "get_shifted_ar": "integer shift == np.roll; bilinear linearity",
     "get_CoM": "closed form on single-pixel and symmetric images",
     "fit_plane": "exact recovery of a noiseless plane",
-    "forward": "Parseval, projection fixed point, A=0 => chi=-psi, angle(0)=0",
+    "forward": "Parseval, projection fixed point, A=0 => chi=-psi, angle(0)=0; detector mask: "
+               "ones = no mask, masked pixels carry no update and count A^2 in the error",
     "adjoint": "noise-free ground truth is a fixed point; FP64 finite differences",
     "loss_gradients": "FP64 central finite differences (object and probe pixels)",
     "object_constraints": "mean phase outside FOV is 0 after shrinkage; |O|<=1",
@@ -715,9 +716,10 @@ class OracleSingleslicePtychography:
         return positions, object_padding_px
 
     # -- forward / adjoint --------------------------------------------------
-    def forward(self, obj, probe, positions_px, amplitudes):
+    def forward(self, obj, probe, positions_px, amplitudes, fourier_mask=None):
         """``_forward`` for GD (ptychographic_methods.py:1482-1527, 1576-1635,
-        1736-1816).  Returns (shifted_probes, object_patches, exit_waves, error)."""
+        1736-1816), with the optional detector mask of :1603-1616.  Returns
+        (shifted_probes, object_patches, exit_waves, error)."""
         mode = self._mode
         frac = positions_px - np.round(positions_px)
         rows, cols = patch_indices(positions_px, self._region_of_interest_shape, obj.shape)
@@ -725,10 +727,14 @@ class OracleSingleslicePtychography:
         object_patches = obj[rows, cols]
         overlap = shifted_probes * object_patches
         fourier_overlap = _fft2(overlap, mode)
+        if fourier_mask is not None:
+            fourier_overlap *= fourier_mask
         farfield = np.abs(fourier_overlap)
         error = np.sum(np.abs(amplitudes - farfield) ** 2)
         modified = amplitudes * np.exp(1j * np.angle(fourier_overlap))
         modified = modified - fourier_overlap
+        if fourier_mask is not None:
+            modified *= fourier_mask
         exit_waves = _ifft2(modified, mode)
         return shifted_probes, object_patches, exit_waves, error
 
@@ -978,7 +984,8 @@ class OracleSingleslicePtychography:
                     max_position_update_distance=None, max_position_total_distance=None,
                     global_affine_transformation=False, reconstruction_method="gradient-descent",
                     reconstruction_parameter=1.0, reconstruction_parameter_a=None,
-                    reconstruction_parameter_b=None, reconstruction_parameter_c=None, **kwargs):
+                    reconstruction_parameter_b=None, reconstruction_parameter_c=None,
+                    detector_fourier_mask=None, **kwargs):
         """phase/singleslice_ptychography.py:606-996 (gradient descent and the
         projection sets)."""
         use_projection, pa, pb, pc = self.method_parameters(
@@ -988,6 +995,8 @@ class OracleSingleslicePtychography:
             raise ValueError("Stochastic object/probe updating is inconsistent with 'DM_AP', "
                              "'RAAR', 'RRR', and 'SUPERFLIP'. Use reconstruction_method='GD' or "
                              "set max_batch_size=None.")
+        if use_projection and detector_fourier_mask is not None:
+            raise NotImplementedError()  # ptychographic_methods.py:1686-1687
         # ptychographic_methods.py:252-294
         if store_iterations and (not hasattr(self, "object_iterations") or reset):
             self.object_iterations = []
@@ -1039,7 +1048,7 @@ class OracleSingleslicePtychography:
                     ew = None
                 else:
                     sp, op, ew, err = self.forward(self._object, self._probe, pos,
-                                                   self._amplitudes[idx])
+                                                   self._amplitudes[idx], detector_fourier_mask)
                     self._object, self._probe = self.adjoint(
                         self._object, self._probe, op, sp, pos, ew, step_size,
                         normalization_min, fix_probe)
@@ -1135,7 +1144,7 @@ class OracleMixedstatePtychography(OracleSingleslicePtychography):
         self._probe_initial = self._probe.copy()
         return self
 
-    def forward(self, obj, probe, positions_px, amplitudes):
+    def forward(self, obj, probe, positions_px, amplitudes, fourier_mask=None):
         """Mixed-state _forward for GD: shifted probes [B][K], far field
         sqrt(sum_k |F_k|^2), chi_k = ifft2(A/far F_k - F_k)."""
         mode = self._mode
@@ -1145,12 +1154,16 @@ class OracleMixedstatePtychography(OracleSingleslicePtychography):
         object_patches = obj[rows, cols]
         overlap = shifted_probes * np.expand_dims(object_patches, axis=1)
         fourier_overlap = _fft2(overlap, mode)
+        if fourier_mask is not None:
+            fourier_overlap *= fourier_mask
         farfield_amplitudes = np.sqrt(np.sum(np.abs(fourier_overlap) ** 2, axis=1))
         error = np.sum(np.abs(amplitudes - farfield_amplitudes) ** 2)
         farfield_amplitudes[farfield_amplitudes == 0.0] = np.inf
         amplitude_modification = amplitudes / farfield_amplitudes
         fourier_modified_overlap = amplitude_modification[:, None] * fourier_overlap
         fourier_modified_overlap = fourier_modified_overlap - fourier_overlap
+        if fourier_mask is not None:
+            fourier_modified_overlap *= fourier_mask
         exit_waves = _ifft2(fourier_modified_overlap, mode)
         return shifted_probes, object_patches, exit_waves, error
 

@@ -24,7 +24,7 @@ class PtyProblem(ctypes.Structure):
     _fields_ = [("S", c_int), ("Px", c_int), ("Py", c_int), ("N", c_int),
                 ("object", c_void_p), ("probe", c_void_p), ("positions", c_void_p),
                 ("amplitudes", c_void_p), ("fov_mask_inverse", c_void_p),
-                ("mean_intensity", c_float)]
+                ("mean_intensity", c_float), ("fourier_mask", c_void_p)]
 
 
 class PtyGdParams(ctypes.Structure):

@@ -265,6 +265,10 @@ struct FusedArgs {
   const cf* chi_in;        // probe-gradient pass: [B][S][S] chi (natural)
   cf* pgrad_out;           // probe-gradient pass: [B][S][S] (kernel layout)
   float* sim_out;          // simulation pass: [B][S][S] |F|^2 (natural)
+  // optional detector mask [S][S] in kernel layout: multiplies F before the
+  // projection and the far-field update after it (ptychographic_methods.py:
+  // 1603-1616, 2773-2792); the staged F of fwd_out stays unmasked
+  const float* fmask;
   // mixed state: K probe modes; probe_hat, ps_out and chi_out hold K slices
   // (mode-major: mode k of slot b at (k * B + b) * S * S), fwd_out the
   // [B][K] far fields in the staging layout
@@ -440,19 +444,23 @@ template <int S> struct Fused {
 
   // P8: row pass B (completing fft2), amplitude projection, row pass B'
   // PROJ = false: g = A e^{i angle F} - F (gradient descent);
-  // PROJ = true:  g = A e^{i angle F}     (projection sets)
-  template <class L, bool PROJ = false>
+  // PROJ = true:  g = A e^{i angle F}     (projection sets);
+  // MASK: a.fmask multiplies F before and g after the projection (a
+  // compile-time choice, so the unmasked kernels carry no mask code)
+  template <class L, bool PROJ = false, bool MASK = false>
   static PTY_HD void p_project(int tid, const FusedArgs& a, int b, Regs& r, L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
     cf* far = a.fwd_out ? a.fwd_out + (size_t)b * S * S + tid : nullptr;
+    const float* fm = MASK ? a.fmask + rr * S + N1 * t : nullptr;
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
-      const cf f = r.v[j];
-      if (far) far[(size_t)j * T] = f;
+      if (far) far[(size_t)j * T] = r.v[j];
+      const float mj = MASK ? fm[j] : 1.f;
+      const cf f = MASK ? cscale(r.v[j], mj) : r.v[j];
       const float amp = r.amp[j];
       const float m2 = cabs2(f);
       cf g;
@@ -468,7 +476,7 @@ template <int S> struct Fused {
       }
       const float d = amp - mag;
       acc += (double)(d * d);
-      r.v[j] = g;
+      r.v[j] = MASK ? cscale(g, mj) : g;
     }
     r.err += acc;
     P::template B<true>(t, r.v, tw);
@@ -543,8 +551,10 @@ template <int S> struct Fused {
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
     cf* far = a.fwd_out + ((size_t)b * a.K + k) * S * S + tid;
+    const float* fm = a.fmask ? a.fmask + rr * S + N1 * t : nullptr;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
+      if (fm) r.v[j] = cscale(r.v[j], fm[j]);
       far[(size_t)j * T] = r.v[j];
       const float m = hypot_f(r.v[j].x, r.v[j].y);
       r.amp[j] = (k == 0 ? 0.f : r.amp[j]) + m * m;
@@ -579,10 +589,12 @@ template <int S> struct Fused {
 #endif
     const int rr = F::row_line(tid), t = F::row_q(tid);
     const cf* far = a.fwd_out + ((size_t)b * a.K + k) * S * S + tid;
+    const float* fm = a.fmask ? a.fmask + rr * S + N1 * t : nullptr;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
       const cf f = far[(size_t)j * T];
-      r.v[j] = csub(cscale(f, r.amp[j]), f);
+      const cf g = csub(cscale(f, r.amp[j]), f);
+      r.v[j] = fm ? cscale(g, fm[j]) : g;
     }
     P::template B<true>(t, r.v, tw);
 #pragma unroll
@@ -651,7 +663,7 @@ template <int S> struct Fused {
 // (the ramp table, column passes, row <-> column); ``run.wave_barrier()``
 // only orders the LDS accesses of one wave, enough between two passes along
 // rows.  Nine work-group barriers per position.
-template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
+template <int S, class Run, bool MASK = false> PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
   using K = Fused<S>;
   using Regs = typename K::Regs;
   // ifft2(probe_hat * ramp): B', A' along rows, then B', A' along columns
@@ -671,7 +683,9 @@ template <int S, class Run> PTY_HD void fused_position(Run& run, const FusedArgs
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<false>(tid, r, lds, tw); });
   run.wave_barrier();
   // far field -> projection -> start of ifft2
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_project(tid, a, b, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::template p_project<decltype(lds), false, MASK>(tid, a, b, r, lds, tw);
+  });
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
   run.barrier();

@@ -101,7 +101,7 @@ template <int S> struct GpuRunner {
   }
 };
 
-template <int S> __device__ __forceinline__ void fused_body(const FusedArgs& a, int b) {
+template <int S, bool MASK = false> __device__ __forceinline__ void fused_body(const FusedArgs& a, int b) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
   __shared__ cf tw[S];
   __shared__ double red[Frame<S>::T / 64];
@@ -111,12 +111,12 @@ template <int S> __device__ __forceinline__ void fused_body(const FusedArgs& a, 
   Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
-  fused_position<S>(run, a, b);
+  fused_position<S, GpuRunner<S>, MASK>(run, a, b);
 }
 
-template <int S>
+template <int S, bool MASK>
 __global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
-  fused_body<S>(a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
+  fused_body<S, MASK>(a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
 }
 
 // mixed-state forward/adjoint terms, one work group per position
@@ -900,8 +900,13 @@ bool frame_supported(int S) { return S == 64 || S == 128; }
 int launch_fused(const FusedArgs& a, int S, hipStream_t stream) {
   const int grid = a.B - a.b0;
   if (grid <= 0) return PTY_OK;
-  if (S == 64) hipLaunchKernelGGL(k_fused<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
-  else hipLaunchKernelGGL(k_fused<128>, dim3(grid), dim3(Frame<128>::T), 0, stream, a);
+  if (a.fmask) {
+    if (S == 64) hipLaunchKernelGGL((k_fused<64, true>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+    else hipLaunchKernelGGL((k_fused<128, true>), dim3(grid), dim3(Frame<128>::T), 0, stream, a);
+  } else {
+    if (S == 64) hipLaunchKernelGGL((k_fused<64, false>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+    else hipLaunchKernelGGL((k_fused<128, false>), dim3(grid), dim3(Frame<128>::T), 0, stream, a);
+  }
   return check_launch("k_fused");
 }
 
@@ -1178,6 +1183,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
         a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
         a.order = ob + s0; a.N = p.N; a.B = ns; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
         a.ps_out = ps; a.chi_out = chi; a.err = err + start + s0; a.fwd_out = far;
+        a.fmask = p.fourier_mask;
         if ((rc = launch_fused(a, p.S, stream))) return rc;
         if (!q.fix_probe) {
           ProbePartialArgs pa{chi, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
@@ -1238,6 +1244,7 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
   if (q.num_iter < 0 || q.max_batch != p.N)
     return fail(PTY_E_ARG, "projection sets update from all %d positions at once (max_batch=%d)", p.N, q.max_batch);
   if (!order || !errors || !workspace || !jp->exit_waves) return fail(PTY_E_ARG, "null order/errors/workspace/exit waves");
+  if (p.fourier_mask) return fail(PTY_E_UNSUPPORTED, "a detector Fourier mask is not defined for projection sets");
   if (!aligned8(jp->exit_waves)) return fail(PTY_E_ARG, "exit waves must be 8-byte aligned");
   const WsLayout L = layout(p, q, false, true);
   if (workspace_bytes < L.total)
@@ -1398,6 +1405,7 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
       a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
       a.order = ob; a.N = p.N; a.B = nb; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
       a.ps_out = ps; a.chi_out = chi; a.err = err + start; a.fwd_out = far; a.K = K;
+      a.fmask = p.fourier_mask;
       if ((rc = launch_mixed(a, p.S, stream))) return rc;
       for (int k = 0; k < K; ++k) {
         const cf* ps_k = ps + (size_t)k * nb * S2;

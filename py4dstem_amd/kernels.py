@@ -151,14 +151,16 @@ def probe_com_constraint(probe):
     return probe
 
 
-def make_problem(obj, probe, positions, amplitudes_k, mask_inverse, mean_intensity):
+def make_problem(obj, probe, positions, amplitudes_k, mask_inverse, mean_intensity,
+                 fourier_mask_k=None):
     S = probe.shape[-1]
     Px, Py = obj.shape
     return PtyProblem(S, Px, Py, positions.shape[0], c_void_p(obj.data_ptr()),
                       c_void_p(probe.data_ptr()), c_void_p(positions.data_ptr()),
                       c_void_p(amplitudes_k.data_ptr()),
                       c_void_p(mask_inverse.data_ptr()) if mask_inverse is not None else None,
-                      float(mean_intensity))
+                      float(mean_intensity),
+                      c_void_p(fourier_mask_k.data_ptr()) if fourier_mask_k is not None else None)
 
 
 def gd_workspace_bytes(problem, params):

@@ -46,6 +46,10 @@ class MultiPtychography:
                 raise NotImplementedError("projection-set methods run through "
                                           "SingleslicePtychography.reconstruct(), not the batched "
                                           "runner")
+            if run["fourier_mask"] is not None:
+                raise NotImplementedError("detector Fourier masks run through "
+                                          "SingleslicePtychography.reconstruct(), not the batched "
+                                          "runner")
             if run["position_params"] is not None:
                 raise NotImplementedError("position correction (fix_positions=False) runs through "
                                           "SingleslicePtychography.reconstruct(), not the batched "

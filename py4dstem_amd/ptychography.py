@@ -605,8 +605,9 @@ class SingleslicePtychography:
             if not fix_positions:
                 raise NotImplementedError("position correction with projection-set methods is "
                                           "not supported on this path")
+            if detector_fourier_mask is not None:  # ptychographic_methods.py:1686-1687
+                raise NotImplementedError()
         unsupported = {
-            "detector_fourier_mask": detector_fourier_mask is not None,
             "gaussian_filter_sigma": gaussian_filter and gaussian_filter_sigma is not None,
             "q_lowpass/q_highpass": butterworth_filter and (q_lowpass is not None
                                                             or q_highpass is not None),
@@ -666,8 +667,18 @@ class SingleslicePtychography:
         use_mask = fix_potential_baseline and self._object_fov_mask_inverse.sum() > 0
         mask_dev = (torch.from_numpy(self._object_fov_mask_inverse.astype(np.uint8)).to(dev)
                     if use_mask else None)
+        fmask_dev = None
+        if detector_fourier_mask is not None:
+            fmask = np.asarray(detector_fourier_mask, dtype=np.float32)
+            if fmask.shape != tuple(self._amplitudes_shape):
+                raise ValueError(f"detector_fourier_mask must have shape "
+                                 f"{tuple(self._amplitudes_shape)}, got {fmask.shape}")
+            # corner-centred like the amplitudes, stored in their kernel layout
+            fmask_dev = K.to_kernel_layout(
+                torch.from_numpy(np.ascontiguousarray(fmask[None])).to(dev))[0].contiguous()
         problem = K.make_problem(self._object_dev, self._probe_dev, self._positions_dev,
-                                 self._amplitudes_dev, mask_dev, self._mean_diffraction_intensity)
+                                 self._amplitudes_dev, mask_dev, self._mean_diffraction_intensity,
+                                 fmask_dev)
         per_call = 1 if (store_iterations or progress_bar or affine) else max(num_iter, 1)
         params = PtyGdParams(per_call, B, int(bool(fix_probe)), int(bool(fix_probe_com)),
                              int(bool(pure_phase_object)), float(step_size),
@@ -706,7 +717,8 @@ class SingleslicePtychography:
         orders_dev = torch.from_numpy(orders).to(dev)
         errors_dev = torch.zeros(max(num_iter, 1), dtype=torch.float64, device=dev)
         self._last_run = dict(problem=problem, params=params, workspace=ws, orders=orders_dev,
-                              errors=errors_dev, mask=mask_dev, num_iter=num_iter,
+                              errors=errors_dev, mask=mask_dev, fourier_mask=fmask_dev,
+                              num_iter=num_iter,
                               per_call=per_call, store_iterations=store_iterations,
                               progress_bar=progress_bar, position_params=pos_params,
                               singular=singular, affine=affine, shuffled=shuffled,

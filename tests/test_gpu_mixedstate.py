@@ -6,8 +6,15 @@ Tolerances:
     <= 1e-6, chi <= 2e-5 relative L2, error sum <= 1e-5;
   * mode recentring: <= 2e-6;
   * whole reconstructions (K = 2, well-conditioned start, probe modes of
-    distinct intensity so that the orthogonalisation's eigenvectors are
-    well defined): object, probe modes and every per-iteration error <= 1e-4.
+    distinct intensity): object and every per-iteration error <= 1e-4; probe
+    modes <= 1e-4, with the orthogonalisation after aligning each mode's
+    global phase.  The orthogonalisation takes each mode's phase from the
+    eigenvectors of the 2x2 Gram matrix, i.e. from the phase of its small
+    off-diagonal element, so float32-level differences between two
+    implementations come out as a per-mode global phase (measured 7e-5 to
+    3e-4 raw); the mode phase is a gauge freedom (intensities add
+    incoherently), which is why the object and the errors still agree to
+    1e-6.
 """
 
 import numpy as np
@@ -132,10 +139,16 @@ def test_reconstruct_mixed_state_matches_oracle(dev, orthogonalize):
     err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
     say(f"object {rel(g.object, o.object):.2e} probe {rel(np.asarray(g.probe), o.probe):.2e} "
         f"error {err:.2e} errors {['%.4g' % e for e in o.error_iterations]}")
-    assert np.asarray(g.probe).shape == (2, 64, 64)
+    gp = np.asarray(g.probe)
+    assert gp.shape == (2, 64, 64)
     assert rel(g.object, o.object) <= 1e-4
-    assert rel(np.asarray(g.probe), o.probe) <= 1e-4
     assert err <= 1e-4
+    if orthogonalize:
+        aligned = np.stack([m * np.exp(1j * np.angle(np.vdot(m, r))) for m, r in zip(gp, o.probe)])
+        say(f"probe after aligning each mode's global phase {rel(aligned, o.probe):.2e}")
+        assert rel(aligned, o.probe) <= 1e-4
+    else:
+        assert rel(gp, o.probe) <= 1e-4
 
 
 def test_mixed_state_argument_errors(dev):

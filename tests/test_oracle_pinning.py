@@ -740,3 +740,27 @@ def test_mixed_adjoint_step_is_normalised_loss_gradient(mixed_tiny):
         gi = (loss(obj64 + 1j * d) - loss(obj64 - 1j * d)) / (2 * h)
         assert abs(-gr / 512 - step[r, c].real) < 1e-4 * scale
         assert abs(-gi / 512 - step[r, c].imag) < 1e-4 * scale
+
+
+def test_detector_mask_closed_forms(tiny):
+    o, probe, obj, pos = tiny
+    rng = np.random.default_rng(13)
+    A = rng.uniform(1, 50, (len(pos), 16, 16)).astype(np.float32)
+    _, _, chi0, e0 = o.forward(obj, probe, pos, A)
+    _, _, chi1, e1 = o.forward(obj, probe, pos, A, np.ones((16, 16), np.float32))
+    np.testing.assert_array_equal(chi1, chi0)
+    assert e1 == e0
+    mask = (rng.random((16, 16)) > 0.3).astype(np.float32)
+    _, _, chim, em = o.forward(obj, probe, pos, A, mask)
+    G0 = np.fft.fft2(chi0.astype(np.complex128))
+    Gm = np.fft.fft2(chim.astype(np.complex128))
+    # masked pixels carry no update, the others the unmasked one
+    assert np.abs(Gm[:, mask == 0]).max() < 1e-5 * np.abs(G0).max()
+    np.testing.assert_allclose(Gm[:, mask == 1], G0[:, mask == 1], rtol=0,
+                               atol=1e-5 * np.abs(G0).max())
+    # and count A^2 in the error
+    rows, cols = po.patch_indices(pos, (16, 16), obj.shape)
+    F = np.abs(np.fft.fft2((po.fft_shift(probe, pos - np.round(pos)) * obj[rows, cols])
+                           .astype(np.complex128)))
+    ref = np.sum(np.where(mask == 1, (A - F) ** 2, A.astype(np.float64) ** 2))
+    assert em == pytest.approx(ref, rel=1e-4)
