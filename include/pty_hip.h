@@ -188,6 +188,14 @@ typedef struct pty_gd_params {
   float shrinkage_rad;
   int stage;            /* positions per fused launch (0 = whole mini-batch); a
                            performance knob only: results are bitwise identical */
+  /* probe constraints after the recentring, when the probe is updated
+   * (ptychographic_constraints.py:1091-1147; with modes the aperture acts on
+   * mode 0, the amplitude top-hat on every mode) */
+  int fix_probe_aperture;         /* Fourier amplitude := probe_aperture (:1004-1033) */
+  const float* probe_aperture;    /* float32 [S][S] kernel layout: |fft2(initial probe)| */
+  int constrain_probe_amplitude;  /* real-space erf top-hat (:920-957) */
+  double probe_amplitude_radius;  /* constrain_probe_amplitude_relative_radius */
+  double probe_amplitude_width;   /* constrain_probe_amplitude_relative_width */
 } pty_gd_params;
 
 size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);

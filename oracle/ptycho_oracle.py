@@ -41,6 +41,10 @@ PINNING = {
     "loss_gradients": "FP64 central finite differences (object and probe pixels)",
     "object_constraints": "mean phase outside FOV is 0 after shrinkage; |O|<=1",
     "probe_com_constraint": "probe CoM ~ 0 after recentring",
+    "probe_aperture_constraint": "output Fourier amplitude proportional to the aperture, "
+                                 "Fourier phase and intensity kept",
+    "probe_amplitude_constraint": "intensity kept; erf edge value 1/2 at the radius; identity "
+                                  "inside for a narrow edge",
     "position_partials": "brute-force DFT loops on tiny frames with np.roll'ed objects",
     "position_correction": "exact recovery of a planted least-squares step; clipping "
                            "closed forms; sign/magnitude on a sub-pixel displaced scan",
@@ -662,6 +666,7 @@ class OracleSingleslicePtychography:
             probe = np.asarray(self._probe, dtype=np.complex64)
         self._probe = probe
         self._probe_initial = probe.copy()
+        self._probe_initial_aperture = np.abs(_fft2(probe, mode))
 
         # FOV mask (singleslice_ptychography.py:504-537)
         if object_fov_mask is None:
@@ -957,6 +962,30 @@ class OracleSingleslicePtychography:
         """Probe constraints of subclasses (none for a single probe)."""
         return probe
 
+    def probe_aperture_constraint(self, probe, initial_aperture):
+        """ptychographic_constraints.py:1004-1033: Fourier amplitude replaced
+        by the initial aperture, intensity kept."""
+        current_probe_sum = np.sum(np.abs(probe) ** 2)
+        phase = np.angle(_fft2(probe, self._mode))
+        updated = _ifft2(np.exp(1j * phase) * initial_aperture, self._mode)
+        updated_sum = np.sum(np.abs(updated) ** 2)
+        return updated * np.sqrt(current_probe_sum / updated_sum)
+
+    @staticmethod
+    def probe_amplitude_constraint(probe, relative_radius, relative_width):
+        """ptychographic_constraints.py:920-957: real-space top-hat with an
+        erf edge, intensity kept."""
+        from scipy.special import erf
+        current_probe_sum = np.sum(np.abs(probe) ** 2)
+        X = np.fft.fftfreq(probe.shape[0])[:, None]
+        Y = np.fft.fftfreq(probe.shape[1])[None]
+        r = np.hypot(X, Y) - relative_radius
+        sigma = np.sqrt(np.pi) / relative_width
+        tophat_mask = 0.5 * (1 - erf(sigma * r / (1 - r ** 2)))
+        updated = probe * tophat_mask
+        updated_sum = np.sum(np.abs(updated) ** 2)
+        return updated * np.sqrt(current_probe_sum / updated_sum)
+
     @staticmethod
     def positions_constraints(current_positions, initial_positions, fix_positions,
                               fix_positions_com, global_affine_transformation):
@@ -985,7 +1014,10 @@ class OracleSingleslicePtychography:
                     global_affine_transformation=False, reconstruction_method="gradient-descent",
                     reconstruction_parameter=1.0, reconstruction_parameter_a=None,
                     reconstruction_parameter_b=None, reconstruction_parameter_c=None,
-                    detector_fourier_mask=None, **kwargs):
+                    detector_fourier_mask=None, fix_probe_aperture=False,
+                    constrain_probe_amplitude=False,
+                    constrain_probe_amplitude_relative_radius=0.5,
+                    constrain_probe_amplitude_relative_width=0.05, **kwargs):
         """phase/singleslice_ptychography.py:606-996 (gradient descent and the
         projection sets)."""
         use_projection, pa, pb, pc = self.method_parameters(
@@ -1063,6 +1095,25 @@ class OracleSingleslicePtychography:
                                                    shrinkage_rad, pure_phase_object)
             if fix_probe_com and not fix_probe:
                 self._probe = self.probe_com_constraint(self._probe)
+            # ptychographic_constraints.py:1091-1147 (single probe) and
+            # 1218-1286 (modes: the aperture on mode 0, the amplitude on every mode)
+            if fix_probe_aperture and not fix_probe:
+                if self._probe.ndim == 3:
+                    self._probe[0] = self.probe_aperture_constraint(self._probe[0],
+                                                                    self._probe_initial_aperture[0])
+                else:
+                    self._probe = self.probe_aperture_constraint(self._probe,
+                                                                 self._probe_initial_aperture)
+            if constrain_probe_amplitude and not fix_probe:
+                if self._probe.ndim == 3:
+                    for k in range(self._probe.shape[0]):
+                        self._probe[k] = self.probe_amplitude_constraint(
+                            self._probe[k], constrain_probe_amplitude_relative_radius,
+                            constrain_probe_amplitude_relative_width)
+                else:
+                    self._probe = self.probe_amplitude_constraint(
+                        self._probe, constrain_probe_amplitude_relative_radius,
+                        constrain_probe_amplitude_relative_width)
             self._probe = self.probe_extra_constraints(self._probe)
             self._positions_px = self.positions_constraints(
                 self._positions_px, self._positions_px_initial, fix_positions,
@@ -1142,6 +1193,7 @@ class OracleMixedstatePtychography(OracleSingleslicePtychography):
         self._probe = (given.copy() if given is not None
                        else mixed_probe_modes(self._probe, self._num_probes))
         self._probe_initial = self._probe.copy()
+        self._probe_initial_aperture = np.abs(_fft2(self._probe, self._mode))
         return self
 
     def forward(self, obj, probe, positions_px, amplitudes, fourier_mask=None):

@@ -31,7 +31,10 @@ class PtyGdParams(ctypes.Structure):
     _fields_ = [("num_iter", c_int), ("max_batch", c_int), ("fix_probe", c_int),
                 ("fix_probe_com", c_int), ("pure_phase_object", c_int),
                 ("step_size", c_float), ("normalization_min", c_float),
-                ("shrinkage_rad", c_float), ("stage", c_int)]
+                ("shrinkage_rad", c_float), ("stage", c_int),
+                ("fix_probe_aperture", c_int), ("probe_aperture", c_void_p),
+                ("constrain_probe_amplitude", c_int), ("probe_amplitude_radius", ctypes.c_double),
+                ("probe_amplitude_width", ctypes.c_double)]
 
 
 class PtyPositionParams(ctypes.Structure):

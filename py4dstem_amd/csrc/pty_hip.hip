@@ -892,6 +892,136 @@ __global__ __launch_bounds__(1024) void k_error_reduce(const double* __restrict_
   error_reduce_body(err, n, norm, out);
 }
 
+// sum over the work group of a per-thread float64 value (thread 0 holds it
+// after the call; every thread must call it)
+template <int T> __device__ __forceinline__ double block_sum_f64(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < T / 64; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// fix_probe_aperture (ptychographic_constraints.py:1004-1033): the probe's
+// Fourier amplitude replaced by the initial aperture (kernel layout), its
+// Fourier phase kept, the real-space intensity restored; one work group
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_probe_aperture(cf* __restrict__ probe,
+                                                                const float* __restrict__ aperture) {
+#pragma clang fp contract(off)
+  using F = Frame<S>;
+  using K = Fused<S>;
+  constexpr int T = F::T;
+  __shared__ cf lds_mem[F::LDS_WORDS];
+  __shared__ cf tw[S];
+  __shared__ double red[T / 64];
+  __shared__ float norm;
+  const int tid = threadIdx.x;
+  K::twiddles(tid, LdsPtr{tw});
+  LdsPtr lds{lds_mem};
+  typename K::Regs r;
+  const int cc = F::col_line(tid), n2 = F::col_q(tid);
+  double s = 0.0;
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) {
+    r.v[m] = probe[(n2 + F::N2 * m) * S + cc];
+    const float a = hypot_f(r.v[m].x, r.v[m].y);
+    s += (double)(a * a);
+  }
+  const double s0 = block_sum_f64<T>(s, red);  // also publishes the twiddles
+  Pass<S>::template A<false>(n2, r.v, tw);
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) lds.st(F::at(n2 + F::N2 * m, cc), r.v[m]);
+  __syncthreads();
+  K::template p_colB<false>(tid, r, lds, tw);
+  __syncthreads();
+  K::template p_rowA<false>(tid, r, lds, tw);
+  __syncthreads();
+  {
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
+    Pass<S>::template B<false>(t, r.v, tw);
+    const float* ap = aperture + rr * S + F::N1 * t;
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) {
+      const cf f = r.v[j];
+      const float m2 = cabs2(f);
+      const cf u = m2 > 0.f ? cscale(f, frsq(m2)) : mkcf(1.f, 0.f);  // exp(i angle F)
+      r.v[j] = cscale(u, ap[j]);
+    }
+    Pass<S>::template B<true>(t, r.v, tw);
+#pragma unroll
+    for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
+  }
+  __syncthreads();
+  K::template p_rowA<true>(tid, r, lds, tw);
+  __syncthreads();
+  K::template p_colB<true>(tid, r, lds, tw);
+  __syncthreads();
+  K::p_colAinv_to_regs(tid, r, lds, tw);
+  s = 0.0;
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) {
+    const float a = hypot_f(r.v[m].x, r.v[m].y);
+    s += (double)(a * a);
+  }
+  const double s1 = block_sum_f64<T>(s, red);
+  if (tid == 0) norm = sqrt_rn((float)s0 / (float)s1);
+  __syncthreads();
+  const float n = norm;
+#pragma unroll
+  for (int m = 0; m < F::N1; ++m) probe[(n2 + F::N2 * m) * S + cc] = cscale(r.v[m], n);
+}
+
+// constrain_probe_amplitude (ptychographic_constraints.py:920-957): real-space
+// top-hat 0.5 (1 - erf(sigma r / (1 - r^2))), r = |q| - radius, in float64 as
+// the reference evaluates it, intensity restored; one work group per mode
+template <int S> __device__ __forceinline__ double tophat_mask(int i, int c, double radius, double sigma) {
+  using F = Frame<S>;
+  const double x = (double)F::fidx(i) / S, y = (double)F::fidx(c) / S;
+  const double r = sqrt(x * x + y * y) - radius;
+  return 0.5 * (1.0 - erf(sigma * r / (1.0 - r * r)));
+}
+
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_probe_amplitude(cf* __restrict__ probe, double radius,
+                                                                 double width) {
+  using F = Frame<S>;
+  constexpr int T = F::T;
+  __shared__ double red[T / 64];
+  __shared__ double norm;
+  const int tid = threadIdx.x;
+  cf* pk = probe + (size_t)blockIdx.x * S * S;
+  const int cc = F::col_line(tid), n2 = F::col_q(tid);
+  const double sigma = 1.7724538509055160273 / width;  // sqrt(pi) / width
+  double s = 0.0, s1 = 0.0;
+  for (int m = 0; m < F::N1; ++m) {  // sums first; the mask is recomputed below
+    const int i = n2 + F::N2 * m;
+    const cf v = pk[i * S + cc];
+    const float a = hypot_f(v.x, v.y);
+    s += (double)(a * a);
+    const double mk = tophat_mask<S>(i, cc, radius, sigma);
+    const double ux = (double)v.x * mk, uy = (double)v.y * mk;
+    s1 += ux * ux + uy * uy;
+  }
+  const double s0 = block_sum_f64<T>(s, red);
+  const double t1 = block_sum_f64<T>(s1, red);
+  if (tid == 0) norm = sqrt((double)(float)s0 / t1);
+  __syncthreads();
+  const double n = norm;
+  for (int m = 0; m < F::N1; ++m) {
+    const int i = n2 + F::N2 * m;
+    const cf v = pk[i * S + cc];
+    const double mk = tophat_mask<S>(i, cc, radius, sigma) * n;
+    pk[i * S + cc] = mkcf((float)((double)v.x * mk), (float)((double)v.y * mk));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
@@ -1016,6 +1146,28 @@ int launch_probe_com(cf* probe, int S, hipStream_t stream) {
   if (S == 64) hipLaunchKernelGGL(k_probe_com<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe);
   else hipLaunchKernelGGL(k_probe_com<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe);
   return check_launch("k_probe_com");
+}
+
+// the probe constraints after the recentring (ptychographic_constraints.py:
+// 1091-1147, 1218-1286): the aperture on mode 0, the amplitude top-hat on
+// every mode
+int launch_probe_extra(cf* probe, int K, int S, const pty_gd_params& q, hipStream_t stream) {
+  int rc = PTY_OK;
+  if (q.fix_probe_aperture) {
+    if (!q.probe_aperture) return fail(PTY_E_ARG, "fix_probe_aperture needs the initial aperture");
+    if (S == 64) hipLaunchKernelGGL(k_probe_aperture<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe,
+                                    q.probe_aperture);
+    else hipLaunchKernelGGL(k_probe_aperture<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe,
+                            q.probe_aperture);
+    if ((rc = check_launch("k_probe_aperture"))) return rc;
+  }
+  if (q.constrain_probe_amplitude) {
+    const double rad = q.probe_amplitude_radius, wid = q.probe_amplitude_width;
+    if (S == 64) hipLaunchKernelGGL(k_probe_amplitude<64>, dim3(K), dim3(Frame<64>::T), 0, stream, probe, rad, wid);
+    else hipLaunchKernelGGL(k_probe_amplitude<128>, dim3(K), dim3(Frame<128>::T), 0, stream, probe, rad, wid);
+    if ((rc = check_launch("k_probe_amplitude"))) return rc;
+  }
+  return rc;
 }
 
 // every one of K modes shifted by mode 0's centre of mass (shift: 2 floats of scratch)
@@ -1223,6 +1375,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
     if (!q.fix_probe && q.fix_probe_com) {
       if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
     }
+    if (!q.fix_probe && (rc = launch_probe_extra(probe, 1, p.S, q, stream))) return rc;
     if (correct && pp->center_of_mass) {
       if ((rc = launch_positions_com(pp->positions, p.N, pp->initial_com, stream))) return rc;
     }
@@ -1319,6 +1472,7 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
     if (!q.fix_probe && q.fix_probe_com) {
       if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
     }
+    if (!q.fix_probe && (rc = launch_probe_extra(probe, 1, p.S, q, stream))) return rc;
   }
   return PTY_OK;
 }
@@ -1445,6 +1599,7 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
     if (!q.fix_probe && q.fix_probe_com) {
       if ((rc = launch_probe_com_modes(probe, K, p.S, (float*)(ws + L.shift), stream))) return rc;
     }
+    if (!q.fix_probe && (rc = launch_probe_extra(probe, K, p.S, q, stream))) return rc;
   }
   return PTY_OK;
 }

@@ -145,6 +145,8 @@ class MixedstatePtychography(SingleslicePtychography):
             fix_probe_com=fix_probe_com, fix_probe=fix_probe,
             fix_probe_aperture=fix_probe_aperture,
             constrain_probe_amplitude=constrain_probe_amplitude,
+            constrain_probe_amplitude_relative_radius=constrain_probe_amplitude_relative_radius,
+            constrain_probe_amplitude_relative_width=constrain_probe_amplitude_relative_width,
             constrain_probe_fourier_amplitude=constrain_probe_fourier_amplitude,
             gaussian_filter_sigma=gaussian_filter_sigma, gaussian_filter=gaussian_filter,
             fit_probe_aberrations=fit_probe_aberrations, butterworth_filter=butterworth_filter,

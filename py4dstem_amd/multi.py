@@ -54,6 +54,10 @@ class MultiPtychography:
                 raise NotImplementedError("position correction (fix_positions=False) runs through "
                                           "SingleslicePtychography.reconstruct(), not the batched "
                                           "runner")
+            if run["params"].fix_probe_aperture or run["params"].constrain_probe_amplitude:
+                raise NotImplementedError("probe aperture/amplitude constraints run through "
+                                          "SingleslicePtychography.reconstruct(), not the batched "
+                                          "runner")
             runs.append(run)
         return runs
 

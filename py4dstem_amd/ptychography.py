@@ -613,9 +613,9 @@ class SingleslicePtychography:
                                                             or q_highpass is not None),
             "tv_denoise_weight": tv_denoise and tv_denoise_weight is not None,
             "fit_probe_aberrations": fit_probe_aberrations and not fix_probe,
-            "fix_probe_aperture": fix_probe_aperture and not fix_probe,
-            "constrain_probe_amplitude": constrain_probe_amplitude and not fix_probe,
-            "constrain_probe_fourier_amplitude": constrain_probe_fourier_amplitude and not fix_probe,
+            # the aperture constraint takes precedence (ptychographic_constraints.py:1127-1137)
+            "constrain_probe_fourier_amplitude": (constrain_probe_fourier_amplitude
+                                                  and not fix_probe and not fix_probe_aperture),
         }
         bad = [k for k, v in unsupported.items() if v]
         if bad:
@@ -684,6 +684,24 @@ class SingleslicePtychography:
                              int(bool(pure_phase_object)), float(step_size),
                              float(normalization_min), float(shrinkage_rad),
                              int(self._stage_for(B)))
+        aperture_dev = None
+        if fix_probe_aperture and not fix_probe:
+            # |fft2| of the initial probe (mode 0 with several modes), in the
+            # kernel layout of the transforms
+            ap = getattr(self, "_probe_initial_aperture", None)
+            if ap is None:
+                raise ValueError("fix_probe_aperture needs the initial probe aperture, which "
+                                 "preprocess(store_initial_arrays=True) records")
+            ap = np.asarray(ap, dtype=np.float32)
+            ap = ap[0] if ap.ndim == 3 else ap
+            aperture_dev = K.to_kernel_layout(
+                torch.from_numpy(np.ascontiguousarray(ap[None])).to(dev))[0].contiguous()
+            params.fix_probe_aperture = 1
+            params.probe_aperture = aperture_dev.data_ptr()
+        if constrain_probe_amplitude and not fix_probe:
+            params.constrain_probe_amplitude = 1
+            params.probe_amplitude_radius = float(constrain_probe_amplitude_relative_radius)
+            params.probe_amplitude_width = float(constrain_probe_amplitude_relative_width)
         pos_params = singular = proj_params = None
         if projection is not None:
             if getattr(self, "_exit_waves", None) is None:
@@ -718,6 +736,7 @@ class SingleslicePtychography:
         errors_dev = torch.zeros(max(num_iter, 1), dtype=torch.float64, device=dev)
         self._last_run = dict(problem=problem, params=params, workspace=ws, orders=orders_dev,
                               errors=errors_dev, mask=mask_dev, fourier_mask=fmask_dev,
+                              probe_aperture=aperture_dev,
                               num_iter=num_iter,
                               per_call=per_call, store_iterations=store_iterations,
                               progress_bar=progress_bar, position_params=pos_params,

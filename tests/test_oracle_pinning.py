@@ -764,3 +764,29 @@ def test_detector_mask_closed_forms(tiny):
                            .astype(np.complex128)))
     ref = np.sum(np.where(mask == 1, (A - F) ** 2, A.astype(np.float64) ** 2))
     assert em == pytest.approx(ref, rel=1e-4)
+
+
+def test_probe_aperture_and_amplitude_constraints():
+    o = po.OracleSingleslicePtychography(energy=300e3)
+    o._mode = "c64"
+    rng = np.random.default_rng(17)
+    p = po.ComplexProbeOracle(300e3, (32, 32), (0.3, 0.3), 20.0, 2.0,
+                              parameters={"defocus": 80.0}).build()._array.astype(np.complex64)
+    noisy = (p + 0.05 * np.abs(p).max() * (rng.standard_normal(p.shape)
+                                           + 1j * rng.standard_normal(p.shape))).astype(np.complex64)
+    aperture = np.abs(np.fft.fft2(p)).astype(np.float32)
+    out = o.probe_aperture_constraint(noisy, aperture)
+    Fo = np.fft.fft2(out.astype(np.complex128))
+    scale = np.sum(np.abs(Fo)) / np.sum(aperture)
+    np.testing.assert_allclose(np.abs(Fo), scale * aperture, rtol=0, atol=1e-4 * aperture.max() * scale)
+    Fn = np.fft.fft2(noisy.astype(np.complex128))
+    on = aperture > 1e-3 * aperture.max()
+    np.testing.assert_allclose(np.angle(Fo[on] * np.conj(Fn[on])), 0, atol=2e-3)
+    assert np.sum(np.abs(out) ** 2) == pytest.approx(np.sum(np.abs(noisy) ** 2), rel=1e-5)
+    # amplitude top-hat: intensity kept, factor 1/2 on the radius, ~1 well inside
+    q = po.OracleSingleslicePtychography.probe_amplitude_constraint(noisy, 0.25, 0.01)
+    assert np.sum(np.abs(q) ** 2) == pytest.approx(np.sum(np.abs(noisy) ** 2), rel=1e-5)
+    ratio = np.abs(q) / np.maximum(np.abs(noisy), 1e-30)
+    c = ratio[0, 0]  # the normalisation factor (mask 1 at r = 0)
+    np.testing.assert_allclose(ratio[8, 0], 0.5 * c, rtol=1e-4)  # r = 8/32 = 0.25
+    assert ratio[0, 12] < 1e-6 * c and abs(ratio[2, 2] / c - 1) < 1e-6
