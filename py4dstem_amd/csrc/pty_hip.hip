@@ -524,7 +524,10 @@ __global__ __launch_bounds__(256) void k_mixed_object_update(cf* __restrict__ ob
 // ---------------------------------------------------------------------------
 // probe update: partial sums over chunks of the batch, then one work group
 // ---------------------------------------------------------------------------
-constexpr int PROBE_CHUNK = 64;
+#ifndef PTY_PROBE_CHUNK
+#define PTY_PROBE_CHUNK 64
+#endif
+constexpr int PROBE_CHUNK = PTY_PROBE_CHUNK;  // batch slots per partial sum (a multiple of 16)
 
 struct ProbePartialArgs {
   const cf* chi;
@@ -536,24 +539,34 @@ struct ProbePartialArgs {
 };
 
 __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, int blk, int chunk) {
+  __shared__ int anchor[2 * PROBE_CHUNK];
   const int S = a.S;
+  const int b0 = chunk * PROBE_CHUNK;
+  if (b0 >= a.B) return;  // uniform over the work group
+  const int nb = min(a.B - b0, PROBE_CHUNK);
+  // the chunk's rounded patch anchors, once per work group: the loop below
+  // then issues only independent object and chi loads
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+    int idx = a.order[b0 + t];
+    idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
+    anchor[2 * t] = (int)rintf(a.positions[2 * idx]);
+    anchor[2 * t + 1] = (int)rintf(a.positions[2 * idx + 1]);
+  }
+  __syncthreads();
   const int pix = blk * blockDim.x + threadIdx.x;
-  if (chunk * PROBE_CHUNK >= a.B) return;
   if (pix >= S * S) return;
   const int i = pix / S, j = pix % S;
   const int fi = i < S / 2 ? i : i - S;
   const int fj = j < S / 2 ? j : j - S;
-  const int b1 = min(a.B, (chunk + 1) * PROBE_CHUNK);
+  const cf* chi = a.chi + (size_t)b0 * S * S + pix;
   double re = 0.0, im = 0.0, so = 0.0;
-  for (int b = chunk * PROBE_CHUNK; b < b1; ++b) {
-    int idx = a.order[b];
-    idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
-    const int x0 = (int)rintf(a.positions[2 * idx]);
-    const int y0 = (int)rintf(a.positions[2 * idx + 1]);
+#pragma unroll 8
+  for (int k = 0; k < nb; ++k) {  // batch order
+    const int x0 = anchor[2 * k], y0 = anchor[2 * k + 1];
     const cf o = a.object[(size_t)wrapi(x0 + fi, a.Px) * a.Py + wrapi(y0 + fj, a.Py)];
     const float ao = hypotf(o.x, o.y);
     so += (double)(ao * ao);
-    const cf g = cconjmul(o, a.chi[((size_t)b * S + i) * S + j]);
+    const cf g = cconjmul(o, chi[(size_t)k * S * S]);
     re += (double)g.x;
     im += (double)g.y;
   }
@@ -567,30 +580,60 @@ __global__ __launch_bounds__(256) void k_probe_partial(ProbePartialArgs a) {
   probe_partial_body(a, blockIdx.x, blockIdx.y);
 }
 
-// per-pixel sum of the chunk partials; per-block max of sum |O|^2
+// per-pixel sum of the chunk partials: PROBE_REDUCE_GROUPS contiguous chunk
+// ranges summed side by side, then added in range order; max of sum |O|^2
+// per 256 pixels.  Work group: 256 pixels x PROBE_REDUCE_GROUPS threads.
+constexpr int PROBE_REDUCE_GROUPS = 4;
+constexpr int PROBE_REDUCE_THREADS = 256 * PROBE_REDUCE_GROUPS;
+
 __device__ __forceinline__ void probe_reduce_body(const double* __restrict__ part, int nchunk, int npix,
                                                   cf* __restrict__ gsum, float* __restrict__ osum,
                                                   float* __restrict__ bmax, int blk) {
+  constexpr int G = PROBE_REDUCE_GROUPS;
+  __shared__ double acc[G - 1][3][256];
   __shared__ float sh[256];
-  const int pix = blk * blockDim.x + threadIdx.x;
-  float so = 0.f;
+  const int lane = threadIdx.x & 255, g = threadIdx.x >> 8;
+  const int pix = blk * 256 + lane;
+  const int c0 = nchunk * g / G, c1 = nchunk * (g + 1) / G;
+  double re = 0.0, im = 0.0, s = 0.0;
   if (pix < npix) {
-    double re = 0.0, im = 0.0, s = 0.0;
-    for (int c = 0; c < nchunk; ++c) {
+#pragma unroll 4
+    for (int c = c0; c < c1; ++c) {
       const double* p = part + ((size_t)c * npix + pix) * 3;
       re += p[0];
       im += p[1];
       s += p[2];
     }
-    gsum[pix] = mkcf((float)re, (float)im);
-    so = (float)s;
-    osum[pix] = so;
   }
-  const float m = block_max(so, sh);
-  if (threadIdx.x == 0) bmax[blk] = m;
+  if (g > 0) {
+    acc[g - 1][0][lane] = re;
+    acc[g - 1][1][lane] = im;
+    acc[g - 1][2][lane] = s;
+  }
+  __syncthreads();
+  if (g == 0) {
+    for (int h = 0; h < G - 1; ++h) {
+      re += acc[h][0][lane];
+      im += acc[h][1][lane];
+      s += acc[h][2][lane];
+    }
+    float so = 0.f;
+    if (pix < npix) {
+      gsum[pix] = mkcf((float)re, (float)im);
+      so = (float)s;
+      osum[pix] = so;
+    }
+    sh[lane] = so;
+  }
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[blk] = sh[0];
 }
 
-__global__ __launch_bounds__(256) void k_probe_reduce(const double* __restrict__ part, int nchunk,
+__global__ __launch_bounds__(PROBE_REDUCE_THREADS) void k_probe_reduce(const double* __restrict__ part, int nchunk,
                                                       int npix, cf* __restrict__ gsum,
                                                       float* __restrict__ osum,
                                                       float* __restrict__ bmax) {
@@ -1107,7 +1150,7 @@ int launch_probe_apply(cf* probe, double* ws, int S, int B, float step, float nm
   float* osum = (float*)((char*)gsum + align256(S2 * 8));
   float* bmax = (float*)((char*)osum + align256(S2 * 4));
   const int npix = (int)S2, nblk = (npix + 255) / 256;
-  hipLaunchKernelGGL(k_probe_reduce, dim3(nblk), dim3(256), 0, stream, part, probe_nchunk(B), npix, gsum, osum,
+  hipLaunchKernelGGL(k_probe_reduce, dim3(nblk), dim3(PROBE_REDUCE_THREADS), 0, stream, part, probe_nchunk(B), npix, gsum, osum,
                      bmax);
   int rc = check_launch("k_probe_reduce");
   if (rc) return rc;
