@@ -110,6 +110,14 @@ int pty_object_constraints(void* object, int Px, int Py, const unsigned char* ma
                            float shrinkage_rad, int pure_phase_object, void* partials,
                            pty_stream_t stream);
 
+/* Gaussian smoothing of a complex object (ptychographic_constraints.py:
+ * 119-151): scipy.ndimage.gaussian_filter along rows then columns with
+ * reflect boundaries, float64 sums, float32 after each axis; the phase only
+ * when pure_phase_object.  `weights`: device float64 [2 * radius + 1] (the
+ * scipy kernel for sigma in pixels); `scratch`: Px * Py * 8 bytes. */
+int pty_object_gaussian(void* object, int Px, int Py, const double* weights, int radius,
+                        int pure_phase_object, void* scratch, pty_stream_t stream);
+
 /* Probe centre-of-mass recentring (ptychographic_constraints.py:894-918). */
 int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream);
 
@@ -196,6 +204,11 @@ typedef struct pty_gd_params {
   int constrain_probe_amplitude;  /* real-space erf top-hat (:920-957) */
   double probe_amplitude_radius;  /* constrain_probe_amplitude_relative_radius */
   double probe_amplitude_width;   /* constrain_probe_amplitude_relative_width */
+  /* object smoothing before the shrinkage and threshold
+   * (ptychographic_constraints.py:119-151, 431-435): scipy.ndimage
+   * gaussian_filter weights (float64, device, 2 * radius + 1 values), or NULL */
+  const double* gaussian_weights;
+  int gaussian_radius;
 } pty_gd_params;
 
 size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);

@@ -938,9 +938,18 @@ class OracleSingleslicePtychography:
 
     # -- constraints --------------------------------------------------------
     @staticmethod
-    def object_constraints(obj, object_mask, shrinkage_rad=0.0, pure_phase_object=False):
-        """ptychographic_constraints.py:29-97, 411-464 (complex object,
-        no smoothing filters)."""
+    def object_constraints(obj, object_mask, shrinkage_rad=0.0, pure_phase_object=False,
+                           gaussian_sigma_px=None):
+        """ptychographic_constraints.py:29-97, 411-464 (complex object); the
+        gaussian smoothing of :119-151 when gaussian_sigma_px is given (the
+        reference's scipy.ndimage.gaussian_filter call with its defaults)."""
+        if gaussian_sigma_px is not None:
+            import scipy.ndimage
+            if not pure_phase_object:
+                obj = scipy.ndimage.gaussian_filter(obj, gaussian_sigma_px)
+            else:
+                phase = scipy.ndimage.gaussian_filter(np.angle(obj), gaussian_sigma_px)
+                obj = np.exp(1.0j * phase)
         if shrinkage_rad > 0.0 or object_mask is not None:
             phase = np.angle(obj)
             amp = np.abs(obj)
@@ -1017,9 +1026,13 @@ class OracleSingleslicePtychography:
                     detector_fourier_mask=None, fix_probe_aperture=False,
                     constrain_probe_amplitude=False,
                     constrain_probe_amplitude_relative_radius=0.5,
-                    constrain_probe_amplitude_relative_width=0.05, **kwargs):
+                    constrain_probe_amplitude_relative_width=0.05, gaussian_filter_sigma=None,
+                    gaussian_filter=True, **kwargs):
         """phase/singleslice_ptychography.py:606-996 (gradient descent and the
         projection sets)."""
+        # ptychographic_constraints.py:142, flag as singleslice_ptychography.py:959
+        gaussian_sigma_px = (float(gaussian_filter_sigma) / float(self.sampling[0])
+                             if gaussian_filter and gaussian_filter_sigma is not None else None)
         use_projection, pa, pb, pc = self.method_parameters(
             reconstruction_method, reconstruction_parameter, reconstruction_parameter_a,
             reconstruction_parameter_b, reconstruction_parameter_c)
@@ -1092,7 +1105,8 @@ class OracleSingleslicePtychography:
                 error += err
             error /= self._mean_diffraction_intensity * N
             self._object = self.object_constraints(self._object, object_mask,
-                                                   shrinkage_rad, pure_phase_object)
+                                                   shrinkage_rad, pure_phase_object,
+                                                   gaussian_sigma_px)
             if fix_probe_com and not fix_probe:
                 self._probe = self.probe_com_constraint(self._probe)
             # ptychographic_constraints.py:1091-1147 (single probe) and

@@ -34,7 +34,8 @@ class PtyGdParams(ctypes.Structure):
                 ("shrinkage_rad", c_float), ("stage", c_int),
                 ("fix_probe_aperture", c_int), ("probe_aperture", c_void_p),
                 ("constrain_probe_amplitude", c_int), ("probe_amplitude_radius", ctypes.c_double),
-                ("probe_amplitude_width", ctypes.c_double)]
+                ("probe_amplitude_width", ctypes.c_double),
+                ("gaussian_weights", c_void_p), ("gaussian_radius", c_int)]
 
 
 class PtyPositionParams(ctypes.Structure):
@@ -80,6 +81,8 @@ _SIGNATURES = {
     "pty_object_constraints": (c_int, [c_void_p, c_int, c_int, c_void_p, c_float, c_int,
                                        c_void_p, c_void_p]),
     "pty_probe_com_constraint": (c_int, [c_void_p, c_int, c_void_p]),
+    "pty_object_gaussian": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_void_p,
+                                    c_void_p]),
     "pty_loss_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
     "pty_loss_gradients": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p,
                                    c_void_p, c_int, c_int, c_int, ctypes.c_double, c_void_p,

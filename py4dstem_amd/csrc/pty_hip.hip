@@ -1171,6 +1171,105 @@ int launch_probe_update(cf* probe, const cf* chi, const cf* obj, int Px, int Py,
   return launch_probe_apply(probe, part, S, B, step, nmin, stream);
 }
 
+// ---------------------------------------------------------------------------
+// gaussian smoothing of the object (ptychographic_constraints.py:119-151,
+// which calls scipy.ndimage.gaussian_filter with its defaults: reflect
+// boundaries, truncate 4).  One pass per axis, rows (axis 0) first; each
+// output is scipy's correlate1d sum for a symmetric kernel in float64,
+// w[r] x[i] + sum over j = r .. 1 of (x[i-j] + x[i+j]) w[r-j], rounded to
+// float32 after the axis.  The weights are computed on the host as scipy
+// computes them.  Pure-phase objects smooth the phase and rebuild exp(i phi).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int reflect_index(int i, int n) {  // half-sample symmetric, period 2n
+  const int p = 2 * n;
+  int m = i % p;
+  m += m < 0 ? p : 0;
+  return m < n ? m : p - 1 - m;
+}
+
+template <class T> struct Lanes;
+template <> struct Lanes<float> {
+  static constexpr int N = 1;
+  static __device__ double get(float v, int) { return (double)v; }
+  static __device__ void set(float& v, int, double d) { v = (float)d; }
+};
+template <> struct Lanes<cf> {
+  static constexpr int N = 2;
+  static __device__ double get(cf v, int c) { return (double)(c ? v.y : v.x); }
+  static __device__ void set(cf& v, int c, double d) { (c ? v.y : v.x) = (float)d; }
+};
+
+template <class T, int AXIS>
+__global__ __launch_bounds__(256) void k_gaussian_axis(const T* __restrict__ in, T* __restrict__ out, int Px,
+                                                       int Py, const double* __restrict__ w, int r) {
+#pragma clang fp contract(off)
+  const long long npix = (long long)Px * Py;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  const int x = (int)(p / Py), y = (int)(p % Py);
+  const int n = AXIS == 0 ? Px : Py;
+  const int i = AXIS == 0 ? x : y;
+  const long long base = AXIS == 0 ? y : (long long)x * Py;
+  const long long step = AXIS == 0 ? Py : 1;
+  T res;
+#pragma unroll
+  for (int c = 0; c < Lanes<T>::N; ++c) {
+    double acc = Lanes<T>::get(in[base + (long long)i * step], c) * w[r];
+    for (int j = r; j >= 1; --j) {
+      const double lo = Lanes<T>::get(in[base + (long long)reflect_index(i - j, n) * step], c);
+      const double hi = Lanes<T>::get(in[base + (long long)reflect_index(i + j, n) * step], c);
+      acc += (lo + hi) * w[r - j];
+    }
+    Lanes<T>::set(res, c, acc);
+  }
+  out[p] = res;
+}
+
+__global__ __launch_bounds__(256) void k_object_phase(const cf* __restrict__ obj, float* __restrict__ phase,
+                                                      long long npix) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < npix) phase[p] = atan2f(obj[p].y, obj[p].x);
+}
+
+__global__ __launch_bounds__(256) void k_object_expi(const float* __restrict__ phase, cf* __restrict__ obj,
+                                                     long long npix) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < npix) {
+    float s, c;
+    sincosf(phase[p], &s, &c);
+    obj[p] = mkcf(c, s);
+  }
+}
+
+// tmp: Px * Py * 8 bytes
+int launch_object_gaussian(cf* obj, int Px, int Py, const double* w, int r, int pure, void* tmp,
+                           hipStream_t stream) {
+  const long long npix = (long long)Px * Py;
+  const dim3 grid(grid_for(npix, 256)), blk(256);
+  if (pure) {
+    float* ph = (float*)tmp;
+    float* ph2 = ph + npix;
+    hipLaunchKernelGGL(k_object_phase, grid, blk, 0, stream, obj, ph, npix);
+    hipLaunchKernelGGL((k_gaussian_axis<float, 0>), grid, blk, 0, stream, ph, ph2, Px, Py, w, r);
+    hipLaunchKernelGGL((k_gaussian_axis<float, 1>), grid, blk, 0, stream, ph2, ph, Px, Py, w, r);
+    hipLaunchKernelGGL(k_object_expi, grid, blk, 0, stream, ph, obj, npix);
+  } else {
+    cf* t = (cf*)tmp;
+    hipLaunchKernelGGL((k_gaussian_axis<cf, 0>), grid, blk, 0, stream, obj, t, Px, Py, w, r);
+    hipLaunchKernelGGL((k_gaussian_axis<cf, 1>), grid, blk, 0, stream, t, obj, Px, Py, w, r);
+  }
+  return check_launch("k_gaussian_axis");
+}
+
+// smoothing filters ahead of the shrinkage and threshold
+// (ptychographic_constraints.py:431-446); tmp: Px * Py * 8 bytes
+int launch_object_filters(cf* obj, int Px, int Py, const pty_gd_params& q, void* tmp, hipStream_t stream) {
+  if (q.gaussian_weights && q.gaussian_radius >= 0)
+    return launch_object_gaussian(obj, Px, Py, q.gaussian_weights, q.gaussian_radius, q.pure_phase_object, tmp,
+                                  stream);
+  return PTY_OK;
+}
+
 int launch_constraints(cf* obj, int Px, int Py, const unsigned char* mask, float shrink, int pure,
                        double* part, hipStream_t stream) {
   const long long npix = (long long)Px * Py;
@@ -1412,6 +1511,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
     }
     hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
     if ((rc = check_launch("k_error_reduce"))) return rc;
+    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, stream))) return rc;
     if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
                                  stream)))
       return rc;
@@ -1509,6 +1609,7 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
     }
     hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
     if ((rc = check_launch("k_error_reduce"))) return rc;
+    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, stream))) return rc;
     if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
                                  stream)))
       return rc;
@@ -1636,6 +1737,7 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
     }
     hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
     if ((rc = check_launch("k_error_reduce"))) return rc;
+    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, stream))) return rc;
     if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
                                  stream)))
       return rc;
@@ -1740,6 +1842,13 @@ int pty_object_constraints(void* object, int Px, int Py, const unsigned char* ma
   if (!object || Px <= 0 || Py <= 0 || (mask && !partials)) return fail(PTY_E_ARG, "bad constraint arguments");
   return launch_constraints((cf*)object, Px, Py, mask, shrinkage_rad, pure_phase_object, (double*)partials,
                             st(stream));
+}
+
+int pty_object_gaussian(void* object, int Px, int Py, const double* weights, int radius, int pure_phase_object,
+                        void* scratch, pty_stream_t stream) {
+  if (!object || !weights || !scratch || Px <= 0 || Py <= 0 || radius < 0)
+    return fail(PTY_E_ARG, "bad gaussian filter arguments");
+  return launch_object_gaussian((cf*)object, Px, Py, weights, radius, pure_phase_object, scratch, st(stream));
 }
 
 int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream) {

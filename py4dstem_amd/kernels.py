@@ -135,6 +135,32 @@ def probe_update(probe, chi, obj, positions, order, step_size, normalization_min
     return probe
 
 
+def gaussian_kernel1d(sigma, truncate=4.0):
+    """The float64 kernel scipy.ndimage.gaussian_filter uses for one axis
+    (radius int(truncate * sigma + 0.5), exp(-x^2 / (2 sigma^2)) normalised to
+    unit sum), evaluated with NumPy as scipy does; returns (weights, radius)."""
+    sigma = float(sigma)
+    radius = int(truncate * sigma + 0.5)
+    x = np.arange(-radius, radius + 1)
+    phi = np.exp(-0.5 / (sigma * sigma) * x ** 2)
+    return phi / phi.sum(), radius
+
+
+def object_gaussian(obj, sigma_px, pure_phase_object=False):
+    """In-place gaussian smoothing of a complex64 object (sigma in pixels), as
+    scipy.ndimage.gaussian_filter(obj, sigma_px) (or of its phase)."""
+    Px, Py = obj.shape
+    _req(obj, torch.complex64, name="object")
+    if not float(sigma_px) > 1e-15:  # scipy leaves such axes unfiltered
+        return obj
+    w, r = gaussian_kernel1d(sigma_px)
+    wd = torch.from_numpy(w).to(obj.device)
+    scratch = torch.empty(Px * Py * 8, dtype=torch.uint8, device=obj.device)
+    _lib.call("pty_object_gaussian", ptr(obj), Px, Py, ptr(wd), r, int(bool(pure_phase_object)),
+              ptr(scratch), stream_handle())
+    return obj
+
+
 def object_constraints(obj, mask_inverse=None, shrinkage_rad=0.0, pure_phase_object=False):
     Px, Py = obj.shape
     part = torch.empty(_lib.load().pty_constraint_partials_bytes(), dtype=torch.uint8,

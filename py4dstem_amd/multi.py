@@ -54,8 +54,10 @@ class MultiPtychography:
                 raise NotImplementedError("position correction (fix_positions=False) runs through "
                                           "SingleslicePtychography.reconstruct(), not the batched "
                                           "runner")
-            if run["params"].fix_probe_aperture or run["params"].constrain_probe_amplitude:
-                raise NotImplementedError("probe aperture/amplitude constraints run through "
+            if (run["params"].fix_probe_aperture or run["params"].constrain_probe_amplitude
+                    or run["params"].gaussian_weights):
+                raise NotImplementedError("probe aperture/amplitude constraints and object "
+                                          "smoothing run through "
                                           "SingleslicePtychography.reconstruct(), not the batched "
                                           "runner")
             runs.append(run)

@@ -608,7 +608,6 @@ class SingleslicePtychography:
             if detector_fourier_mask is not None:  # ptychographic_methods.py:1686-1687
                 raise NotImplementedError()
         unsupported = {
-            "gaussian_filter_sigma": gaussian_filter and gaussian_filter_sigma is not None,
             "q_lowpass/q_highpass": butterworth_filter and (q_lowpass is not None
                                                             or q_highpass is not None),
             "tv_denoise_weight": tv_denoise and tv_denoise_weight is not None,
@@ -702,6 +701,15 @@ class SingleslicePtychography:
             params.constrain_probe_amplitude = 1
             params.probe_amplitude_radius = float(constrain_probe_amplitude_relative_radius)
             params.probe_amplitude_width = float(constrain_probe_amplitude_relative_width)
+        gauss_dev = None
+        if gaussian_filter and gaussian_filter_sigma is not None:
+            # ptychographic_constraints.py:142: sigma from Angstrom to pixels
+            sigma_px = float(gaussian_filter_sigma) / float(self.sampling[0])
+            if sigma_px > 1e-15:
+                w, r = K.gaussian_kernel1d(sigma_px)
+                gauss_dev = torch.from_numpy(w).to(dev)
+                params.gaussian_weights = gauss_dev.data_ptr()
+                params.gaussian_radius = r
         pos_params = singular = proj_params = None
         if projection is not None:
             if getattr(self, "_exit_waves", None) is None:
@@ -736,7 +744,7 @@ class SingleslicePtychography:
         errors_dev = torch.zeros(max(num_iter, 1), dtype=torch.float64, device=dev)
         self._last_run = dict(problem=problem, params=params, workspace=ws, orders=orders_dev,
                               errors=errors_dev, mask=mask_dev, fourier_mask=fmask_dev,
-                              probe_aperture=aperture_dev,
+                              probe_aperture=aperture_dev, gaussian_weights=gauss_dev,
                               num_iter=num_iter,
                               per_call=per_call, store_iterations=store_iterations,
                               progress_bar=progress_bar, position_params=pos_params,

@@ -1,0 +1,96 @@
+"""GPU object smoothing (gaussian_filter_sigma, ptychographic_constraints.py:
+119-151) against scipy.ndimage.gaussian_filter and the oracle.
+
+Tolerances:
+  * the smoothing kernel on a complex64 array: bitwise equal to scipy (the
+    same float64 sums in the same order, float32 after each axis);
+  * pure-phase smoothing: <= 2e-6 relative (float32 atan2 and sincos);
+  * whole reconstructions with the constraint: object, probe and every
+    per-iteration error <= 1e-4.
+"""
+
+import numpy as np
+import pytest
+import scipy.ndimage
+import scipy.ndimage as nd
+
+from oracle import OracleMixedstatePtychography, OracleSingleslicePtychography
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    from py4dstem_amd import _lib
+    _lib.load()
+    return torch.device("cuda", 0)
+
+
+@pytest.mark.parametrize("shape,sigma", [((200, 173), 1.7), ((64, 64), 0.6), ((24, 40), 7.0)])
+def test_gaussian_kernel_bitwise_scipy(dev, shape, sigma):
+    from py4dstem_amd import kernels as K
+    say(f"gaussian smoothing {shape} sigma {sigma}")
+    rng = np.random.default_rng(3)
+    z = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    t = torch.from_numpy(z.copy()).to(dev)
+    K.object_gaussian(t, sigma)
+    assert np.array_equal(t.cpu().numpy(), nd.gaussian_filter(z, sigma))
+
+
+def test_gaussian_pure_phase(dev):
+    from py4dstem_amd import kernels as K
+    say("gaussian smoothing of the phase")
+    rng = np.random.default_rng(4)
+    z = (np.exp(1j * rng.uniform(-3, 3, (150, 190))) * rng.uniform(0.5, 1, (150, 190))
+         ).astype(np.complex64)
+    t = torch.from_numpy(z.copy()).to(dev)
+    K.object_gaussian(t, 2.3, pure_phase_object=True)
+    ref = OracleSingleslicePtychography.object_constraints(z.copy(), None, 0.0, True, 2.3)
+    assert rel(t.cpu().numpy(), ref) <= 2e-6
+
+
+def _init_object(shape, seed):
+    rng = np.random.default_rng(seed)
+    return np.exp(0.02j * scipy.ndimage.gaussian_filter(rng.standard_normal(shape), 3)
+                  / 0.05).astype(np.complex64)
+
+
+@pytest.mark.parametrize("mixed,pure", [(False, False), (False, True), (True, False)])
+def test_reconstruct_with_gaussian_filter_matches_oracle(dev, mixed, pure):
+    from py4dstem_amd import MixedstatePtychography
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say(f"reconstruct with gaussian_filter_sigma, mixed={mixed}, pure_phase={pure}")
+    w = small_workload(seed=1)
+    recon = dict(w["reconstruct"], max_batch_size=256, fix_probe=False, num_iter=4,
+                 gaussian_filter_sigma=0.3, pure_phase_object=pure)
+    if mixed:
+        recon["orthogonalize_probe"] = False
+    init_obj = _init_object((256, 256), 17)
+    out = {}
+    pairs = ((("gpu", MixedstatePtychography), ("oracle", OracleMixedstatePtychography)) if mixed
+             else (("gpu", SingleslicePtychography), ("oracle", OracleSingleslicePtychography)))
+    for kind, cls in pairs:
+        kw = dict(w["init"], initial_object_guess=init_obj.copy())
+        if mixed:
+            kw["num_probes"] = 2
+        p = cls(datacube=w["datacube"], **kw)
+        np.random.seed(5)
+        p.preprocess(**w["preprocess"])
+        np.random.seed(1234)
+        out[kind] = p.reconstruct(**recon)
+    g, o = out["gpu"], out["oracle"]
+    err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(np.asarray(g.probe), o.probe):.2e} "
+        f"error {err:.2e}")
+    assert rel(g.object, o.object) <= 1e-4
+    assert rel(np.asarray(g.probe), o.probe) <= 1e-4
+    assert err <= 1e-4

@@ -790,3 +790,48 @@ def test_probe_aperture_and_amplitude_constraints():
     c = ratio[0, 0]  # the normalisation factor (mask 1 at r = 0)
     np.testing.assert_allclose(ratio[8, 0], 0.5 * c, rtol=1e-4)  # r = 8/32 = 0.25
     assert ratio[0, 12] < 1e-6 * c and abs(ratio[2, 2] / c - 1) < 1e-6
+
+
+def test_gaussian_smoothing_order_and_closed_forms():
+    """The object smoothing goes through scipy.ndimage.gaussian_filter, as the
+    reference's (ptychographic_constraints.py:119-151).  Pinned here: the
+    float64 sum the GPU kernel evaluates (centre tap, then symmetric pairs
+    from the outside in, reflect boundaries, float32 after each axis)
+    reproduces scipy bit for bit, also when the radius exceeds the axis; a
+    delta returns the kernel weights; a constant stays constant."""
+    import scipy.ndimage as nd
+    from py4dstem_amd.kernels import gaussian_kernel1d
+
+    def refl(i, n):
+        p = 2 * n
+        m = i % p
+        return m if m < n else p - 1 - m
+
+    def axis_pass(a, w, r, axis):
+        a = np.moveaxis(a, axis, 0).astype(np.float64)
+        n = a.shape[0]
+        out = a * w[r]
+        for j in range(r, 0, -1):
+            lo = [refl(i - j, n) for i in range(n)]
+            hi = [refl(i + j, n) for i in range(n)]
+            out = out + (a[lo] + a[hi]) * w[r - j]
+        return np.moveaxis(out.astype(np.float32), 0, axis)
+
+    rng = np.random.default_rng(0)
+    for shape, sigma in [((37, 53), 1.5), ((64, 40), 0.7), ((9, 11), 3.3)]:
+        z = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+        w, r = gaussian_kernel1d(sigma)
+        assert r == int(4.0 * sigma + 0.5)
+        mine = (axis_pass(axis_pass(z.real, w, r, 0), w, r, 1)
+                + 1j * axis_pass(axis_pass(z.imag, w, r, 0), w, r, 1)).astype(np.complex64)
+        assert np.array_equal(mine, nd.gaussian_filter(z, sigma))
+    w, r = gaussian_kernel1d(2.2)
+    delta = np.zeros(64)
+    delta[32] = 1.0
+    np.testing.assert_array_equal(nd.gaussian_filter1d(delta, 2.2)[32 - r:33 + r], w)
+    assert abs(w.sum() - 1) < 1e-15 and np.array_equal(w, w[::-1])
+    c = np.full((20, 30), 0.7 - 0.2j, np.complex64)
+    np.testing.assert_allclose(nd.gaussian_filter(c, 1.9), c, rtol=1e-6)
+    o = po.OracleSingleslicePtychography.object_constraints(
+        np.exp(1j * rng.standard_normal((16, 16))).astype(np.complex64), None, 0.0, True, 1.2)
+    np.testing.assert_allclose(np.abs(o), 1, rtol=1e-6)  # pure phase stays unit modulus
