@@ -118,6 +118,15 @@ int pty_object_constraints(void* object, int Px, int Py, const unsigned char* ma
 int pty_object_gaussian(void* object, int Px, int Py, const double* weights, int radius,
                         int pure_phase_object, void* scratch, pty_stream_t stream);
 
+/* Butterworth filter of a complex object (ptychographic_constraints.py:
+ * 153-201): mean removed, fft2 (any Px, Py: DFT products along rows and
+ * columns), times the low/high-pass envelope on the fftfreq grid, ifft2,
+ * mean restored.  `scratch`: Px * Py * 8 bytes; `partials`:
+ * pty_constraint_partials_bytes(). */
+int pty_object_butterworth(void* object, int Px, int Py, double q_lowpass, double q_highpass,
+                           double butterworth_order, double sampling_x, double sampling_y,
+                           void* scratch, void* partials, pty_stream_t stream);
+
 /* Probe centre-of-mass recentring (ptychographic_constraints.py:894-918). */
 int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream);
 
@@ -209,6 +218,14 @@ typedef struct pty_gd_params {
    * gaussian_filter weights (float64, device, 2 * radius + 1 values), or NULL */
   const double* gaussian_weights;
   int gaussian_radius;
+  /* Butterworth filter of the object after the smoothing
+   * (ptychographic_constraints.py:153-201, 436-442): 0 = off.  A cut-off of
+   * 0 leaves its factor out, as the reference's `if q_lowpass:` does. */
+  int butterworth;
+  double q_lowpass;          /* A^-1 */
+  double q_highpass;         /* A^-1 */
+  double butterworth_order;
+  double sampling[2];        /* object pixel size (A) along axes 0 and 1 */
 } pty_gd_params;
 
 size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);

@@ -938,8 +938,27 @@ class OracleSingleslicePtychography:
 
     # -- constraints --------------------------------------------------------
     @staticmethod
+    def object_butterworth_constraint(obj, q_lowpass, q_highpass, butterworth_order, sampling):
+        """ptychographic_constraints.py:153-201 (complex object): NumPy dtypes
+        as the reference's, so the float64 envelope promotes the object to
+        complex128 from here on."""
+        qx = np.fft.fftfreq(obj.shape[-2], sampling[0])
+        qy = np.fft.fftfreq(obj.shape[-1], sampling[1])
+        qya, qxa = np.meshgrid(qy, qx)
+        qra = np.sqrt(qxa ** 2 + qya ** 2)
+        env = np.ones_like(qra)
+        if q_highpass:
+            env *= 1 - 1 / (1 + (qra / q_highpass) ** (2 * butterworth_order))
+        if q_lowpass:
+            env *= 1 / (1 + (qra / q_lowpass) ** (2 * butterworth_order))
+        mean = np.mean(obj, axis=(-2, -1), keepdims=True)
+        obj = obj - mean
+        obj = np.fft.ifft2(np.fft.fft2(obj) * env)
+        return obj + mean
+
+    @staticmethod
     def object_constraints(obj, object_mask, shrinkage_rad=0.0, pure_phase_object=False,
-                           gaussian_sigma_px=None):
+                           gaussian_sigma_px=None, butterworth=None):
         """ptychographic_constraints.py:29-97, 411-464 (complex object); the
         gaussian smoothing of :119-151 when gaussian_sigma_px is given (the
         reference's scipy.ndimage.gaussian_filter call with its defaults)."""
@@ -950,6 +969,8 @@ class OracleSingleslicePtychography:
             else:
                 phase = scipy.ndimage.gaussian_filter(np.angle(obj), gaussian_sigma_px)
                 obj = np.exp(1.0j * phase)
+        if butterworth is not None:  # (q_lowpass, q_highpass, order, sampling)
+            obj = OracleSingleslicePtychography.object_butterworth_constraint(obj, *butterworth)
         if shrinkage_rad > 0.0 or object_mask is not None:
             phase = np.angle(obj)
             amp = np.abs(obj)
@@ -1027,12 +1048,16 @@ class OracleSingleslicePtychography:
                     constrain_probe_amplitude=False,
                     constrain_probe_amplitude_relative_radius=0.5,
                     constrain_probe_amplitude_relative_width=0.05, gaussian_filter_sigma=None,
-                    gaussian_filter=True, **kwargs):
+                    gaussian_filter=True, butterworth_filter=True, q_lowpass=None, q_highpass=None,
+                    butterworth_order=2, **kwargs):
         """phase/singleslice_ptychography.py:606-996 (gradient descent and the
         projection sets)."""
         # ptychographic_constraints.py:142, flag as singleslice_ptychography.py:959
         gaussian_sigma_px = (float(gaussian_filter_sigma) / float(self.sampling[0])
                              if gaussian_filter and gaussian_filter_sigma is not None else None)
+        butterworth = ((q_lowpass, q_highpass, butterworth_order, self.sampling)
+                       if butterworth_filter and (q_lowpass is not None or q_highpass is not None)
+                       else None)
         use_projection, pa, pb, pc = self.method_parameters(
             reconstruction_method, reconstruction_parameter, reconstruction_parameter_a,
             reconstruction_parameter_b, reconstruction_parameter_c)
@@ -1106,7 +1131,7 @@ class OracleSingleslicePtychography:
             error /= self._mean_diffraction_intensity * N
             self._object = self.object_constraints(self._object, object_mask,
                                                    shrinkage_rad, pure_phase_object,
-                                                   gaussian_sigma_px)
+                                                   gaussian_sigma_px, butterworth)
             if fix_probe_com and not fix_probe:
                 self._probe = self.probe_com_constraint(self._probe)
             # ptychographic_constraints.py:1091-1147 (single probe) and

@@ -35,7 +35,10 @@ class PtyGdParams(ctypes.Structure):
                 ("fix_probe_aperture", c_int), ("probe_aperture", c_void_p),
                 ("constrain_probe_amplitude", c_int), ("probe_amplitude_radius", ctypes.c_double),
                 ("probe_amplitude_width", ctypes.c_double),
-                ("gaussian_weights", c_void_p), ("gaussian_radius", c_int)]
+                ("gaussian_weights", c_void_p), ("gaussian_radius", c_int),
+                ("butterworth", c_int), ("q_lowpass", ctypes.c_double),
+                ("q_highpass", ctypes.c_double), ("butterworth_order", ctypes.c_double),
+                ("sampling", ctypes.c_double * 2)]
 
 
 class PtyPositionParams(ctypes.Structure):
@@ -83,6 +86,9 @@ _SIGNATURES = {
     "pty_probe_com_constraint": (c_int, [c_void_p, c_int, c_void_p]),
     "pty_object_gaussian": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_void_p,
                                     c_void_p]),
+    "pty_object_butterworth": (c_int, [c_void_p, c_int, c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       c_void_p, c_void_p, c_void_p]),
     "pty_loss_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
     "pty_loss_gradients": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p,
                                    c_void_p, c_int, c_int, c_int, ctypes.c_double, c_void_p,

@@ -1261,13 +1261,178 @@ int launch_object_gaussian(cf* obj, int Px, int Py, const double* w, int r, int 
   return check_launch("k_gaussian_axis");
 }
 
+// ---------------------------------------------------------------------------
+// Butterworth filter of the object (ptychographic_constraints.py:153-201):
+// O - mean(O), fft2, times env(|q|) on the fftfreq grid, ifft2, + mean.  The
+// object is any size, so each 2-D transform is two complex products with the
+// DFT matrices (rows, then columns), W[a][b] = exp(-+2 pi i ((a b) mod n) / n)
+// generated in float64 while the tiles are loaded; float32 products, float32
+// accumulation over n terms.
+// ---------------------------------------------------------------------------
+constexpr int DFT_T = 32;  // output tile DFT_T x DFT_T, k-step DFT_T
+
+__device__ __forceinline__ cf dft_twiddle(int a, int b, int n, int inverse) {
+  const int m = (int)(((long long)a * b) % n);
+  double s, c;
+  sincospi((inverse ? 2.0 : -2.0) * (double)m / (double)n, &s, &c);
+  return mkcf((float)c, (float)s);
+}
+
+// LEFT:  Y[i][j] = sum_k W_M[i][k] X[k][j]   (transform along axis 0)
+// RIGHT: Y[i][j] = sum_k X[i][k] W_N[k][j]   (transform along axis 1)
+template <bool LEFT>
+__global__ __launch_bounds__(256) void k_dft(const cf* __restrict__ X, cf* __restrict__ Y, int M, int N,
+                                             int inverse) {
+  __shared__ cf As[DFT_T][DFT_T + 1];
+  __shared__ cf Bs[DFT_T][DFT_T + 1];
+  const int K = LEFT ? M : N;
+  const int i0 = blockIdx.y * DFT_T, j0 = blockIdx.x * DFT_T;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float ar[2][2] = {{0.f, 0.f}, {0.f, 0.f}}, ai[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  for (int k0 = 0; k0 < K; k0 += DFT_T) {
+    for (int e = threadIdx.x; e < DFT_T * DFT_T; e += 256) {
+      const int r = e / DFT_T, c = e % DFT_T;
+      const int gi = i0 + r, gk = k0 + c;
+      cf v = mkcf(0.f, 0.f);
+      if (gi < M && gk < K) v = LEFT ? dft_twiddle(gi, gk, K, inverse) : X[(size_t)gi * N + gk];
+      As[r][c] = v;
+      const int hk = k0 + r, gj = j0 + c;
+      cf u = mkcf(0.f, 0.f);
+      if (hk < K && gj < N) u = LEFT ? X[(size_t)hk * N + gj] : dft_twiddle(hk, gj, K, inverse);
+      Bs[r][c] = u;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int k = 0; k < DFT_T; ++k) {
+      const cf a0 = As[2 * ty][k], a1 = As[2 * ty + 1][k];
+      const cf b0 = Bs[k][2 * tx], b1 = Bs[k][2 * tx + 1];
+      const cf a[2] = {a0, a1}, b[2] = {b0, b1};
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          ar[u][v] += a[u].x * b[v].x - a[u].y * b[v].y;
+          ai[u][v] += a[u].x * b[v].y + a[u].y * b[v].x;
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int gi = i0 + 2 * ty + u, gj = j0 + 2 * tx + v;
+      if (gi < M && gj < N) Y[(size_t)gi * N + gj] = mkcf(ar[u][v], ai[u][v]);
+    }
+}
+
+// float64 partial sums of the object (re, im) per block, fixed grid
+constexpr int MEAN_BLOCKS = 256;
+static_assert(MEAN_BLOCKS <= CONSTRAINT_BLOCKS, "mean partials share the constraint partial buffer");
+__global__ __launch_bounds__(256) void k_complex_sum_partial(const cf* __restrict__ obj, long long npix,
+                                                             double* __restrict__ part) {
+  __shared__ double sr[256], si[256];
+  double a = 0.0, b = 0.0;
+  for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < npix; p += (long long)MEAN_BLOCKS * 256) {
+    a += (double)obj[p].x;
+    b += (double)obj[p].y;
+  }
+  sr[threadIdx.x] = a;
+  si[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sr[0];
+    part[2 * blockIdx.x + 1] = si[0];
+  }
+}
+
+__device__ __forceinline__ cf mean_from_partials(const double* part, long long npix) {
+  double a = 0.0, b = 0.0;
+  for (int k = 0; k < MEAN_BLOCKS; ++k) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  return mkcf((float)(a / (double)npix), (float)(b / (double)npix));
+}
+
+// sign = -1: O -= mean; sign = +1: O += mean
+__global__ __launch_bounds__(256) void k_shift_by_mean(cf* __restrict__ obj, long long npix,
+                                                       const double* __restrict__ part, float sign) {
+  __shared__ cf m_sh;
+  if (threadIdx.x == 0) m_sh = mean_from_partials(part, npix);
+  __syncthreads();
+  const cf m = m_sh;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    const cf o = obj[p];
+    obj[p] = mkcf(o.x + sign * m.x, o.y + sign * m.y);
+  }
+}
+
+// spectrum times env(|q|) / (Px Py) (the inverse transform's normalisation),
+// env evaluated in float64 as the reference evaluates it
+__global__ __launch_bounds__(256) void k_butterworth_env(cf* __restrict__ spec, int Px, int Py, double dx,
+                                                         double dy, double q_lowpass, double q_highpass,
+                                                         double order) {
+  const long long npix = (long long)Px * Py;
+  const double vx = 1.0 / (Px * dx), vy = 1.0 / (Py * dy);
+  const double norm = 1.0 / ((double)Px * (double)Py);
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(p / Py), y = (int)(p % Py);
+    const int kx = x < (Px - 1) / 2 + 1 ? x : x - Px;  // numpy fftfreq order
+    const int ky = y < (Py - 1) / 2 + 1 ? y : y - Py;
+    const double qx = kx * vx, qy = ky * vy;
+    const double qr = sqrt(qx * qx + qy * qy);
+    double env = 1.0;
+    if (q_highpass != 0.0) env *= 1.0 - 1.0 / (1.0 + pow(qr / q_highpass, 2.0 * order));
+    if (q_lowpass != 0.0) env *= 1.0 / (1.0 + pow(qr / q_lowpass, 2.0 * order));
+    const double f = env * norm;
+    const cf v = spec[p];
+    spec[p] = mkcf((float)((double)v.x * f), (float)((double)v.y * f));
+  }
+}
+
+// tmp: Px * Py * 8 bytes; part: MEAN_BLOCKS * 16 bytes
+int launch_object_butterworth(cf* obj, int Px, int Py, double q_lowpass, double q_highpass, double order,
+                              double dx, double dy, void* tmp, double* part, hipStream_t stream) {
+  const long long npix = (long long)Px * Py;
+  cf* t = (cf*)tmp;
+  const dim3 g2((Py + DFT_T - 1) / DFT_T, (Px + DFT_T - 1) / DFT_T), b2(256);
+  const dim3 g1(grid_for(npix, 256)), b1(256);
+  hipLaunchKernelGGL(k_complex_sum_partial, dim3(MEAN_BLOCKS), dim3(256), 0, stream, obj, npix, part);
+  hipLaunchKernelGGL(k_shift_by_mean, g1, b1, 0, stream, obj, npix, part, -1.f);
+  hipLaunchKernelGGL(k_dft<true>, g2, b2, 0, stream, obj, t, Px, Py, 0);
+  hipLaunchKernelGGL(k_dft<false>, g2, b2, 0, stream, t, obj, Px, Py, 0);
+  hipLaunchKernelGGL(k_butterworth_env, g1, b1, 0, stream, obj, Px, Py, dx, dy, q_lowpass, q_highpass, order);
+  hipLaunchKernelGGL(k_dft<true>, g2, b2, 0, stream, obj, t, Px, Py, 1);
+  hipLaunchKernelGGL(k_dft<false>, g2, b2, 0, stream, t, obj, Px, Py, 1);
+  hipLaunchKernelGGL(k_shift_by_mean, g1, b1, 0, stream, obj, npix, part, 1.f);
+  return check_launch("k_dft");
+}
+
 // smoothing filters ahead of the shrinkage and threshold
-// (ptychographic_constraints.py:431-446); tmp: Px * Py * 8 bytes
-int launch_object_filters(cf* obj, int Px, int Py, const pty_gd_params& q, void* tmp, hipStream_t stream) {
-  if (q.gaussian_weights && q.gaussian_radius >= 0)
-    return launch_object_gaussian(obj, Px, Py, q.gaussian_weights, q.gaussian_radius, q.pure_phase_object, tmp,
-                                  stream);
-  return PTY_OK;
+// (ptychographic_constraints.py:431-446); tmp: Px * Py * 8 bytes, part:
+// MEAN_BLOCKS * 16 bytes
+int launch_object_filters(cf* obj, int Px, int Py, const pty_gd_params& q, void* tmp, double* part,
+                          hipStream_t stream) {
+  int rc = PTY_OK;
+  if (q.gaussian_weights && q.gaussian_radius >= 0 &&
+      (rc = launch_object_gaussian(obj, Px, Py, q.gaussian_weights, q.gaussian_radius, q.pure_phase_object, tmp,
+                                   stream)))
+    return rc;
+  if (q.butterworth &&
+      (rc = launch_object_butterworth(obj, Px, Py, q.q_lowpass, q.q_highpass, q.butterworth_order,
+                                      q.sampling[0], q.sampling[1], tmp, part, stream)))
+    return rc;
+  return rc;
 }
 
 int launch_constraints(cf* obj, int Px, int Py, const unsigned char* mask, float shrink, int pure,
@@ -1511,7 +1676,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
     }
     hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
     if ((rc = check_launch("k_error_reduce"))) return rc;
-    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, stream))) return rc;
+    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, cpart, stream))) return rc;
     if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
                                  stream)))
       return rc;
@@ -1609,7 +1774,7 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
     }
     hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
     if ((rc = check_launch("k_error_reduce"))) return rc;
-    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, stream))) return rc;
+    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, cpart, stream))) return rc;
     if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
                                  stream)))
       return rc;
@@ -1737,7 +1902,7 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
     }
     hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
     if ((rc = check_launch("k_error_reduce"))) return rc;
-    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, stream))) return rc;
+    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, cpart, stream))) return rc;
     if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
                                  stream)))
       return rc;
@@ -1849,6 +2014,15 @@ int pty_object_gaussian(void* object, int Px, int Py, const double* weights, int
   if (!object || !weights || !scratch || Px <= 0 || Py <= 0 || radius < 0)
     return fail(PTY_E_ARG, "bad gaussian filter arguments");
   return launch_object_gaussian((cf*)object, Px, Py, weights, radius, pure_phase_object, scratch, st(stream));
+}
+
+int pty_object_butterworth(void* object, int Px, int Py, double q_lowpass, double q_highpass,
+                           double butterworth_order, double sampling_x, double sampling_y, void* scratch,
+                           void* partials, pty_stream_t stream) {
+  if (!object || !scratch || !partials || Px <= 0 || Py <= 0 || !(sampling_x > 0) || !(sampling_y > 0))
+    return fail(PTY_E_ARG, "bad butterworth filter arguments");
+  return launch_object_butterworth((cf*)object, Px, Py, q_lowpass, q_highpass, butterworth_order, sampling_x,
+                                   sampling_y, scratch, (double*)partials, st(stream));
 }
 
 int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream) {

@@ -161,6 +161,21 @@ def object_gaussian(obj, sigma_px, pure_phase_object=False):
     return obj
 
 
+def object_butterworth(obj, q_lowpass, q_highpass, butterworth_order, sampling):
+    """In-place Butterworth filter of a complex64 object (cut-offs in 1/A,
+    None or 0 = that factor off), as the reference's
+    _object_butterworth_constraint."""
+    Px, Py = obj.shape
+    _req(obj, torch.complex64, name="object")
+    scratch = torch.empty(Px * Py * 8, dtype=torch.uint8, device=obj.device)
+    part = torch.empty(_lib.load().pty_constraint_partials_bytes(), dtype=torch.uint8,
+                       device=obj.device)
+    _lib.call("pty_object_butterworth", ptr(obj), Px, Py, float(q_lowpass or 0.0),
+              float(q_highpass or 0.0), float(butterworth_order), float(sampling[0]),
+              float(sampling[1]), ptr(scratch), ptr(part), stream_handle())
+    return obj
+
+
 def object_constraints(obj, mask_inverse=None, shrinkage_rad=0.0, pure_phase_object=False):
     Px, Py = obj.shape
     part = torch.empty(_lib.load().pty_constraint_partials_bytes(), dtype=torch.uint8,

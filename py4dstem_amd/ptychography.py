@@ -608,8 +608,6 @@ class SingleslicePtychography:
             if detector_fourier_mask is not None:  # ptychographic_methods.py:1686-1687
                 raise NotImplementedError()
         unsupported = {
-            "q_lowpass/q_highpass": butterworth_filter and (q_lowpass is not None
-                                                            or q_highpass is not None),
             "tv_denoise_weight": tv_denoise and tv_denoise_weight is not None,
             "fit_probe_aberrations": fit_probe_aberrations and not fix_probe,
             # the aperture constraint takes precedence (ptychographic_constraints.py:1127-1137)
@@ -710,6 +708,15 @@ class SingleslicePtychography:
                 gauss_dev = torch.from_numpy(w).to(dev)
                 params.gaussian_weights = gauss_dev.data_ptr()
                 params.gaussian_radius = r
+        if butterworth_filter and (q_lowpass is not None or q_highpass is not None):
+            # singleslice_ptychography.py:961-965; a cut-off of None or 0 drops
+            # its factor (ptychographic_constraints.py:188-191)
+            params.butterworth = 1
+            params.q_lowpass = float(q_lowpass or 0.0)
+            params.q_highpass = float(q_highpass or 0.0)
+            params.butterworth_order = float(butterworth_order)
+            params.sampling[0] = float(self.sampling[0])
+            params.sampling[1] = float(self.sampling[1])
         pos_params = singular = proj_params = None
         if projection is not None:
             if getattr(self, "_exit_waves", None) is None:

@@ -1,11 +1,15 @@
-"""GPU object smoothing (gaussian_filter_sigma, ptychographic_constraints.py:
-119-151) against scipy.ndimage.gaussian_filter and the oracle.
+"""GPU object filters against scipy / the oracle: gaussian smoothing
+(gaussian_filter_sigma, ptychographic_constraints.py:119-151) and the
+Butterworth filter (q_lowpass / q_highpass, :153-201).
 
 Tolerances:
   * the smoothing kernel on a complex64 array: bitwise equal to scipy (the
     same float64 sums in the same order, float32 after each axis);
   * pure-phase smoothing: <= 2e-6 relative (float32 atan2 and sincos);
-  * whole reconstructions with the constraint: object, probe and every
+  * Butterworth on a complex64 array: <= 2e-6 relative against the oracle's
+    NumPy (complex128 after the float64 envelope): the GPU's DFT products
+    accumulate n terms in float32;
+  * whole reconstructions with a filter: object, probe and every
     per-iteration error <= 1e-4.
 """
 
@@ -63,6 +67,38 @@ def _init_object(shape, seed):
                   / 0.05).astype(np.complex64)
 
 
+@pytest.mark.parametrize("shape,lo,hi,order", [((200, 173), 1.5, None, 2), ((64, 64), None, 0.4, 2),
+                                               ((45, 90), 2.0, 0.3, 1.5), ((256, 256), 0.8, 0.1, 3)])
+def test_butterworth_matches_oracle(dev, shape, lo, hi, order):
+    from py4dstem_amd import kernels as K
+    say(f"butterworth {shape} low {lo} high {hi} order {order}")
+    rng = np.random.default_rng(5)
+    z = (np.exp(1j * nd.gaussian_filter(rng.standard_normal(shape), 1.0))
+         * (1 + 0.1 * rng.standard_normal(shape))).astype(np.complex64)
+    d = (0.2, 0.17)
+    t = torch.from_numpy(z.copy()).to(dev)
+    K.object_butterworth(t, lo, hi, order, d)
+    ref = OracleSingleslicePtychography.object_butterworth_constraint(z.copy(), lo, hi, order, d)
+    say(f"rel {rel(t.cpu().numpy(), ref):.2e}")
+    assert rel(t.cpu().numpy(), ref) <= 2e-6
+
+
+def test_butterworth_plane_wave(dev):
+    """A constant plus one plane wave: the wave is scaled by the envelope at
+    its frequency, the constant is kept."""
+    from py4dstem_amd import kernels as K
+    Px, Py, d = 96, 80, (0.2, 0.25)
+    x = np.arange(Px)[:, None]
+    y = np.arange(Py)[None, :]
+    kx, ky = 9, -13
+    wave = np.exp(2j * np.pi * (kx * x / Px + ky * y / Py))
+    z = (0.8 + 0.1j + 0.3 * wave).astype(np.complex64)
+    q = np.hypot(kx / (Px * d[0]), ky / (Py * d[1]))
+    t = torch.from_numpy(z.copy()).to(dev)
+    K.object_butterworth(t, q, None, 4, d)  # exactly at the low-pass cut-off: factor 1/2
+    np.testing.assert_allclose(t.cpu().numpy(), 0.8 + 0.1j + 0.15 * wave, atol=5e-6)
+
+
 @pytest.mark.parametrize("mixed,pure", [(False, False), (False, True), (True, False)])
 def test_reconstruct_with_gaussian_filter_matches_oracle(dev, mixed, pure):
     from py4dstem_amd import MixedstatePtychography
@@ -72,6 +108,39 @@ def test_reconstruct_with_gaussian_filter_matches_oracle(dev, mixed, pure):
     w = small_workload(seed=1)
     recon = dict(w["reconstruct"], max_batch_size=256, fix_probe=False, num_iter=4,
                  gaussian_filter_sigma=0.3, pure_phase_object=pure)
+    if mixed:
+        recon["orthogonalize_probe"] = False
+    init_obj = _init_object((256, 256), 17)
+    out = {}
+    pairs = ((("gpu", MixedstatePtychography), ("oracle", OracleMixedstatePtychography)) if mixed
+             else (("gpu", SingleslicePtychography), ("oracle", OracleSingleslicePtychography)))
+    for kind, cls in pairs:
+        kw = dict(w["init"], initial_object_guess=init_obj.copy())
+        if mixed:
+            kw["num_probes"] = 2
+        p = cls(datacube=w["datacube"], **kw)
+        np.random.seed(5)
+        p.preprocess(**w["preprocess"])
+        np.random.seed(1234)
+        out[kind] = p.reconstruct(**recon)
+    g, o = out["gpu"], out["oracle"]
+    err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(np.asarray(g.probe), o.probe):.2e} "
+        f"error {err:.2e}")
+    assert rel(g.object, o.object) <= 1e-4
+    assert rel(np.asarray(g.probe), o.probe) <= 1e-4
+    assert err <= 1e-4
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_reconstruct_with_butterworth_matches_oracle(dev, mixed):
+    from py4dstem_amd import MixedstatePtychography
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say(f"reconstruct with q_lowpass/q_highpass, mixed={mixed}")
+    w = small_workload(seed=1)
+    recon = dict(w["reconstruct"], max_batch_size=256, fix_probe=False, num_iter=4,
+                 q_lowpass=1.2, q_highpass=0.02, butterworth_order=2)
     if mixed:
         recon["orthogonalize_probe"] = False
     init_obj = _init_object((256, 256), 17)

@@ -835,3 +835,32 @@ def test_gaussian_smoothing_order_and_closed_forms():
     o = po.OracleSingleslicePtychography.object_constraints(
         np.exp(1j * rng.standard_normal((16, 16))).astype(np.complex64), None, 0.0, True, 1.2)
     np.testing.assert_allclose(np.abs(o), 1, rtol=1e-6)  # pure phase stays unit modulus
+
+
+def test_butterworth_closed_forms():
+    """ptychographic_constraints.py:153-201 restated in the oracle: a constant
+    plus one plane wave comes back as the constant plus the wave scaled by
+    the Butterworth envelope at its frequency (on the fftfreq grid in 1/A);
+    the low-pass envelope is 1/2 at the cut-off; cut-offs of 0 leave the
+    object unchanged."""
+    f = po.OracleSingleslicePtychography.object_butterworth_constraint
+    Px, Py, d = 48, 40, (0.2, 0.25)
+    x = np.arange(Px)[:, None]
+    y = np.arange(Py)[None, :]
+    kx, ky = 5, -7
+    wave = np.exp(2j * np.pi * (kx * x / Px + ky * y / Py))
+    obj = (0.8 + 0.1j + 0.3 * wave).astype(np.complex64)
+    q = np.hypot(kx / (Px * d[0]), ky / (Py * d[1]))
+    for lo, hi, order in [(1.0, None, 2), (None, 0.9, 3), (2.0, 0.5, 1.5)]:
+        env = 1.0
+        if hi:
+            env *= 1 - 1 / (1 + (q / hi) ** (2 * order))
+        if lo:
+            env *= 1 / (1 + (q / lo) ** (2 * order))
+        out = f(obj.copy(), lo, hi, order, d)
+        ref = 0.8 + 0.1j + 0.3 * env * wave
+        np.testing.assert_allclose(out, ref, atol=2e-6)
+    # the low-pass envelope is 1/2 at the cut-off: a wave exactly at q_c
+    out = f(obj.copy(), q, None, 4, d)
+    np.testing.assert_allclose(out, 0.8 + 0.1j + 0.15 * wave, atol=2e-6)
+    np.testing.assert_allclose(f(obj.copy(), 0, 0, 2, d), obj, atol=2e-6)
