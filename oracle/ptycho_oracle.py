@@ -35,6 +35,8 @@ PINNING = {
     "get_shifted_ar": "integer shift == np.roll; bilinear linearity",
     "get_CoM": "closed form on single-pixel and symmetric images",
     "fit_plane": "exact recovery of a noiseless plane",
+    "fit_origin_2d": "exact recovery of noiseless parabola / Bezier surfaces; equals "
+                     "scipy curve_fit on the reference's model functions",
     "forward": "Parseval, projection fixed point, A=0 => chi=-psi, angle(0)=0; detector mask: "
                "ones = no mask, masked pixels carry no update and count A^2 in the error",
     "adjoint": "noise-free ground truth is a fixed point; FP64 finite differences",
@@ -330,6 +332,33 @@ def get_shifted_ar(ar, xshift, yshift):
     )
 
 
+def fit_origin_2d(data, mask=None, fitfunction="plane"):
+    """``fit_origin`` (py4DSTEM/process/calibration/origin.py:76-190,
+    non-robust) with the models of py4DSTEM/process/fit/fit.py:127-162.
+    'constant' is the mean of the whole map (no mask, as the reference); the
+    other models are linear in their parameters, so ``curve_fit``'s optimum
+    is the least-squares solution, computed here with ``lstsq``."""
+    assert fitfunction in ("plane", "parabola", "bezier_two", "constant")
+    if fitfunction == "constant":
+        return np.mean(data) * np.ones_like(data)
+    if fitfunction == "plane":
+        return fit_plane(data, mask)
+    shape = data.shape
+    x, y = np.linspace(0, 1, shape[0]), np.linspace(0, 1, shape[1])
+    ry, rx = np.meshgrid(y, x)
+    if mask is None:
+        mask = np.ones(shape, dtype=bool)
+    if fitfunction == "parabola":
+        cols = [np.ones_like(rx), rx, rx ** 2, ry, ry ** 2, rx * ry]
+    else:
+        bx = [(1 - rx) ** 2, 2 * (1 - rx) * rx, rx ** 2]
+        by = [(1 - ry) ** 2, 2 * (1 - ry) * ry, ry ** 2]
+        cols = [bx[i] * by[j] for i in range(3) for j in range(3)]
+    A = np.stack([c[mask] for c in cols], axis=1)
+    popt, *_ = np.linalg.lstsq(A, np.asarray(data[mask], dtype=np.float64), rcond=None)
+    return sum(p * c for p, c in zip(popt, cols))
+
+
 def fit_plane(data, mask=None):
     """Least-squares plane ``mx*x + my*y + b`` on coordinates normalised to
     [0, 1] (py4DSTEM/process/fit/fit.py:38-124, 127-128, non-robust).
@@ -596,8 +625,8 @@ class OracleSingleslicePtychography:
         if force_com_shifts is None:
             if fit_function is not None:
                 finite = np.isfinite(com_mx)
-                com_fx = np.asarray(fit_plane(com_mx, finite), dtype=np.float32)
-                com_fy = np.asarray(fit_plane(com_my, finite), dtype=np.float32)
+                com_fx = np.asarray(fit_origin_2d(com_mx, finite, fit_function), dtype=np.float32)
+                com_fy = np.asarray(fit_origin_2d(com_my, finite, fit_function), dtype=np.float32)
             else:
                 com_fx = np.asarray(com_mx, dtype=np.float32)
                 com_fy = np.asarray(com_my, dtype=np.float32)
@@ -655,10 +684,19 @@ class OracleSingleslicePtychography:
 
         # probe (ptychographic_methods.py:1011-1102)
         if self._probe is None:
+            vacuum = None
+            if self._vacuum_probe_intensity is not None:  # :1033-1064 (no crop, no resampling)
+                vacuum = np.asarray(self._vacuum_probe_intensity, dtype=np.float32)
+                x0, y0 = get_CoM(vacuum)
+                vacuum = get_shifted_ar(vacuum, -x0, -y0)
+                if vacuum.shape != tuple(self._region_of_interest_shape):
+                    raise NotImplementedError("vacuum probe resampling")
+                self._semiangle_cutoff = np.inf
             probe = ComplexProbeOracle(
                 energy=self._energy, gpts=self._region_of_interest_shape,
                 sampling=self.sampling, semiangle_cutoff=self._semiangle_cutoff,
-                rolloff=self._rolloff, parameters=self._polar_parameters, mode=mode,
+                rolloff=self._rolloff, vacuum_probe_intensity=vacuum,
+                parameters=self._polar_parameters, mode=mode,
             ).build()._array
             probe_intensity = np.sum(np.abs(_fft2(probe, mode)) ** 2)
             probe *= np.sqrt(self._mean_diffraction_intensity / probe_intensity)

@@ -91,18 +91,44 @@ def real_space_sampling(energy, angular_sampling, frame_shape):
 # process/calibration/origin.py:76-190; process/fit/fit.py:38-128)
 # ---------------------------------------------------------------------------
 
-def fit_origin_plane(com, mask=None):
-    """Least-squares plane through a CoM map on [0,1]-normalised coordinates.
+FIT_FUNCTIONS = ("plane", "parabola", "bezier_two", "constant")
 
-    The reference calls ``scipy.optimize.curve_fit`` on the linear model
-    ``mx*x + my*y + b``; its optimum is the linear least-squares solution.
+
+def _origin_basis(name, x, y):
+    """Basis functions of the origin models (py4DSTEM/process/fit/fit.py:
+    127-162): the fitted surface is their linear combination."""
+    if name == "plane":
+        return [x, y, np.ones_like(x)]
+    if name == "parabola":
+        return [np.ones_like(x), x, x ** 2, y, y ** 2, x * y]
+    # quadratic Bernstein polynomials in x times those in y
+    bx = [(1 - x) ** 2, 2 * (1 - x) * x, x ** 2]
+    by = [(1 - y) ** 2, 2 * (1 - y) * y, y ** 2]
+    return [bx[i] * by[j] for i in range(3) for j in range(3)]
+
+
+def fit_origin(com, mask=None, fit_function="plane"):
+    """Fitted origin map of a centre-of-mass map (py4DSTEM/process/calibration/
+    origin.py:76-190, non-robust), coordinates normalised to [0, 1].
+
+    'constant' is the mean of the whole map (the mask is not used, as in the
+    reference).  The other models are linear in their parameters; the
+    reference fits them with ``scipy.optimize.curve_fit``, whose optimum is the
+    linear least-squares solution computed here.
     """
+    if fit_function not in FIT_FUNCTIONS:
+        raise AssertionError(f"fit_function must be one of {FIT_FUNCTIONS}, got {fit_function!r}")
+    com = np.asarray(com)
+    if fit_function == "constant":
+        return np.mean(com) * np.ones_like(com)
     shape = com.shape
     ry, rx = np.meshgrid(np.linspace(0, 1, shape[1]), np.linspace(0, 1, shape[0]))
     sel = np.ones(shape, dtype=bool) if mask is None else mask
-    design = np.column_stack([rx[sel], ry[sel], np.ones(int(sel.sum()))])
+    basis = _origin_basis(fit_function, rx, ry)
+    design = np.column_stack([b[sel] for b in basis])
     coef = np.linalg.lstsq(design, np.asarray(com[sel], dtype=np.float64), rcond=None)[0]
-    return coef[0] * rx + coef[1] * ry + coef[2]
+    return sum(c * b for c, b in zip(coef, basis))
+
 
 
 def _curl_div(cx, cy, maximize_divergence):
@@ -249,3 +275,29 @@ def center_positions(positions_px32, object_shape):
     com = positions_px32.mean(0)
     positions_px32 -= com - np.array(object_shape) / 2
     return positions_px32
+
+
+def center_of_mass(ar):
+    """Intensity centre of mass in pixel coordinates (rows, columns) of a
+    2-D array (py4DSTEM/process/utils/utils.py:166-187, not corner-centred):
+    integer index grids times the array, summed, over the array's own sum."""
+    ar = np.asarray(ar)
+    nx, ny = ar.shape
+    ry, rx = np.meshgrid(np.arange(ny), np.arange(nx))
+    total = np.sum(ar)
+    return np.sum(rx * ar) / total, np.sum(ry * ar) / total
+
+
+def bilinear_shift(ar, xshift, yshift):
+    """Periodic bilinear shift by (xshift, yshift) pixels
+    (py4DSTEM/preprocess/utils.py:59-121 with bilinear=True): the four
+    integer rolls around the shift weighted by the fractional parts."""
+    ar = np.asarray(ar)
+    xf = int(np.floor(xshift))
+    yf = int(np.floor(yshift))
+    wx = xshift - xf
+    wy = yshift - yf
+    return (np.roll(ar, (xf, yf), axis=(0, 1)) * ((1 - wx) * (1 - wy))
+            + np.roll(ar, (xf + 1, yf), axis=(0, 1)) * (wx * (1 - wy))
+            + np.roll(ar, (xf, yf + 1), axis=(0, 1)) * ((1 - wx) * wy)
+            + np.roll(ar, (xf + 1, yf + 1), axis=(0, 1)) * (wx * wy))

@@ -27,9 +27,11 @@ import torch
 from . import kernels as K
 from ._lib import PtyGdParams, require_gpu
 from .preprocess import (
+    bilinear_shift,
+    center_of_mass,
     center_positions,
     extract_calibrations,
-    fit_origin_plane,
+    fit_origin,
     initial_object_shape,
     real_space_sampling,
     scan_positions_in_pixels,
@@ -248,10 +250,6 @@ class SingleslicePtychography:
             raise NotImplementedError("crop_patterns is not supported")
         if self._object_type != "complex":
             raise NotImplementedError("only object_type='complex' is supported")
-        if fit_function not in (None, "plane"):
-            raise NotImplementedError(f"fit_function={fit_function!r} is not supported (plane only)")
-        if self._vacuum_probe_intensity is not None:
-            raise NotImplementedError("vacuum_probe_intensity is not supported")
 
         dc = self._datacube
         Rshape = tuple(dc.data.shape[:2])
@@ -296,8 +294,8 @@ class SingleslicePtychography:
         if force_com_shifts is None:
             if fit_function is not None:
                 finite = np.isfinite(com_mx)
-                com_fx = np.asarray(fit_origin_plane(com_mx, finite), dtype=np.float32)
-                com_fy = np.asarray(fit_origin_plane(com_my, finite), dtype=np.float32)
+                com_fx = np.asarray(fit_origin(com_mx, finite, fit_function), dtype=np.float32)
+                com_fy = np.asarray(fit_origin(com_my, finite, fit_function), dtype=np.float32)
             else:
                 com_fx = np.asarray(com_mx, dtype=np.float32)
                 com_fy = np.asarray(com_my, dtype=np.float32)
@@ -419,9 +417,24 @@ class SingleslicePtychography:
     def _initialize_probe(self, initial_probe):
         """ptychographic_methods.py:1011-1102."""
         if initial_probe is None:
+            vacuum = None
+            if self._vacuum_probe_intensity is not None:
+                # the measured vacuum intensity, its centre of mass moved to
+                # the corner by a periodic bilinear shift, is the aperture
+                # (:1033-1064); the semiangle cut-off no longer applies
+                vacuum = np.asarray(self._vacuum_probe_intensity, dtype=np.float32)
+                x0, y0 = center_of_mass(vacuum)
+                vacuum = bilinear_shift(vacuum, -x0, -y0)
+                if vacuum.shape != tuple(self._region_of_interest_shape):
+                    raise NotImplementedError(
+                        f"vacuum_probe_intensity of shape {vacuum.shape} would need resampling to "
+                        f"{tuple(self._region_of_interest_shape)}, which is not supported")
+                self._semiangle_cutoff = np.inf
             probe = ComplexProbe(energy=self._energy, gpts=self._region_of_interest_shape,
-                                 sampling=self.sampling, semiangle_cutoff=self._semiangle_cutoff,
-                                 rolloff=self._rolloff,
+                                 sampling=self.sampling,
+                                 semiangle_cutoff=(np.inf if vacuum is not None
+                                                   else self._semiangle_cutoff),
+                                 rolloff=self._rolloff, vacuum_probe_intensity=vacuum,
                                  parameters=self._polar_parameters).build()._array
             intensity = np.sum(np.abs(np.fft.fft2(probe)) ** 2)
             probe *= np.sqrt(self._mean_diffraction_intensity / intensity)

@@ -12,8 +12,10 @@
 
 import numpy as np
 import pytest
+import scipy.ndimage
 
 import oracle.ptycho_oracle as po
+from tests.conftest import rel
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -78,3 +80,64 @@ def test_preprocess_matches_oracle(dev, small_case):
     np.testing.assert_array_equal(p._amplitudes, o._amplitudes)
     assert abs(p._mean_diffraction_intensity / o._mean_diffraction_intensity - 1) < 1e-6
     np.testing.assert_array_equal(p._object_fov_mask, o._object_fov_mask)
+
+
+def test_vacuum_probe_matches_oracle(dev):
+    """vacuum_probe_intensity (ptychographic_methods.py:1033-1064): the
+    measured intensity, off-centre here, is moved to the corner by its centre
+    of mass and its square root becomes the aperture.  Probe <= 1e-6 against
+    the oracle; a short reconstruction from it <= 1e-4."""
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say("vacuum probe intensity")
+    w = small_workload(seed=1)
+    ref = po.OracleSingleslicePtychography(datacube=w["datacube"], **w["init"])
+    ref.preprocess(**w["preprocess"])
+    V = np.fft.fftshift(np.abs(np.fft.fft2(ref._probe)) ** 2).astype(np.float32)
+    V = po.get_shifted_ar(V, 2.3, -1.7).astype(np.float32)  # off-centre measurement
+    rng = np.random.default_rng(17)  # the well-conditioned start of the reconstruction parity tests
+    init_obj = np.exp(0.02j * scipy.ndimage.gaussian_filter(rng.standard_normal((256, 256)), 3)
+                      / 0.05).astype(np.complex64)
+    out = {}
+    for kind, cls in (("gpu", SingleslicePtychography), ("oracle", po.OracleSingleslicePtychography)):
+        p = cls(datacube=w["datacube"], vacuum_probe_intensity=V,
+                **dict(w["init"], initial_object_guess=init_obj.copy()))
+        p.preprocess(**w["preprocess"])
+        out[kind] = p
+    g, o = out["gpu"], out["oracle"]
+    say(f"probe {rel(g._probe, o._probe):.2e}")
+    assert g._semiangle_cutoff == np.inf
+    assert rel(g._probe, o._probe) <= 1e-6
+    recon = dict(w["reconstruct"], max_batch_size=256, fix_probe=False, num_iter=3)
+    g.reconstruct(**recon)
+    o.reconstruct(**recon)
+    err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(g.probe, o.probe):.2e} error {err:.2e}")
+    assert rel(g.object, o.object) <= 1e-4
+    assert rel(g.probe, o.probe) <= 1e-4
+    assert err <= 1e-4
+
+
+@pytest.mark.parametrize("fit_function", ["plane", "parabola", "bezier_two", "constant"])
+def test_preprocess_fit_functions_match_oracle(dev, fit_function):
+    """The measured pattern centres fitted with each origin model of
+    fit_origin (origin.py:76-190), then the bilinear shift: amplitudes
+    <= 1e-5 relative and fitted centres <= 1e-5 px against the oracle (the
+    GPU centres of mass agree with NumPy's to 1e-6 relative)."""
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say(f"preprocess with fit_function={fit_function}")
+    w = small_workload(seed=3, scan=(12, 12), object_px=160)
+    pre = {k: v for k, v in w["preprocess"].items() if k != "force_com_shifts"}
+    out = {}
+    for kind, cls in (("gpu", SingleslicePtychography), ("oracle", po.OracleSingleslicePtychography)):
+        p = cls(datacube=w["datacube"], **w["init"])
+        p.preprocess(fit_function=fit_function, **pre)
+        out[kind] = p
+    g, o = out["gpu"], out["oracle"]
+    dc = max(np.abs(g._com_fitted_x - o._com_fitted_x).max(),
+             np.abs(g._com_fitted_y - o._com_fitted_y).max())
+    say(f"amplitudes {rel(g._amplitudes, o._amplitudes):.2e} fitted centres {dc:.2e} px")
+    assert dc <= 1e-5
+    assert rel(g._amplitudes, o._amplitudes) <= 1e-5
+    assert abs(g._mean_diffraction_intensity / o._mean_diffraction_intensity - 1) < 1e-6

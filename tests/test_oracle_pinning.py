@@ -864,3 +864,82 @@ def test_butterworth_closed_forms():
     out = f(obj.copy(), q, None, 4, d)
     np.testing.assert_allclose(out, 0.8 + 0.1j + 0.15 * wave, atol=2e-6)
     np.testing.assert_allclose(f(obj.copy(), 0, 0, 2, d), obj, atol=2e-6)
+
+
+def test_vacuum_probe_closed_form():
+    """ptychographic_methods.py:1033-1064 restated in the oracle.  A centred,
+    point-symmetric, integer-valued vacuum intensity V (float32 sums exact, so
+    the centre of mass is exactly the grid centre and the bilinear shift an
+    exact roll) gives the probe ifft2(sqrt(V at the corner) exp(-i chi)),
+    normalised to the mean diffraction intensity, with exp(-i chi) the
+    spectral phase of the default probe; the semiangle cut-off becomes inf.
+    The host helpers of the drop-in class equal the oracle's.  (With
+    non-exact sums the reference's centre of mass is a few 1e-6 px off the
+    grid, and the shift leaks that fraction of the aperture edge into the
+    zero pixels next to it, which the square root raises to ~1e-3.)"""
+    from py4dstem_amd.preprocess import bilinear_shift, center_of_mass
+    from py4dstem_amd.synthetic import small_workload
+    w = small_workload(seed=1, scan=(8, 8), object_px=128)
+    a = po.OracleSingleslicePtychography(datacube=w["datacube"], **w["init"])
+    a.preprocess(**w["preprocess"])
+    F = np.fft.fft2(a._probe.astype(np.complex128))
+    V = np.abs(F) ** 2
+    V = 0.5 * (V + np.roll(V[::-1, ::-1], 1, axis=(0, 1)))  # V(k) = V(-k)
+    V = np.round(V / V.max() * 1000).astype(np.float32)
+    S = V.shape[0]
+    b = po.OracleSingleslicePtychography(datacube=w["datacube"],
+                                         vacuum_probe_intensity=np.fft.fftshift(V), **w["init"])
+    b.preprocess(**w["preprocess"])
+    assert b._semiangle_cutoff == np.inf
+    assert po.get_CoM(np.fft.fftshift(V)) == (S // 2, S // 2)
+    phase = np.where(np.abs(F) > 0, F / np.maximum(np.abs(F), 1e-300), 0)
+    p = np.fft.ifft2(np.sqrt(V) * phase)
+    p = p / np.sqrt(np.sum(np.abs(p) ** 2))
+    p = p * np.sqrt(b._mean_diffraction_intensity / np.sum(np.abs(np.fft.fft2(p)) ** 2))
+    assert np.linalg.norm(b._probe - p) / np.linalg.norm(p) < 2e-6
+    rng = np.random.default_rng(2)
+    ar = rng.uniform(0, 1, (24, 30)).astype(np.float32)
+    assert center_of_mass(ar) == po.get_CoM(ar)
+    for dx, dy in [(2.3, -1.7), (-5.0, 3.0), (0.49, -0.51)]:
+        assert np.array_equal(bilinear_shift(ar, dx, dy), po.get_shifted_ar(ar, dx, dy))
+
+
+def test_fit_origin_models_match_curve_fit():
+    """fit_origin's models (origin.py:76-190, fit.py:127-162): noiseless
+    surfaces are recovered exactly; on noisy masked data the least-squares
+    solution equals scipy.optimize.curve_fit (the reference's call) on the
+    model functions written out term by term; 'constant' is the mean; the
+    drop-in class's host fit equals the oracle's."""
+    from scipy.optimize import curve_fit
+
+    from py4dstem_amd.preprocess import fit_origin
+
+    def parabola(xy, c0, cx1, cx2, cy1, cy2, cxy):
+        return c0 + cx1 * xy[0] + cy1 * xy[1] + cx2 * xy[0] ** 2 + cy2 * xy[1] ** 2 + cxy * xy[0] * xy[1]
+
+    def bezier_two(xy, c00, c01, c02, c10, c11, c12, c20, c21, c22):
+        x, y = xy
+        return (c00 * (1 - x) ** 2 * (1 - y) ** 2 + c10 * 2 * (1 - x) * x * (1 - y) ** 2
+                + c20 * x ** 2 * (1 - y) ** 2 + c01 * 2 * (1 - x) ** 2 * (1 - y) * y
+                + c11 * 4 * (1 - x) * x * (1 - y) * y + c21 * 2 * x ** 2 * (1 - y) * y
+                + c02 * (1 - x) ** 2 * y ** 2 + c12 * 2 * (1 - x) * x * y ** 2 + c22 * x ** 2 * y ** 2)
+
+    shape = (13, 11)
+    ry, rx = np.meshgrid(np.linspace(0, 1, shape[1]), np.linspace(0, 1, shape[0]))
+    rng = np.random.default_rng(4)
+    mask = rng.uniform(size=shape) > 0.2
+    for name, f, npar in (("parabola", parabola, 6), ("bezier_two", bezier_two, 9)):
+        p = rng.standard_normal(npar)
+        clean = f((rx, ry), *p)
+        np.testing.assert_allclose(po.fit_origin_2d(clean, None, name), clean, atol=1e-10)
+        noisy = clean + 0.05 * rng.standard_normal(shape)
+        popt, _ = curve_fit(f, np.vstack((rx[mask], ry[mask])), noisy[mask], p0=np.zeros(npar))
+        np.testing.assert_allclose(po.fit_origin_2d(noisy, mask, name), f((rx, ry), *popt),
+                                   atol=1e-6)
+        np.testing.assert_allclose(fit_origin(noisy, mask, name), po.fit_origin_2d(noisy, mask, name),
+                                   atol=1e-12)
+    d = rng.standard_normal(shape).astype(np.float32)
+    assert np.all(po.fit_origin_2d(d, mask, "constant") == np.mean(d))
+    assert np.array_equal(fit_origin(d, mask, "constant"), po.fit_origin_2d(d, mask, "constant"))
+    with pytest.raises(AssertionError):
+        fit_origin(d, mask, "cubic")
