@@ -181,9 +181,15 @@ int pty_shift_amplitudes(const float* intensities, int N_in, int S, const int* s
 
 /* ---- whole reconstruction loop ------------------------------------------ */
 
+/* pty_problem.object_type */
+#define PTY_OBJECT_COMPLEX 0   /* object: complex64 [Px][Py] */
+#define PTY_OBJECT_POTENTIAL 1 /* object: float32 potential V [Px][Py], patches exp(i V)
+                                  (object_type='potential', ptychographic_methods.py:
+                                  40-79, 1522-1523, 1868-1880); gradient descent only */
+
 typedef struct pty_problem {
   int S, Px, Py, N;
-  void* object;                          /* complex64 [Px][Py], updated in place */
+  void* object;                          /* [Px][Py] (see object_type), updated in place */
   void* probe;                           /* complex64 [S][S], updated in place */
   const float* positions;                /* float32 [N][2] object pixels */
   const float* amplitudes;               /* float32 [N][S][S], kernel layout */
@@ -192,6 +198,7 @@ typedef struct pty_problem {
   const float* fourier_mask;             /* float32 [S][S] kernel layout or NULL: the
                                             detector_fourier_mask of the GD projection
                                             (ptychographic_methods.py:1603-1616) */
+  int object_type;                       /* PTY_OBJECT_COMPLEX or PTY_OBJECT_POTENTIAL */
 } pty_problem;
 
 typedef struct pty_gd_params {
@@ -226,6 +233,9 @@ typedef struct pty_gd_params {
   double q_highpass;         /* A^-1 */
   double butterworth_order;
   double sampling[2];        /* object pixel size (A) along axes 0 and 1 */
+  /* potential objects: clip V at zero after the shrinkage
+   * (ptychographic_constraints.py:99-118, 458-462) */
+  int object_positivity;
 } pty_gd_params;
 
 size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);

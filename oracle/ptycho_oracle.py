@@ -541,8 +541,11 @@ class OracleSingleslicePtychography:
                  vacuum_probe_intensity=None, polar_parameters=None,
                  initial_object_guess=None, initial_probe_guess=None,
                  initial_scan_positions=None, object_padding_px=None,
-                 verbose=False, mode="c64", **kwargs):
+                 verbose=False, mode="c64", object_type="complex", **kwargs):
         # singleslice_ptychography.py:142-189
+        if object_type != "potential" and object_type != "complex":
+            raise ValueError(f"object_type must be either 'potential' or 'complex', not {object_type}")
+        self._object_type = object_type
         for key in kwargs:
             if key not in POLAR_SYMBOLS and key not in POLAR_ALIASES:
                 raise ValueError("{} not a recognized parameter".format(key))
@@ -669,9 +672,11 @@ class OracleSingleslicePtychography:
             p, q = np.round(np.max(self._positions_px, axis=0))
             p = np.max([np.round(p + pad_x), self._region_of_interest_shape[0]]).astype("int")
             q = np.max([np.round(q + pad_y), self._region_of_interest_shape[1]]).astype("int")
-            self._object = np.ones((p, q), dtype=np.complex64)
+            self._object = (np.zeros((p, q), dtype=np.float32) if self._object_type == "potential"
+                            else np.ones((p, q), dtype=np.complex64))
         else:
-            self._object = np.asarray(self._object, dtype=np.complex64)
+            self._object = np.asarray(self._object, dtype=np.float32
+                                      if self._object_type == "potential" else np.complex64)
         self._object_initial = self._object.copy()
         self._object_shape = self._object.shape
 
@@ -768,6 +773,8 @@ class OracleSingleslicePtychography:
         rows, cols = patch_indices(positions_px, self._region_of_interest_shape, obj.shape)
         shifted_probes = fft_shift(probe, frac, mode)
         object_patches = obj[rows, cols]
+        if getattr(self, "_object_type", "complex") == "potential":  # :1522-1523
+            object_patches = np.exp(1j * object_patches)
         overlap = shifted_probes * object_patches
         fourier_overlap = _fft2(overlap, mode)
         if fourier_mask is not None:
@@ -783,9 +790,9 @@ class OracleSingleslicePtychography:
 
     def adjoint(self, obj, probe, object_patches, shifted_probes, positions_px,
                 exit_waves, step_size, normalization_min, fix_probe):
-        """``_gradient_descent_adjoint`` for a complex object
-        (ptychographic_methods.py:1818-1908).  Updates in place, as the
-        reference's ``+=`` does."""
+        """``_gradient_descent_adjoint`` (ptychographic_methods.py:1818-1908),
+        complex or potential object.  Updates in place, as the reference's
+        ``+=`` does."""
         roi = self._region_of_interest_shape
         shape = obj.shape
         probe_normalization = scatter_bincount(np.abs(shifted_probes) ** 2,
@@ -795,10 +802,17 @@ class OracleSingleslicePtychography:
             + ((1 - normalization_min) * probe_normalization) ** 2
             + (normalization_min * np.max(probe_normalization)) ** 2
         )
-        obj += step_size * (
-            scatter_bincount(np.conj(shifted_probes) * exit_waves, positions_px, roi, shape)
-            * probe_normalization
-        )
+        if getattr(self, "_object_type", "complex") == "potential":  # :1868-1880
+            obj += step_size * (
+                scatter_bincount(np.real(-1j * np.conj(object_patches) * np.conj(shifted_probes)
+                                         * exit_waves), positions_px, roi, shape)
+                * probe_normalization
+            )
+        else:
+            obj += step_size * (
+                scatter_bincount(np.conj(shifted_probes) * exit_waves, positions_px, roi, shape)
+                * probe_normalization
+            )
         if not fix_probe:
             object_normalization = np.sum(np.abs(object_patches) ** 2, axis=0)
             object_normalization = 1 / np.sqrt(
@@ -996,19 +1010,31 @@ class OracleSingleslicePtychography:
 
     @staticmethod
     def object_constraints(obj, object_mask, shrinkage_rad=0.0, pure_phase_object=False,
-                           gaussian_sigma_px=None, butterworth=None):
-        """ptychographic_constraints.py:29-97, 411-464 (complex object); the
-        gaussian smoothing of :119-151 when gaussian_sigma_px is given (the
-        reference's scipy.ndimage.gaussian_filter call with its defaults)."""
+                           gaussian_sigma_px=None, butterworth=None, object_type="complex",
+                           object_positivity=True):
+        """ptychographic_constraints.py:29-118, 411-464; the gaussian
+        smoothing of :119-151 when gaussian_sigma_px is given (the reference's
+        scipy.ndimage.gaussian_filter call with its defaults).  A potential
+        object is smoothed directly, shrinks by subtraction and is clipped at
+        zero (object_positivity) instead of the amplitude threshold."""
+        potential = object_type == "potential"
         if gaussian_sigma_px is not None:
             import scipy.ndimage
-            if not pure_phase_object:
+            if not pure_phase_object or potential:
                 obj = scipy.ndimage.gaussian_filter(obj, gaussian_sigma_px)
             else:
                 phase = scipy.ndimage.gaussian_filter(np.angle(obj), gaussian_sigma_px)
                 obj = np.exp(1.0j * phase)
         if butterworth is not None:  # (q_lowpass, q_highpass, order, sampling)
             obj = OracleSingleslicePtychography.object_butterworth_constraint(obj, *butterworth)
+            if potential:
+                obj = np.real(obj)
+        if potential:
+            if shrinkage_rad > 0.0 or object_mask is not None:
+                if object_mask is not None:
+                    shrinkage_rad += obj[..., object_mask].mean()
+                obj -= shrinkage_rad
+            return obj.clip(0.0) if object_positivity else obj
         if shrinkage_rad > 0.0 or object_mask is not None:
             phase = np.angle(obj)
             amp = np.abs(obj)
@@ -1087,9 +1113,14 @@ class OracleSingleslicePtychography:
                     constrain_probe_amplitude_relative_radius=0.5,
                     constrain_probe_amplitude_relative_width=0.05, gaussian_filter_sigma=None,
                     gaussian_filter=True, butterworth_filter=True, q_lowpass=None, q_highpass=None,
-                    butterworth_order=2, **kwargs):
+                    butterworth_order=2, object_positivity=True, **kwargs):
         """phase/singleslice_ptychography.py:606-996 (gradient descent and the
         projection sets)."""
+        potential = getattr(self, "_object_type", "complex") == "potential"
+        if potential and (reconstruction_method not in ("GD", "gradient-descent")
+                          or not fix_positions):
+            raise NotImplementedError("potential objects: gradient descent with fixed positions")
+        pure_phase_object = pure_phase_object and not potential  # :976
         # ptychographic_constraints.py:142, flag as singleslice_ptychography.py:959
         gaussian_sigma_px = (float(gaussian_filter_sigma) / float(self.sampling[0])
                              if gaussian_filter and gaussian_filter_sigma is not None else None)
@@ -1169,7 +1200,9 @@ class OracleSingleslicePtychography:
             error /= self._mean_diffraction_intensity * N
             self._object = self.object_constraints(self._object, object_mask,
                                                    shrinkage_rad, pure_phase_object,
-                                                   gaussian_sigma_px, butterworth)
+                                                   gaussian_sigma_px, butterworth,
+                                                   getattr(self, "_object_type", "complex"),
+                                                   object_positivity)
             if fix_probe_com and not fix_probe:
                 self._probe = self.probe_com_constraint(self._probe)
             # ptychographic_constraints.py:1091-1147 (single probe) and
@@ -1347,5 +1380,7 @@ class OracleMixedstatePtychography(OracleSingleslicePtychography):
         method = kwargs.get("reconstruction_method", "gradient-descent")
         if method not in ("GD", "gradient-descent"):
             raise NotImplementedError("the mixed-state oracle restates gradient descent only")
+        if getattr(self, "_object_type", "complex") == "potential":
+            raise NotImplementedError("the mixed-state oracle restates complex objects only")
         self._orthogonalize_probe = orthogonalize_probe
         return super().reconstruct(*args, **kwargs)

@@ -24,7 +24,8 @@ class PtyProblem(ctypes.Structure):
     _fields_ = [("S", c_int), ("Px", c_int), ("Py", c_int), ("N", c_int),
                 ("object", c_void_p), ("probe", c_void_p), ("positions", c_void_p),
                 ("amplitudes", c_void_p), ("fov_mask_inverse", c_void_p),
-                ("mean_intensity", c_float), ("fourier_mask", c_void_p)]
+                ("mean_intensity", c_float), ("fourier_mask", c_void_p),
+                ("object_type", c_int)]
 
 
 class PtyGdParams(ctypes.Structure):
@@ -38,7 +39,7 @@ class PtyGdParams(ctypes.Structure):
                 ("gaussian_weights", c_void_p), ("gaussian_radius", c_int),
                 ("butterworth", c_int), ("q_lowpass", ctypes.c_double),
                 ("q_highpass", ctypes.c_double), ("butterworth_order", ctypes.c_double),
-                ("sampling", ctypes.c_double * 2)]
+                ("sampling", ctypes.c_double * 2), ("object_positivity", c_int)]
 
 
 class PtyPositionParams(ctypes.Structure):

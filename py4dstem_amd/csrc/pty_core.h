@@ -279,7 +279,18 @@ struct FusedArgs {
   // (slot-indexed, may alias); exit_valid = 0 takes the overlap instead
   float proj_a, proj_b, proj_c, proj_x, proj_y;
   int exit_valid;
+  // potential object (object_type 'potential', ptychographic_methods.py:
+  // 1522-1523): float32 V [Px][Py]; the patch is exp(i V).  Read by the
+  // gradient-descent pass instantiated for it, in place of `object`
+  const float* object_pot;
 };
+
+// exp(i v) as NumPy's complex64 exp(1j * v): (cos v, sin v)
+PTY_HD cf expi(float v) {
+  float s, c;
+  sincosf(v, &s, &c);
+  return mkcf(c, s);
+}
 
 template <int S> struct Fused {
   using F = Frame<S>;
@@ -426,8 +437,9 @@ template <int S> struct Fused {
 
   // P4+P5: shifted probe out, exit wave psi = P * O_patch, then the forward
   // column pass A on psi straight from registers
-  template <class L> static PTY_HD void p_exitwave_colA(int tid, const FusedArgs& a, int b, Regs& r,
-                                                        L lds, const cf* tw) {
+  // POT: the object is a potential V and the patch exp(i V)
+  template <class L, bool POT = false>
+  static PTY_HD void p_exitwave_colA(int tid, const FusedArgs& a, int b, Regs& r, L lds, const cf* tw) {
     p_colAinv_to_regs(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
     cf* ps_b = a.ps_out ? a.ps_out + (size_t)b * S * S : nullptr;
@@ -435,7 +447,8 @@ template <int S> struct Fused {
     for (int m = 0; m < N1; ++m) {
       const int i = n2 + N2 * m;
       if (ps_b) ps_b[i * S + cc] = r.v[m];
-      r.v[m] = cmul(r.v[m], a.object[object_offset(a, r, i, cc)]);
+      const unsigned o = object_offset(a, r, i, cc);
+      r.v[m] = cmul(r.v[m], POT ? expi(a.object_pot[o]) : a.object[o]);
     }
     P::template A<false>(n2, r.v, tw);
 #pragma unroll
@@ -663,7 +676,8 @@ template <int S> struct Fused {
 // (the ramp table, column passes, row <-> column); ``run.wave_barrier()``
 // only orders the LDS accesses of one wave, enough between two passes along
 // rows.  Nine work-group barriers per position.
-template <int S, class Run, bool MASK = false> PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
+template <int S, class Run, bool MASK = false, bool POT = false>
+PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
   using K = Fused<S>;
   using Regs = typename K::Regs;
   // ifft2(probe_hat * ramp): B', A' along rows, then B', A' along columns
@@ -676,7 +690,9 @@ template <int S, class Run, bool MASK = false> PTY_HD void fused_position(Run& r
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
   run.barrier();
   // shifted probe in registers; exit wave; fft2 starts with columns
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_exitwave_colA(tid, a, b, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::template p_exitwave_colA<decltype(lds), POT>(tid, a, b, r, lds, tw);
+  });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<false>(tid, r, lds, tw); });
   run.barrier();

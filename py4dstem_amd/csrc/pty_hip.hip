@@ -101,7 +101,8 @@ template <int S> struct GpuRunner {
   }
 };
 
-template <int S, bool MASK = false> __device__ __forceinline__ void fused_body(const FusedArgs& a, int b) {
+template <int S, bool MASK = false, bool POT = false>
+__device__ __forceinline__ void fused_body(const FusedArgs& a, int b) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
   __shared__ cf tw[S];
   __shared__ double red[Frame<S>::T / 64];
@@ -111,12 +112,12 @@ template <int S, bool MASK = false> __device__ __forceinline__ void fused_body(c
   Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
-  fused_position<S, GpuRunner<S>, MASK>(run, a, b);
+  fused_position<S, GpuRunner<S>, MASK, POT>(run, a, b);
 }
 
-template <int S, bool MASK>
+template <int S, bool MASK, bool POT = false>
 __global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
-  fused_body<S, MASK>(a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
+  fused_body<S, MASK, POT>(a, a.b0 + (int)blockIdx.x);  // grid covers slots b0..B-1
 }
 
 // mixed-state forward/adjoint terms, one work group per position
@@ -472,6 +473,31 @@ __device__ __forceinline__ void object_update_body(cf* __restrict__ obj, const c
   }
 }
 
+// potential object (ptychographic_methods.py:1868-1880): V += step * (R * n),
+// R = Re(-i conj(exp(i V)) G) = cos V Im G - sin V Re G with G the float64
+// scatter of conj(P_shift) chi (conj(exp(i V)) is common to a pixel's terms;
+// the reference rounds each term's real part to float32 before its float64
+// sum, a difference at float32 rounding level)
+__global__ __launch_bounds__(256) void k_object_update_potential(float* __restrict__ V, const cf* __restrict__ grad,
+                                                                 const float* __restrict__ spsum,
+                                                                 const float* __restrict__ tilemax, int ntiles,
+                                                                 long long npix, float step, float c1, float c2) {
+#pragma clang fp contract(off)
+  __shared__ float sh[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < ntiles; i += blockDim.x) m = fmaxf(m, tilemax[i]);
+  m = block_max(m, sh);
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    const float v = V[p];
+    const cf o = expi(v);
+    const cf g = grad[p];
+    const float R = (float)((double)o.x * (double)g.y - (double)o.y * (double)g.x);
+    const float n = overlap_norm(spsum[p], m, c1, c2);
+    V[p] = v + step * (R * n);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_object_update(cf* __restrict__ obj, const cf* __restrict__ grad,
                                                        const float* __restrict__ spsum,
                                                        const float* __restrict__ tilemax, int ntiles,
@@ -531,11 +557,12 @@ constexpr int PROBE_CHUNK = PTY_PROBE_CHUNK;  // batch slots per partial sum (a 
 
 struct ProbePartialArgs {
   const cf* chi;
-  const cf* object;
+  const cf* object;        // complex object, or null with object_pot
   const float* positions;
   const int* order;
   int N, B, S, Px, Py;
   double* part;  // [nchunk][S*S][3] (re, im, |O|^2)
+  const float* object_pot = nullptr;  // potential V: the patch is exp(i V)
 };
 
 __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, int blk, int chunk) {
@@ -563,7 +590,8 @@ __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, in
 #pragma unroll 8
   for (int k = 0; k < nb; ++k) {  // batch order
     const int x0 = anchor[2 * k], y0 = anchor[2 * k + 1];
-    const cf o = a.object[(size_t)wrapi(x0 + fi, a.Px) * a.Py + wrapi(y0 + fj, a.Py)];
+    const size_t off = (size_t)wrapi(x0 + fi, a.Px) * a.Py + wrapi(y0 + fj, a.Py);
+    const cf o = a.object_pot ? expi(a.object_pot[off]) : a.object[off];
     const float ao = hypotf(o.x, o.y);
     so += (double)(ao * ao);
     const cf g = cconjmul(o, chi[(size_t)k * S * S]);
@@ -664,16 +692,22 @@ __global__ __launch_bounds__(256) void k_probe_apply(cf* __restrict__ probe, con
 // ---------------------------------------------------------------------------
 constexpr int CONSTRAINT_BLOCKS = 256;
 
+// masked sum of the phase (complex object) or of the potential (pot != null)
 __device__ __forceinline__ void constraint_partial_body(const cf* __restrict__ obj,
                                                         const unsigned char* __restrict__ mask, long long npix,
-                                                        double* part, int blk, int nblk) {
+                                                        double* part, int blk, int nblk,
+                                                        const float* __restrict__ pot = nullptr) {
   __shared__ double ssum[256];
   __shared__ double scnt[256];
   double s = 0.0, c = 0.0;
   for (long long p = (long long)blk * blockDim.x + threadIdx.x; p < npix; p += (long long)nblk * blockDim.x) {
     if (mask[p]) {
-      const cf o = obj[p];
-      s += (double)atan2f(o.y, o.x);
+      if (pot) {
+        s += (double)pot[p];
+      } else {
+        const cf o = obj[p];
+        s += (double)atan2f(o.y, o.x);
+      }
       c += 1.0;
     }
   }
@@ -695,8 +729,9 @@ __device__ __forceinline__ void constraint_partial_body(const cf* __restrict__ o
 
 __global__ __launch_bounds__(256) void k_constraint_partial(const cf* __restrict__ obj,
                                                             const unsigned char* __restrict__ mask,
-                                                            long long npix, double* part) {
-  constraint_partial_body(obj, mask, npix, part, blockIdx.x, gridDim.x);
+                                                            long long npix, double* part,
+                                                            const float* __restrict__ pot) {
+  constraint_partial_body(obj, mask, npix, part, blockIdx.x, gridDim.x, pot);
 }
 
 __device__ __forceinline__ void constraint_apply_body(cf* __restrict__ obj, long long npix,
@@ -740,6 +775,38 @@ __global__ __launch_bounds__(256) void k_constraint_apply(cf* __restrict__ obj, 
                                                           int use_mask, float shrinkage_rad,
                                                           int pure_phase) {
   constraint_apply_body(obj, npix, part, nparts, use_mask, shrinkage_rad, pure_phase, blockIdx.x, gridDim.x);
+}
+
+// potential object (ptychographic_constraints.py:60-118, 447-462): V minus
+// (shrinkage_rad + masked mean of V) when shrinking, then clip at zero when
+// object_positivity; NaN stays NaN as under numpy clip
+__global__ __launch_bounds__(256) void k_constraint_apply_potential(float* __restrict__ V, long long npix,
+                                                                    const double* __restrict__ part, int nparts,
+                                                                    int use_mask, float shrinkage_rad,
+                                                                    int positivity) {
+  __shared__ float shrink_sh;
+  if (threadIdx.x == 0) {
+    float shrink = shrinkage_rad;
+    if (use_mask) {
+      double s = 0.0, c = 0.0;
+      for (int k = 0; k < nparts; ++k) {
+        s += part[2 * k];
+        c += part[2 * k + 1];
+      }
+      shrink = shrink + (float)(s / c);
+    }
+    shrink_sh = shrink;
+  }
+  __syncthreads();
+  const float shrink = shrink_sh;
+  const bool do_shrink = use_mask || shrinkage_rad > 0.f;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    float v = V[p];
+    if (do_shrink) v = v - shrink;
+    if (positivity) v = v < 0.f ? 0.f : v;
+    V[p] = v;
+  }
 }
 
 // probe centre of mass (corner-centred coordinates) and Fourier recentring
@@ -1073,7 +1140,11 @@ bool frame_supported(int S) { return S == 64 || S == 128; }
 int launch_fused(const FusedArgs& a, int S, hipStream_t stream) {
   const int grid = a.B - a.b0;
   if (grid <= 0) return PTY_OK;
-  if (a.fmask) {
+  if (a.object_pot) {  // potential object (no detector mask: the caller rejects the pair)
+    if (a.fmask) return fail(PTY_E_UNSUPPORTED, "detector masks with potential objects are not supported");
+    if (S == 64) hipLaunchKernelGGL((k_fused<64, false, true>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+    else hipLaunchKernelGGL((k_fused<128, false, true>), dim3(grid), dim3(Frame<128>::T), 0, stream, a);
+  } else if (a.fmask) {
     if (S == 64) hipLaunchKernelGGL((k_fused<64, true>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
     else hipLaunchKernelGGL((k_fused<128, true>), dim3(grid), dim3(Frame<128>::T), 0, stream, a);
   } else {
@@ -1130,6 +1201,15 @@ int launch_object_update(cf* obj, const cf* grad, const float* spsum, const floa
   hipLaunchKernelGGL(k_object_update, dim3(grid_for(npix, 256)), dim3(256), 0, stream, obj, grad, spsum,
                      tilemax, ntiles, npix, step, c1, c2, replace);
   return check_launch("k_object_update");
+}
+
+int launch_object_update_potential(float* V, const cf* grad, const float* spsum, const float* tilemax, int ntiles,
+                                   int Px, int Py, float step, float nmin, hipStream_t stream) {
+  const long long npix = (long long)Px * Py;
+  const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
+  hipLaunchKernelGGL(k_object_update_potential, dim3(grid_for(npix, 256)), dim3(256), 0, stream, V, grad, spsum,
+                     tilemax, ntiles, npix, step, c1, c2);
+  return check_launch("k_object_update_potential");
 }
 
 __host__ __device__ inline int probe_nchunk(int B) { return (B + PROBE_CHUNK - 1) / PROBE_CHUNK; }
@@ -1258,6 +1338,17 @@ int launch_object_gaussian(cf* obj, int Px, int Py, const double* w, int r, int 
     hipLaunchKernelGGL((k_gaussian_axis<cf, 0>), grid, blk, 0, stream, obj, t, Px, Py, w, r);
     hipLaunchKernelGGL((k_gaussian_axis<cf, 1>), grid, blk, 0, stream, t, obj, Px, Py, w, r);
   }
+  return check_launch("k_gaussian_axis");
+}
+
+// a real array (the potential object): both axes on the float32 values; tmp:
+// Px * Py * 4 bytes
+int launch_gaussian_real(float* V, int Px, int Py, const double* w, int r, void* tmp, hipStream_t stream) {
+  const long long npix = (long long)Px * Py;
+  const dim3 grid(grid_for(npix, 256)), blk(256);
+  float* t = (float*)tmp;
+  hipLaunchKernelGGL((k_gaussian_axis<float, 0>), grid, blk, 0, stream, V, t, Px, Py, w, r);
+  hipLaunchKernelGGL((k_gaussian_axis<float, 1>), grid, blk, 0, stream, t, V, Px, Py, w, r);
   return check_launch("k_gaussian_axis");
 }
 
@@ -1440,13 +1531,27 @@ int launch_constraints(cf* obj, int Px, int Py, const unsigned char* mask, float
   const long long npix = (long long)Px * Py;
   if (mask) {
     hipLaunchKernelGGL(k_constraint_partial, dim3(CONSTRAINT_BLOCKS), dim3(256), 0, stream, obj, mask, npix,
-                       part);
+                       part, (const float*)nullptr);
     int rc = check_launch("k_constraint_partial");
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_constraint_apply, dim3(grid_for(npix, 256)), dim3(256), 0, stream, obj, npix, part,
                      CONSTRAINT_BLOCKS, mask ? 1 : 0, shrink, pure);
   return check_launch("k_constraint_apply");
+}
+
+int launch_constraints_potential(float* V, int Px, int Py, const unsigned char* mask, float shrink, int positivity,
+                                 double* part, hipStream_t stream) {
+  const long long npix = (long long)Px * Py;
+  if (mask) {
+    hipLaunchKernelGGL(k_constraint_partial, dim3(CONSTRAINT_BLOCKS), dim3(256), 0, stream, (const cf*)nullptr, mask,
+                       npix, part, (const float*)V);
+    int rc = check_launch("k_constraint_partial");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_constraint_apply_potential, dim3(grid_for(npix, 256)), dim3(256), 0, stream, V, npix, part,
+                     CONSTRAINT_BLOCKS, mask ? 1 : 0, shrink, positivity);
+  return check_launch("k_constraint_apply_potential");
 }
 
 int launch_probe_com(cf* probe, int S, hipStream_t stream) {
@@ -1578,7 +1683,10 @@ int validate_problem(const pty_problem* p) {
   if (p->N <= 0) return fail(PTY_E_ARG, "no scan positions");
   if (!p->object || !p->probe || !p->positions || !p->amplitudes)
     return fail(PTY_E_ARG, "null device buffer in problem");
-  if (!aligned8(p->object) || !aligned8(p->probe)) return fail(PTY_E_ARG, "complex buffers must be 8-byte aligned");
+  if (p->object_type != PTY_OBJECT_COMPLEX && p->object_type != PTY_OBJECT_POTENTIAL)
+    return fail(PTY_E_ARG, "object_type %d is neither PTY_OBJECT_COMPLEX nor PTY_OBJECT_POTENTIAL", p->object_type);
+  if (!aligned8(p->probe) || (p->object_type == PTY_OBJECT_COMPLEX && !aligned8(p->object)))
+    return fail(PTY_E_ARG, "complex buffers must be 8-byte aligned");
   return PTY_OK;
 }
 
@@ -1605,6 +1713,9 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
   if (!orders || !errors || !workspace) return fail(PTY_E_ARG, "null orders/errors/workspace");
   if ((rc = validate_position_params(pp, p.positions))) return rc;
   const bool correct = pp != nullptr;
+  const bool pot = p.object_type == PTY_OBJECT_POTENTIAL;
+  if (pot && (correct || p.fourier_mask || q.butterworth))
+    return fail(PTY_E_UNSUPPORTED, "potential objects: no position correction, detector mask or Butterworth filter");
   const WsLayout L = layout(p, q, correct);
   if (workspace_bytes < L.total)
     return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.total);
@@ -1621,7 +1732,8 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
   double* runbuf = (double*)(ws + L.run);
   cf* far = correct ? (cf*)(ws + L.far) : nullptr;
   const int stage = stage_for(q, correct);
-  cf* obj = (cf*)p.object;
+  cf* obj = pot ? nullptr : (cf*)p.object;
+  float* V = pot ? (float*)p.object : nullptr;
   cf* probe = (cf*)p.probe;
   const std::vector<int> batches = subdivide(p.N, q.max_batch);
   const int ntiles = num_tiles(p.Px, p.Py);
@@ -1639,7 +1751,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
       for (int s0 = 0; s0 < nb; s0 += stage) {
         const int ns = (nb - s0) < stage ? (nb - s0) : stage;
         FusedArgs a{};
-        a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
+        a.probe_hat = phat; a.object = obj; a.object_pot = V; a.positions = p.positions; a.amps = p.amplitudes;
         a.order = ob + s0; a.N = p.N; a.B = ns; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
         a.ps_out = ps; a.chi_out = chi; a.err = err + start + s0; a.fwd_out = far;
         a.fmask = p.fourier_mask;
@@ -1647,6 +1759,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
         if (!q.fix_probe) {
           ProbePartialArgs pa{chi, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
                               ppart + (size_t)(s0 / PROBE_CHUNK) * p.S * p.S * 3};
+          pa.object_pot = V;
           hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(ns)), dim3(256), 0,
                              stream, pa);
           if ((rc = check_launch("k_probe_partial"))) return rc;
@@ -1662,7 +1775,11 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
         }
         if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
       }
-      if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
+      if (pot) {
+        if ((rc = launch_object_update_potential(V, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
+                                                 q.normalization_min, stream)))
+          return rc;
+      } else if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
                                      q.normalization_min, stream)))
         return rc;
       if (!q.fix_probe) {
@@ -1676,8 +1793,16 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
     }
     hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
     if ((rc = check_launch("k_error_reduce"))) return rc;
-    if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, cpart, stream))) return rc;
-    if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
+    if (pot) {
+      if (q.gaussian_weights && q.gaussian_radius >= 0 &&
+          (rc = launch_gaussian_real(V, p.Px, p.Py, q.gaussian_weights, q.gaussian_radius, grad, stream)))
+        return rc;
+      if ((rc = launch_constraints_potential(V, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.object_positivity,
+                                             cpart, stream)))
+        return rc;
+    } else if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, cpart, stream))) {
+      return rc;
+    } else if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
                                  stream)))
       return rc;
     if (!q.fix_probe && q.fix_probe_com) {
@@ -1702,6 +1827,8 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
   if (!params || !jp) return fail(PTY_E_ARG, "params are NULL");
   const pty_problem& p = *problem;
   const pty_gd_params& q = *params;
+  if (p.object_type != PTY_OBJECT_COMPLEX)
+    return fail(PTY_E_UNSUPPORTED, "potential objects run through pty_gd_run (gradient descent)");
   if (q.num_iter < 0 || q.max_batch != p.N)
     return fail(PTY_E_ARG, "projection sets update from all %d positions at once (max_batch=%d)", p.N, q.max_batch);
   if (!order || !errors || !workspace || !jp->exit_waves) return fail(PTY_E_ARG, "null order/errors/workspace/exit waves");
@@ -1826,6 +1953,8 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
   if (!params) return fail(PTY_E_ARG, "params is NULL");
   const pty_problem& p = *problem;
   const pty_gd_params& q = *params;
+  if (p.object_type != PTY_OBJECT_COMPLEX)
+    return fail(PTY_E_UNSUPPORTED, "potential objects run through pty_gd_run (gradient descent)");
   if (K < 1 || K > 64) return fail(PTY_E_ARG, "num_probes=%d out of range 1..64", K);
   if (q.num_iter < 0 || q.max_batch <= 0 || q.max_batch > p.N)
     return fail(PTY_E_ARG, "bad num_iter=%d / max_batch=%d for N=%d", q.num_iter, q.max_batch, p.N);

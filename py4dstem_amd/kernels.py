@@ -194,14 +194,19 @@ def probe_com_constraint(probe):
 
 def make_problem(obj, probe, positions, amplitudes_k, mask_inverse, mean_intensity,
                  fourier_mask_k=None):
+    """pty_problem for device tensors; a float32 object is a potential
+    (PTY_OBJECT_POTENTIAL), a complex64 one a transmission."""
     S = probe.shape[-1]
     Px, Py = obj.shape
+    if obj.dtype not in (torch.complex64, torch.float32):
+        raise TypeError(f"object must be complex64 or float32 (potential), not {obj.dtype}")
     return PtyProblem(S, Px, Py, positions.shape[0], c_void_p(obj.data_ptr()),
                       c_void_p(probe.data_ptr()), c_void_p(positions.data_ptr()),
                       c_void_p(amplitudes_k.data_ptr()),
                       c_void_p(mask_inverse.data_ptr()) if mask_inverse is not None else None,
                       float(mean_intensity),
-                      c_void_p(fourier_mask_k.data_ptr()) if fourier_mask_k is not None else None)
+                      c_void_p(fourier_mask_k.data_ptr()) if fourier_mask_k is not None else None,
+                      1 if obj.dtype == torch.float32 else 0)
 
 
 def gd_workspace_bytes(problem, params):

@@ -138,6 +138,9 @@ class MixedstatePtychography(SingleslicePtychography):
         if not fix_positions:
             raise NotImplementedError("position correction with several probe modes is not "
                                       "supported on this path")
+        if self._object_type == "potential":
+            raise NotImplementedError("potential objects with several probe modes are not "
+                                      "supported on this path")
         run = self._prepare_run(
             num_iter=num_iter, reconstruction_method=reconstruction_method,
             max_batch_size=max_batch_size, seed_random=seed_random, step_size=step_size,

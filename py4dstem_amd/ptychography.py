@@ -248,8 +248,6 @@ class SingleslicePtychography:
                 raise NotImplementedError(f"{opt} (resampling/padding) is not supported")
         if crop_patterns:
             raise NotImplementedError("crop_patterns is not supported")
-        if self._object_type != "complex":
-            raise NotImplementedError("only object_type='complex' is supported")
 
         dc = self._datacube
         Rshape = tuple(dc.data.shape[:2])
@@ -332,9 +330,11 @@ class SingleslicePtychography:
         if self._object is None:
             shape = initial_object_shape(pos, self._object_padding_px, self._region_of_interest_shape,
                                          self._object_fov_ang, self.sampling)
-            self._object = np.ones(shape, dtype=np.complex64)
+            # ptychographic_methods.py:40-79: zero potential or unit transmission
+            self._object = (np.zeros(shape, dtype=np.float32) if self._object_type == "potential"
+                            else np.ones(shape, dtype=np.complex64))
         else:
-            self._object = np.array(self._object, dtype=np.complex64, copy=True)
+            self._object = np.array(self._object, dtype=self._object_dtype(), copy=True)
         if self._object.shape[0] < S or self._object.shape[1] < S:
             raise ValueError(f"object {self._object.shape} is smaller than the {S}x{S} probe window")
         if store_initial_arrays:
@@ -381,7 +381,8 @@ class SingleslicePtychography:
             del self._probe_dev
         # the starting object stays resident: resets and the learned
         # initialiser's judge use device copies
-        self._object_dev = torch.from_numpy(np.ascontiguousarray(self._object, dtype=np.complex64)).to(dev)
+        self._object_dev = torch.from_numpy(np.ascontiguousarray(self._object,
+                                                                 dtype=self._object_dtype())).to(dev)
         if store_initial_arrays:
             self._object_initial_dev = self._object_dev.clone()
         return self
@@ -413,6 +414,10 @@ class SingleslicePtychography:
                 raise AttributeError("_amplitudes: preprocess() has not run")
             self._amplitudes_host = K.to_kernel_layout(self._amplitudes_dev, inverse=True).cpu().numpy()
         return self._amplitudes_host
+
+    def _object_dtype(self):
+        """float32 potential or complex64 transmission (ptychographic_methods.py:69-77)."""
+        return np.float32 if self._object_type == "potential" else np.complex64
 
     def _initialize_probe(self, initial_probe):
         """ptychographic_methods.py:1011-1102."""
@@ -477,6 +482,8 @@ class SingleslicePtychography:
         preprocess() and before reconstruct(); the choice and timings are in
         ``learned_init_info``.  Returns self."""
         from . import learned_init
+        if self._object_type == "potential":
+            raise NotImplementedError("the learned initialiser predicts complex transmissions")
         if not getattr(self, "_preprocessed", False):
             raise ValueError("preprocess() must run before initialize_object_learned()")
         if model is None or isinstance(model, str):
@@ -669,8 +676,8 @@ class SingleslicePtychography:
 
         dev = require_gpu()
         if not hasattr(self, "_object_dev"):
-            self._object_dev = torch.from_numpy(np.ascontiguousarray(self._object,
-                                                                     dtype=np.complex64)).to(dev)
+            self._object_dev = torch.from_numpy(np.ascontiguousarray(
+                self._object, dtype=self._object_dtype())).to(dev)
         if not hasattr(self, "_probe_dev"):
             self._probe_dev = torch.from_numpy(np.ascontiguousarray(self._probe,
                                                                     dtype=np.complex64)).to(dev)
@@ -686,14 +693,29 @@ class SingleslicePtychography:
             # corner-centred like the amplitudes, stored in their kernel layout
             fmask_dev = K.to_kernel_layout(
                 torch.from_numpy(np.ascontiguousarray(fmask[None])).to(dev))[0].contiguous()
+        potential = self._object_type == "potential"
+        if potential:
+            # the potential forward/adjoint and constraints (ptychographic_methods.py:
+            # 1522-1523, 1868-1880; ptychographic_constraints.py:60-118) run in
+            # gradient descent with fixed positions
+            for what, used in (("projection-set methods", projection is not None),
+                               ("position correction", not fix_positions),
+                               ("detector_fourier_mask", detector_fourier_mask is not None),
+                               ("q_lowpass/q_highpass", butterworth_filter and (
+                                   q_lowpass is not None or q_highpass is not None))):
+                if used:
+                    raise NotImplementedError(f"{what} with object_type='potential' is not "
+                                              "supported on this path")
         problem = K.make_problem(self._object_dev, self._probe_dev, self._positions_dev,
                                  self._amplitudes_dev, mask_dev, self._mean_diffraction_intensity,
                                  fmask_dev)
         per_call = 1 if (store_iterations or progress_bar or affine) else max(num_iter, 1)
         params = PtyGdParams(per_call, B, int(bool(fix_probe)), int(bool(fix_probe_com)),
-                             int(bool(pure_phase_object)), float(step_size),
+                             # singleslice_ptychography.py:976
+                             int(bool(pure_phase_object) and not potential), float(step_size),
                              float(normalization_min), float(shrinkage_rad),
                              int(self._stage_for(B)))
+        params.object_positivity = int(bool(object_positivity))
         aperture_dev = None
         if fix_probe_aperture and not fix_probe:
             # |fft2| of the initial probe (mode 0 with several modes), in the

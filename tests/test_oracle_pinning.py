@@ -943,3 +943,44 @@ def test_fit_origin_models_match_curve_fit():
     assert np.array_equal(fit_origin(d, mask, "constant"), po.fit_origin_2d(d, mask, "constant"))
     with pytest.raises(AssertionError):
         fit_origin(d, mask, "cubic")
+
+
+def test_potential_object_closed_forms(tiny):
+    """Potential objects (ptychographic_methods.py:40-79, 1522-1523,
+    1868-1880; ptychographic_constraints.py:60-118): the forward model uses
+    exp(i V), so a potential V and the complex object exp(i V) give the same
+    exit-wave update and error; the potential's gradient step is the real
+    part of -i conj(exp(i V)) times the complex step (for a pixel covered by
+    one position it is exactly that product); shrinkage subtracts the
+    masked mean, positivity clips at zero."""
+    o, probe, obj, pos = tiny
+    rng = np.random.default_rng(6)
+    V = (0.3 * rng.standard_normal(obj.shape)).astype(np.float32)
+    Oc = np.exp(1j * V).astype(np.complex64)
+    amps = np.abs(np.fft.fft2(o.forward(Oc, probe, pos, np.zeros((len(pos), 16, 16),
+                                                                 np.float32))[0] * 1.1))
+    amps = amps.astype(np.float32)
+    o._object_type = "complex"
+    spc, opc, ewc, errc = o.forward(Oc, probe, pos, amps)
+    o._object_type = "potential"
+    spv, opv, ewv, errv = o.forward(V, probe, pos, amps)
+    np.testing.assert_allclose(opv, opc, atol=1e-6)
+    np.testing.assert_allclose(ewv, ewc, atol=1e-5 * np.abs(ewc).max())
+    assert errv == pytest.approx(errc, rel=1e-5)
+    # one position: the potential update is Re(-i conj(O) g) * n * step per pixel
+    p1 = pos[:1]
+    sp, op, ew, _ = o.forward(V, probe, p1, amps[:1])
+    Vn, _ = o.adjoint(V.copy(), probe.copy(), op, sp, p1, ew, 0.5, 1.0, True)
+    o._object_type = "complex"
+    Oc2, _ = o.adjoint(Oc.copy(), probe.copy(), op, sp, p1, ew, 0.5, 1.0, True)
+    dO = (Oc2 - Oc).astype(np.complex128)
+    np.testing.assert_allclose(Vn - V, np.real(-1j * np.conj(Oc) * dO), atol=2e-6)
+    mask = np.zeros(V.shape, bool)
+    mask[:5] = True
+    out = po.OracleSingleslicePtychography.object_constraints(
+        V.copy(), mask, 0.1, False, None, None, "potential", False)
+    np.testing.assert_allclose(out, V - (0.1 + V[mask].mean()), atol=1e-6)
+    assert abs(out[mask].mean() + 0.1) < 1e-6
+    clipped = po.OracleSingleslicePtychography.object_constraints(
+        V.copy(), None, 0.0, False, None, None, "potential", True)
+    assert np.array_equal(clipped, np.clip(V, 0, None))
