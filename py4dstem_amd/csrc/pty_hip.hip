@@ -506,6 +506,37 @@ __global__ __launch_bounds__(256) void k_object_update(cf* __restrict__ obj, con
   object_update_body(obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, blockIdx.x, gridDim.x, replace);
 }
 
+// the object step of a mini-batch and, in work group 0 of the same launch,
+// fft2 of the just-updated probe for the next mini-batch's fused pass (the
+// two are independent; the same arithmetic as k_object_update and k_fft2)
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_object_update_fft(cf* __restrict__ obj, const cf* __restrict__ grad,
+                                                                  const float* __restrict__ spsum,
+                                                                  const float* __restrict__ tilemax, int ntiles,
+                                                                  long long npix, float step, float c1, float c2,
+                                                                  const cf* __restrict__ probe,
+                                                                  cf* __restrict__ phat) {
+  if (blockIdx.x == 0) {
+    fft2_body<S, false>(probe, phat, 0, 0);
+    return;
+  }
+  constexpr int T = Frame<S>::T;
+  __shared__ float sh[T];
+  const int tid = threadIdx.x;
+  float m = 0.f;
+  for (int i = tid; i < ntiles; i += T) m = fmaxf(m, tilemax[i]);
+  sh[tid] = m;
+  __syncthreads();
+  for (int s = T / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] = fmaxf(sh[tid], sh[tid + s]);
+    __syncthreads();
+  }
+  m = sh[0];
+  const long long nblk = gridDim.x - 1;
+  for (long long p = (long long)(blockIdx.x - 1) * T + tid; p < npix; p += nblk * T)
+    obj[p] = update_value(obj[p], grad[p], overlap_norm(spsum[p], m, c1, c2), step, 0);
+}
+
 // mixed state (ptychographic_methods.py:2945-2975): the mode overlaps summed
 // in float32, modes in order, with a per-block maximum
 __global__ __launch_bounds__(256) void k_mixed_overlap(const float* __restrict__ spsum, int K, long long npix,
@@ -606,6 +637,16 @@ __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, in
 
 __global__ __launch_bounds__(256) void k_probe_partial(ProbePartialArgs a) {
   probe_partial_body(a, blockIdx.x, blockIdx.y);
+}
+
+// the scatter of a stage (work groups [0, ntiles)) and its probe partial
+// sums (the rest, pixel block fastest) in one launch: both only read the
+// fused pass's staged terms
+__global__ __launch_bounds__(256) void k_gather_probe(GatherArgs g, ProbePartialArgs pa, int ntiles,
+                                                      int pix_blocks) {
+  const int b = (int)blockIdx.x;
+  if (b < ntiles) gather_body(g, b);
+  else probe_partial_body(pa, (b - ntiles) % pix_blocks, (b - ntiles) / pix_blocks);
 }
 
 // per-pixel sum of the chunk partials: PROBE_REDUCE_GROUPS contiguous chunk
@@ -1203,6 +1244,19 @@ int launch_object_update(cf* obj, const cf* grad, const float* spsum, const floa
   return check_launch("k_object_update");
 }
 
+int launch_object_update_fft(cf* obj, const cf* grad, const float* spsum, const float* tilemax, int ntiles, int Px,
+                             int Py, float step, float nmin, const cf* probe, cf* phat, int S, hipStream_t stream) {
+  const long long npix = (long long)Px * Py;
+  const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
+  if (S == 64)
+    hipLaunchKernelGGL(k_object_update_fft<64>, dim3(1 + grid_for(npix, Frame<64>::T)), dim3(Frame<64>::T), 0, stream,
+                       obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, probe, phat);
+  else
+    hipLaunchKernelGGL(k_object_update_fft<128>, dim3(1 + grid_for(npix, Frame<128>::T)), dim3(Frame<128>::T), 0,
+                       stream, obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, probe, phat);
+  return check_launch("k_object_update_fft");
+}
+
 int launch_object_update_potential(float* V, const cf* grad, const float* spsum, const float* tilemax, int ntiles,
                                    int Px, int Py, float step, float nmin, hipStream_t stream) {
   const long long npix = (long long)Px * Py;
@@ -1739,6 +1793,10 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
   const int ntiles = num_tiles(p.Px, p.Py);
   const double norm = (double)(float)(p.mean_intensity * (float)p.N);
   bool phat_valid = false;
+  // complex object with probe update: the object step carries the fft2 of
+  // the updated probe (k_object_update_fft), so phat stays current across
+  // the mini-batches of an iteration
+  const bool fft_with_update = !q.fix_probe && !pot;
   for (int it = 0; it < q.num_iter; ++it) {
     const int* order_it = orders + (size_t)it * p.N;
     int start = 0;
@@ -1746,7 +1804,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
       const int* ob = order_it + start;
       if (!phat_valid) {
         if ((rc = launch_fft(probe, phat, 1, p.S, false, 0, stream))) return rc;
-        phat_valid = q.fix_probe != 0;
+        phat_valid = q.fix_probe != 0 || fft_with_update;
       }
       for (int s0 = 0; s0 < nb; s0 += stage) {
         const int ns = (nb - s0) < stage ? (nb - s0) : stage;
@@ -1756,14 +1814,6 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
         a.ps_out = ps; a.chi_out = chi; a.err = err + start + s0; a.fwd_out = far;
         a.fmask = p.fourier_mask;
         if ((rc = launch_fused(a, p.S, stream))) return rc;
-        if (!q.fix_probe) {
-          ProbePartialArgs pa{chi, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
-                              ppart + (size_t)(s0 / PROBE_CHUNK) * p.S * p.S * 3};
-          pa.object_pot = V;
-          hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(ns)), dim3(256), 0,
-                             stream, pa);
-          if ((rc = check_launch("k_probe_partial"))) return rc;
-        }
         GatherArgs g{};
         g.ps = ps; g.chi = chi; g.positions = p.positions; g.order = ob + s0; g.N = p.N; g.B = ns; g.S = p.S;
         g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
@@ -1773,18 +1823,34 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
           g.run_first = s0 == 0;
           g.run_last = s0 + ns == nb;
         }
-        if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+        if (q.fix_probe) {
+          if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+        } else {
+          ProbePartialArgs pa{chi, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
+                              ppart + (size_t)(s0 / PROBE_CHUNK) * p.S * p.S * 3};
+          pa.object_pot = V;
+          const int pix_blocks = (p.S * p.S + 255) / 256;
+          hipLaunchKernelGGL(k_gather_probe, dim3(ntiles + pix_blocks * probe_nchunk(ns)), dim3(GATHER_THREADS), 0,
+                             stream, g, pa, ntiles, pix_blocks);
+          if ((rc = check_launch("k_gather_probe"))) return rc;
+        }
       }
-      if (pot) {
+      // the probe step first: the object step may carry the probe's fft2
+      if (!q.fix_probe) {
+        if ((rc = launch_probe_apply(probe, ppart, p.S, nb, q.step_size, q.normalization_min, stream))) return rc;
+      }
+      if (fft_with_update) {
+        if ((rc = launch_object_update_fft(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
+                                           q.normalization_min, probe, phat, p.S, stream)))
+          return rc;
+      } else if (pot) {
         if ((rc = launch_object_update_potential(V, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
                                                  q.normalization_min, stream)))
           return rc;
       } else if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, q.step_size,
                                      q.normalization_min, stream)))
         return rc;
-      if (!q.fix_probe) {
-        if ((rc = launch_probe_apply(probe, ppart, p.S, nb, q.step_size, q.normalization_min, stream))) return rc;
-      }
+      if (!q.fix_probe && !fft_with_update) phat_valid = false;
       if (correct) {  // singleslice_ptychography.py:913-926, after the adjoint
         const PosArgs pa = position_args(*pp, ps, far, obj, p.Px, p.Py, p.amplitudes, p.N, ob, nb);
         if ((rc = launch_position_correction(pa, p.S, stream))) return rc;
@@ -1809,6 +1875,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
       if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
     }
     if (!q.fix_probe && (rc = launch_probe_extra(probe, 1, p.S, q, stream))) return rc;
+    if (!q.fix_probe) phat_valid = false;  // the recentring and constraints moved the probe
     if (correct && pp->center_of_mass) {
       if ((rc = launch_positions_com(pp->positions, p.N, pp->initial_com, stream))) return rc;
     }
