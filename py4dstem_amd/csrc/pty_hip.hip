@@ -2557,13 +2557,13 @@ int pty_multi_run(const pty_multi_problem* problems, int count, int num_iter, in
       if (e2 == hipSuccess) e2 = hipStreamWaitEvent(stream, done, 0);
       if (e2 != hipSuccess) rc = fail(PTY_E_LAUNCH, "graph join: %s", hipGetErrorString(e2));
     }
-    hipStreamSynchronize(cs);
+    (void)hipStreamSynchronize(cs);
   } else {
     rc = fail(PTY_E_LAUNCH, "graph setup: %s", hipGetErrorString(e));
   }
-  if (exec) hipGraphExecDestroy(exec);
-  if (graph) hipGraphDestroy(graph);
-  if (ready) hipEventDestroy(ready);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (ready) (void)hipEventDestroy(ready);
   if (done) hipEventDestroy(done);
   if (cs) hipStreamDestroy(cs);
   return rc;
