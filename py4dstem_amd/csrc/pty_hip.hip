@@ -1941,13 +1941,6 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
       a.proj_a = (float)jp->a; a.proj_b = (float)jp->b; a.proj_c = (float)jp->c;
       a.proj_x = (float)x; a.proj_y = (float)y; a.exit_valid = valid;
       if ((rc = launch_projection(a, p.S, stream))) return rc;
-      if (!q.fix_probe) {
-        ProbePartialArgs pa{ex, obj, p.positions, order + s0, p.N, ns, p.S, p.Px, p.Py,
-                            ppart + (size_t)(s0 / PROBE_CHUNK) * S2 * 3};
-        hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(ns)), dim3(256), 0,
-                           stream, pa);
-        if ((rc = check_launch("k_probe_partial"))) return rc;
-      }
       GatherArgs g{};
       g.ps = ps; g.chi = ex; g.positions = p.positions; g.order = order + s0; g.N = p.N; g.B = ns; g.S = p.S;
       g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
@@ -1957,7 +1950,16 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
         g.run_first = s0 == 0;
         g.run_last = s0 + ns == p.N;
       }
-      if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+      if (q.fix_probe) {
+        if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+      } else {  // scatter and probe partial sums in one launch
+        ProbePartialArgs pa{ex, obj, p.positions, order + s0, p.N, ns, p.S, p.Px, p.Py,
+                            ppart + (size_t)(s0 / PROBE_CHUNK) * S2 * 3};
+        const int pix_blocks = (p.S * p.S + 255) / 256;
+        hipLaunchKernelGGL(k_gather_probe, dim3(ntiles + pix_blocks * probe_nchunk(ns)), dim3(GATHER_THREADS), 0,
+                           stream, g, pa, ntiles, pix_blocks);
+        if ((rc = check_launch("k_gather_probe"))) return rc;
+      }
     }
     valid = 1;
     if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, 0.f, q.normalization_min, stream,
@@ -2069,18 +2071,20 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
       for (int k = 0; k < K; ++k) {
         const cf* ps_k = ps + (size_t)k * nb * S2;
         const cf* chi_k = chi + (size_t)k * nb * S2;
-        if (!q.fix_probe) {
-          ProbePartialArgs pa{chi_k, obj, p.positions, ob, p.N, nb, p.S, p.Px, p.Py,
-                              (double*)(ws + L.ppart + k * L.pstride)};
-          hipLaunchKernelGGL(k_probe_partial, dim3((p.S * p.S + 255) / 256, probe_nchunk(nb)), dim3(256), 0,
-                             stream, pa);
-          if ((rc = check_launch("k_probe_partial"))) return rc;
-        }
         GatherArgs g{};
         g.ps = ps_k; g.chi = chi_k; g.positions = p.positions; g.order = ob; g.N = p.N; g.B = nb; g.S = p.S;
         g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad + (size_t)k * npix;
         g.spsum = spsum + (size_t)k * npix; g.tilemax = tilemax + (size_t)k * ntiles;
-        if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+        if (q.fix_probe) {
+          if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+        } else {  // the mode's scatter and probe partial sums in one launch
+          ProbePartialArgs pa{chi_k, obj, p.positions, ob, p.N, nb, p.S, p.Px, p.Py,
+                              (double*)(ws + L.ppart + k * L.pstride)};
+          const int pix_blocks = (p.S * p.S + 255) / 256;
+          hipLaunchKernelGGL(k_gather_probe, dim3(ntiles + pix_blocks * probe_nchunk(nb)), dim3(GATHER_THREADS), 0,
+                             stream, g, pa, ntiles, pix_blocks);
+          if ((rc = check_launch("k_gather_probe"))) return rc;
+        }
       }
       hipLaunchKernelGGL(k_mixed_overlap, dim3(nblk), dim3(256), 0, stream, spsum, K, npix, total, bmax);
       if ((rc = check_launch("k_mixed_overlap"))) return rc;
