@@ -179,6 +179,18 @@ int pty_shift_amplitudes(const float* intensities, int N_in, int S, const int* s
                          const float* shift_x, const float* shift_y, float* amplitudes,
                          double* sums, pty_stream_t stream);
 
+/* Resampling and padding of the patterns (phase_base_class.py:223-420:
+ * diffraction_intensities_shape with reshaping_method 'fourier', 'bilinear'
+ * or 'bin', then padded_diffraction_intensities_shape).  Each is linear
+ * along each axis, so pattern b (float32 [Qx][Qy]) becomes
+ * out[b] = Re(Mx in[b] My^T) (float32 [Sx][Sy]); Mx complex64 [Sx][Qx] and
+ * My complex64 [Sy][Qy] hold the per-axis maps (built on the host by applying
+ * the reference's one-dimensional operation to unit vectors).  `workspace`:
+ * pty_resample_workspace_bytes(batch, Qy, Sx); batch <= 65535 per call. */
+size_t pty_resample_workspace_bytes(int batch, int Qy, int Sx);
+int pty_resample_frames(const float* in, int batch, int Qx, int Qy, const void* Mx, const void* My,
+                        int Sx, int Sy, float* out, void* workspace, pty_stream_t stream);
+
 /* ---- whole reconstruction loop ------------------------------------------ */
 
 /* pty_problem.object_type */

@@ -332,6 +332,80 @@ def get_shifted_ar(ar, xshift, yshift):
     )
 
 
+def fourier_resample_4d(data, output_size):
+    """``fourier_resample`` of a 4-D stack with conserve_array_sums=True
+    (py4DSTEM/process/utils/utils.py:412-640): per pattern fft2, the four
+    low-frequency quadrants copied into a zero complex64 array of the new
+    size, ifft2, real part as float32."""
+    Qx, Qy = data.shape[-2:]
+    Sx, Sy = output_size
+
+    def sl(i, o):
+        if o == i:
+            return [(slice(None), slice(None))] * 2
+        m = min(i, o)
+        a, b = (m + 1) // 2, m // 2
+        return [(slice(0, a), slice(0, a)), (slice(o - b, o), slice(i - b, i))]
+
+    xs, ys = sl(Qx, Sx), sl(Qy, Sy)
+    out = np.zeros(data.shape[:2] + (Sx, Sy), np.float32)
+    for rx in range(data.shape[0]):
+        for ry in range(data.shape[1]):
+            spec = np.fft.fft2(data[rx, ry])
+            res = np.zeros((Sx, Sy), dtype=np.complex64)
+            for xo, xi in xs:
+                for yo, yi in ys:
+                    res[xo, yo] = spec[xi, yi]
+            out[rx, ry] = np.real(np.fft.ifft2(res)).astype(np.float32)
+    return out
+
+
+def resample_datacube(data, diffraction_intensities_shape=None, reshaping_method="bilinear",
+                      padded_diffraction_intensities_shape=None):
+    """phase_base_class.py:223-420 on the pattern stack: 'bin'
+    (py4DSTEM/preprocess/preprocess.py:155-215), 'fourier' (:636-663),
+    'bilinear' (:665-711: scipy zoom per pattern, order 1, mode 'nearest',
+    grid_mode, divided by the factor product), then zero padding (:719-790).
+    Returns (data, factor on the reciprocal pixel size)."""
+    factor = 1.0
+    data = np.asarray(data)
+    if diffraction_intensities_shape is not None:
+        Qx, Qy = data.shape[-2:]
+        Sx, Sy = diffraction_intensities_shape
+        rx, ry = Sx / Qx, Sy / Qy
+        if rx != ry:
+            raise ValueError("Datacube calibration can only handle uniform Q-sampling.")
+        if reshaping_method == "bin":
+            b = int(1 / rx)
+            d = data[:, :, : Qx - Qx % b, : Qy - Qy % b]
+            data = d.reshape(d.shape[0], d.shape[1], Qx // b, b, Qy // b, b).sum(
+                axis=(3, 5)).astype(data.dtype)
+            factor = float(b)
+        elif reshaping_method == "fourier":
+            data = fourier_resample_4d(data, (Sx, Sy))
+            factor = 1.0 / rx
+        elif reshaping_method == "bilinear":
+            from scipy.ndimage import zoom
+            r = np.array([rx, rx])
+            out = np.zeros(data.shape[:2] + tuple(np.round(r * np.array(data.shape[-2:])).astype(int)))
+            for i in range(data.shape[0]):
+                for j in range(data.shape[1]):
+                    out[i, j] = zoom(data[i, j].astype(np.float32), r, order=1, mode="nearest",
+                                     grid_mode=True)
+            data = out / r.prod()
+            factor = 1.0 / rx
+        else:
+            raise ValueError(f"reshaping_method needs to be one of 'bilinear', 'fourier', or 'bin', "
+                             f"not {reshaping_method}.")
+    if padded_diffraction_intensities_shape is not None:
+        Qx, Qy = data.shape[-2:]
+        Sx, Sy = padded_diffraction_intensities_shape
+        px, py = Sx - Qx, Sy - Qy
+        data = np.pad(data, ((0, 0), (0, 0), (px // 2, px // 2 + px % 2),
+                             (py // 2, py // 2 + py % 2)), mode="constant")
+    return data, factor
+
+
 def fit_origin_2d(data, mask=None, fitfunction="plane"):
     """``fit_origin`` (py4DSTEM/process/calibration/origin.py:76-190,
     non-robust) with the models of py4DSTEM/process/fit/fit.py:127-162.
@@ -575,9 +649,11 @@ class OracleSingleslicePtychography:
             for dk, n in zip(self._angular_sampling, self._amplitudes_shape)
         )
 
-    def _extract_intensities_and_calibrations(self, datacube):
-        """phase/phase_base_class.py:424-611 (calibrated branch)."""
-        intensities = np.asarray(datacube.data, dtype=np.float32)
+    def _extract_intensities_and_calibrations(self, datacube, data=None, q_factor=1.0):
+        """phase/phase_base_class.py:424-611 (calibrated branch); `data` and
+        `q_factor`: the resampled patterns and the factor on the reciprocal
+        pixel size the resampling leaves in the calibration."""
+        intensities = np.asarray(datacube.data if data is None else data, dtype=np.float32)
         self._grid_scan_shape = intensities.shape[:2]
         cal = datacube.calibration
         r_units = cal.get_R_pixel_units()
@@ -590,11 +666,11 @@ class OracleSingleslicePtychography:
         q_units = cal.get_Q_pixel_units()
         lam = electron_wavelength_angstrom(self._energy)
         if q_units == "A^-1":
-            rs = cal.get_Q_pixel_size()
+            rs = cal.get_Q_pixel_size() * q_factor
             self._reciprocal_sampling = (rs,) * 2
             self._angular_sampling = (rs * lam * 1e3,) * 2
         elif q_units == "mrad":
-            a = cal.get_Q_pixel_size()
+            a = cal.get_Q_pixel_size() * q_factor
             self._angular_sampling = (a,) * 2
             self._reciprocal_sampling = (a / lam / 1e3,) * 2
         else:
@@ -605,7 +681,9 @@ class OracleSingleslicePtychography:
     def preprocess(self, force_com_rotation=None, force_com_transpose=None,
                    force_com_shifts=None, fit_function="plane",
                    object_fov_mask=None, center_positions_in_fov=True,
-                   max_batch_size=None, **kwargs):
+                   max_batch_size=None, diffraction_intensities_shape=None,
+                   reshaping_method="bilinear", padded_diffraction_intensities_shape=None,
+                   **kwargs):
         """phase/singleslice_ptychography.py:191-604 (no plotting)."""
         mode = self._mode
         dc = self._datacube
@@ -614,7 +692,14 @@ class OracleSingleslicePtychography:
             # phase_base_class.py:277-282
             force_com_shifts = (np.ones(Rshape) * force_com_shifts[0],
                                 np.ones(Rshape) * force_com_shifts[1])
-        intensities = self._extract_intensities_and_calibrations(dc)
+        data, q_factor = None, 1.0
+        if diffraction_intensities_shape is not None or padded_diffraction_intensities_shape is not None:
+            data, q_factor = resample_datacube(dc.data, diffraction_intensities_shape, reshaping_method,
+                                               padded_diffraction_intensities_shape)
+            if diffraction_intensities_shape is not None and force_com_shifts is not None:
+                rf = diffraction_intensities_shape[0] / dc.data.shape[-2]  # :290-294
+                force_com_shifts = (force_com_shifts[0] * rf, force_com_shifts[1] * rf)
+        intensities = self._extract_intensities_and_calibrations(dc, data, q_factor)
         if self._semiangle_cutoff_pixels:
             self._semiangle_cutoff = self._semiangle_cutoff_pixels * self._angular_sampling[0]
 

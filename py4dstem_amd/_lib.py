@@ -87,6 +87,9 @@ _SIGNATURES = {
     "pty_probe_com_constraint": (c_int, [c_void_p, c_int, c_void_p]),
     "pty_object_gaussian": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_void_p,
                                     c_void_p]),
+    "pty_resample_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "pty_resample_frames": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
+                                    c_void_p, c_void_p, c_void_p]),
     "pty_object_butterworth": (c_int, [c_void_p, c_int, c_int, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        c_void_p, c_void_p, c_void_p]),

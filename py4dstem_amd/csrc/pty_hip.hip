@@ -1471,6 +1471,94 @@ __global__ __launch_bounds__(256) void k_dft(const cf* __restrict__ X, cf* __res
     }
 }
 
+// ---------------------------------------------------------------------------
+// resampling / padding of the patterns (phase_base_class.py:223-420): every
+// reshaping method there (Fourier crop or embed, order-1 zoom, binning, zero
+// padding) is linear along each axis, so a pattern X (Qx x Qy, real) becomes
+// Re(Mx X My^T) with complex Mx (Sx x Qx), My (Sy x Qy) built on the host.
+// Two batched products (blockIdx.z = pattern), 32 x 32 output tiles.
+// ---------------------------------------------------------------------------
+// T[b] = Mx (m x k) X[b] (k x n): complex m x n
+__global__ __launch_bounds__(256) void k_lin_left(const cf* __restrict__ Mx, const float* __restrict__ X,
+                                                  cf* __restrict__ T, int m, int k, int n) {
+  __shared__ cf As[DFT_T][DFT_T + 1];
+  __shared__ float Bs[DFT_T][DFT_T + 1];
+  const float* Xb = X + (size_t)blockIdx.z * k * n;
+  cf* Tb = T + (size_t)blockIdx.z * m * n;
+  const int i0 = blockIdx.y * DFT_T, j0 = blockIdx.x * DFT_T;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float ar[2][2] = {{0.f, 0.f}, {0.f, 0.f}}, ai[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  for (int k0 = 0; k0 < k; k0 += DFT_T) {
+    for (int e = threadIdx.x; e < DFT_T * DFT_T; e += 256) {
+      const int r = e / DFT_T, c = e % DFT_T;
+      const int gi = i0 + r, gk = k0 + c;
+      As[r][c] = (gi < m && gk < k) ? Mx[(size_t)gi * k + gk] : mkcf(0.f, 0.f);
+      const int hk = k0 + r, gj = j0 + c;
+      Bs[r][c] = (hk < k && gj < n) ? Xb[(size_t)hk * n + gj] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int q = 0; q < DFT_T; ++q) {
+      const cf a[2] = {As[2 * ty][q], As[2 * ty + 1][q]};
+      const float b[2] = {Bs[q][2 * tx], Bs[q][2 * tx + 1]};
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          ar[u][v] += a[u].x * b[v];
+          ai[u][v] += a[u].y * b[v];
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int gi = i0 + 2 * ty + u, gj = j0 + 2 * tx + v;
+      if (gi < m && gj < n) Tb[(size_t)gi * n + gj] = mkcf(ar[u][v], ai[u][v]);
+    }
+}
+
+// Y[b] = Re(T[b] (m x k) My^T), My n x k: real m x n
+__global__ __launch_bounds__(256) void k_lin_right(const cf* __restrict__ T, const cf* __restrict__ My,
+                                                   float* __restrict__ Y, int m, int k, int n) {
+  __shared__ cf As[DFT_T][DFT_T + 1];
+  __shared__ cf Bs[DFT_T][DFT_T + 1];
+  const cf* Tb = T + (size_t)blockIdx.z * m * k;
+  float* Yb = Y + (size_t)blockIdx.z * m * n;
+  const int i0 = blockIdx.y * DFT_T, j0 = blockIdx.x * DFT_T;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float acc[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+  for (int k0 = 0; k0 < k; k0 += DFT_T) {
+    for (int e = threadIdx.x; e < DFT_T * DFT_T; e += 256) {
+      const int r = e / DFT_T, c = e % DFT_T;
+      const int gi = i0 + r, gk = k0 + c;
+      As[r][c] = (gi < m && gk < k) ? Tb[(size_t)gi * k + gk] : mkcf(0.f, 0.f);
+      const int hk = k0 + r, gj = j0 + c;  // Bs[q][j] = My[j][q]
+      Bs[r][c] = (hk < k && gj < n) ? My[(size_t)gj * k + hk] : mkcf(0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int q = 0; q < DFT_T; ++q) {
+      const cf a[2] = {As[2 * ty][q], As[2 * ty + 1][q]};
+      const cf b[2] = {Bs[q][2 * tx], Bs[q][2 * tx + 1]};
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] += a[u].x * b[v].x - a[u].y * b[v].y;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int gi = i0 + 2 * ty + u, gj = j0 + 2 * tx + v;
+      if (gi < m && gj < n) Yb[(size_t)gi * n + gj] = acc[u][v];
+    }
+}
+
 // float64 partial sums of the object (re, im) per block, fixed grid
 constexpr int MEAN_BLOCKS = 256;
 static_assert(MEAN_BLOCKS <= CONSTRAINT_BLOCKS, "mean partials share the constraint partial buffer");
@@ -2223,6 +2311,26 @@ int pty_object_butterworth(void* object, int Px, int Py, double q_lowpass, doubl
     return fail(PTY_E_ARG, "bad butterworth filter arguments");
   return launch_object_butterworth((cf*)object, Px, Py, q_lowpass, q_highpass, butterworth_order, sampling_x,
                                    sampling_y, scratch, (double*)partials, st(stream));
+}
+
+size_t pty_resample_workspace_bytes(int batch, int Qy, int Sx) {
+  return batch > 0 && Qy > 0 && Sx > 0 ? (size_t)batch * Sx * Qy * 8 : 0;
+}
+
+int pty_resample_frames(const float* in, int batch, int Qx, int Qy, const void* Mx, const void* My, int Sx, int Sy,
+                        float* out, void* workspace, pty_stream_t stream) {
+  if (!in || !out || !Mx || !My || !workspace || batch < 0 || Qx <= 0 || Qy <= 0 || Sx <= 0 || Sy <= 0 ||
+      batch > 65535)
+    return fail(PTY_E_ARG, "bad resampling arguments");
+  if (batch == 0) return PTY_OK;
+  cf* T = (cf*)workspace;
+  hipLaunchKernelGGL(k_lin_left, dim3((Qy + DFT_T - 1) / DFT_T, (Sx + DFT_T - 1) / DFT_T, batch), dim3(256), 0,
+                     st(stream), (const cf*)Mx, in, T, Sx, Qx, Qy);
+  int rc = check_launch("k_lin_left");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_lin_right, dim3((Sy + DFT_T - 1) / DFT_T, (Sx + DFT_T - 1) / DFT_T, batch), dim3(256), 0,
+                     st(stream), (const cf*)T, (const cf*)My, out, Sx, Qy, Sy);
+  return check_launch("k_lin_right");
 }
 
 int pty_probe_com_constraint(void* probe, int S, pty_stream_t stream) {

@@ -301,3 +301,21 @@ def bilinear_shift(ar, xshift, yshift):
             + np.roll(ar, (xf + 1, yf), axis=(0, 1)) * (wx * (1 - wy))
             + np.roll(ar, (xf, yf + 1), axis=(0, 1)) * ((1 - wx) * wy)
             + np.roll(ar, (xf + 1, yf + 1), axis=(0, 1)) * (wx * wy))
+
+
+class ScaledQCalibration:
+    """A datacube's calibration with the reciprocal pixel size multiplied by
+    `factor`: the calibration the reference's resampling leaves on the
+    datacube (preprocess.py:209-213, 660-663, 708-710), without modifying the
+    caller's object."""
+
+    def __init__(self, datacube, factor):
+        self._cal = datacube.calibration
+        self._factor = float(factor)
+        self.calibration = self
+
+    def get_Q_pixel_size(self):
+        return self._cal.get_Q_pixel_size() * self._factor
+
+    def __getattr__(self, name):
+        return getattr(self._cal, name)

@@ -26,7 +26,9 @@ import torch
 
 from . import kernels as K
 from ._lib import PtyGdParams, require_gpu
+from .resample import axis_maps, resample_frames
 from .preprocess import (
+    ScaledQCalibration,
     bilinear_shift,
     center_of_mass,
     center_positions,
@@ -241,11 +243,13 @@ class SingleslicePtychography:
         if self._datacube is None:
             raise ValueError("The preprocess() method requires a DataCube. "
                              "Please run ptycho.attach_datacube(DataCube) first.")
-        for opt, val in (("diffraction_intensities_shape", diffraction_intensities_shape),
-                         ("padded_diffraction_intensities_shape", padded_diffraction_intensities_shape),
-                         ("region_of_interest_shape", region_of_interest_shape)):
-            if val is not None:
-                raise NotImplementedError(f"{opt} (resampling/padding) is not supported")
+        if region_of_interest_shape is not None:
+            raise NotImplementedError("region_of_interest_shape (resampled exit waves) is not supported")
+        reshape = (diffraction_intensities_shape is not None
+                   or padded_diffraction_intensities_shape is not None)
+        if reshape and (self._vacuum_probe_intensity is not None or dp_mask is not None):
+            raise NotImplementedError("resampling or padding together with vacuum_probe_intensity or "
+                                      "dp_mask is not supported")
         if crop_patterns:
             raise NotImplementedError("crop_patterns is not supported")
 
@@ -262,13 +266,27 @@ class SingleslicePtychography:
 
         intensities = np.asarray(dc.data, dtype=np.float32)
         self._grid_scan_shape = intensities.shape[:2]
+        cal_source = dc
+        if reshape:
+            # phase_base_class.py:223-420: resample, then pad, every pattern;
+            # forced centres scale with the resampling (:277-300)
+            maps = axis_maps(intensities.shape[-2:], diffraction_intensities_shape,
+                             reshaping_method, padded_diffraction_intensities_shape)
+            cal_source = ScaledQCalibration(dc, maps[2])
+            if diffraction_intensities_shape is not None:
+                rf = diffraction_intensities_shape[0] / intensities.shape[-2]
+                if force_com_shifts is not None:
+                    force_com_shifts = (force_com_shifts[0] * rf, force_com_shifts[1] * rf)
+                if force_com_measured is not None:
+                    force_com_measured = (force_com_measured[0] * rf, force_com_measured[1] * rf)
         (self._scan_sampling, self._angular_sampling,
          self._reciprocal_sampling) = extract_calibrations(
-            dc, self._energy, True, force_scan_sampling, force_angular_sampling,
+            cal_source, self._energy, True, force_scan_sampling, force_angular_sampling,
             force_reciprocal_sampling, self._verbose)
-        if intensities.shape[-1] != intensities.shape[-2]:
+        shape = (maps[0].shape[0], maps[1].shape[0]) if reshape else intensities.shape[-2:]
+        if shape[0] != shape[1]:
             raise NotImplementedError("only square diffraction patterns are supported")
-        S = intensities.shape[-1]
+        S = int(shape[0])
         K.check_frame(S)
         if self._semiangle_cutoff_pixels:
             self._semiangle_cutoff = self._semiangle_cutoff_pixels * self._angular_sampling[0]
@@ -276,7 +294,12 @@ class SingleslicePtychography:
         # the patterns go to the GPU once; centre of mass and normalisation
         # run there (phase_base_class.py:613-767, 1347-1460)
         dev = require_gpu()
-        frames_dev = torch.from_numpy(np.ascontiguousarray(intensities).reshape(-1, S, S)).to(dev)
+        Qx, Qy = intensities.shape[-2:]
+        frames_dev = torch.from_numpy(np.ascontiguousarray(intensities).reshape(-1, Qx, Qy)).to(dev)
+        if reshape:
+            frames_dev = resample_frames(frames_dev, maps[0], maps[1])
+            # only the shape of the host array is used from here on
+            intensities = np.broadcast_to(np.float32(0), tuple(self._grid_scan_shape) + (S, S))
 
         # centre of mass and origin fit (phase_base_class.py:613-767)
         if force_com_measured:

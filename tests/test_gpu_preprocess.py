@@ -141,3 +141,32 @@ def test_preprocess_fit_functions_match_oracle(dev, fit_function):
     assert dc <= 1e-5
     assert rel(g._amplitudes, o._amplitudes) <= 1e-5
     assert abs(g._mean_diffraction_intensity / o._mean_diffraction_intensity - 1) < 1e-6
+
+
+@pytest.mark.parametrize("dis,method,pad", [((64, 64), "fourier", None), ((64, 64), "bilinear", None),
+                                            ((64, 64), "bin", None), ((48, 48), "fourier", (64, 64))])
+def test_resampled_preprocess_matches_oracle(dev, dis, method, pad):
+    """diffraction_intensities_shape / reshaping_method /
+    padded_diffraction_intensities_shape (phase_base_class.py:223-420) on 128
+    px patterns: amplitudes <= 1e-5 relative against the oracle (GPU products
+    with float32 maps against NumPy's complex64 FFTs and scipy's zoom),
+    calibrations scaled as the reference's, probe <= 1e-6."""
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import production_workload
+    say(f"resampled preprocess {dis} {method} pad {pad}")
+    w = production_workload(seed=2, scan=(6, 7), object_px=200, num_iter=1)
+    init = {k: v for k, v in w["init"].items() if k != "initial_object_guess"}
+    kw = dict(w["preprocess"], diffraction_intensities_shape=dis, reshaping_method=method,
+              padded_diffraction_intensities_shape=pad)
+    g = SingleslicePtychography(datacube=w["datacube"], **init).preprocess(**kw)
+    o = po.OracleSingleslicePtychography(datacube=w["datacube"], **init)
+    o.preprocess(**kw)
+    say(f"amplitudes {rel(g._amplitudes, o._amplitudes):.2e} probe {rel(g._probe, o._probe):.2e} "
+        f"sampling {g.sampling} {o.sampling}")
+    assert g._amplitudes.shape == o._amplitudes.shape == (42, 64, 64)
+    assert np.allclose(g._angular_sampling, o._angular_sampling, rtol=1e-12)
+    assert rel(g._amplitudes, o._amplitudes) <= 1e-5
+    assert abs(g._mean_diffraction_intensity / o._mean_diffraction_intensity - 1) < 1e-5
+    assert rel(g._probe, o._probe) <= 1e-6
+    g.reconstruct(num_iter=1, max_batch_size=16, progress_bar=False)
+    assert np.isfinite(g.error)

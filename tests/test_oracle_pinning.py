@@ -984,3 +984,40 @@ def test_potential_object_closed_forms(tiny):
     clipped = po.OracleSingleslicePtychography.object_constraints(
         V.copy(), None, 0.0, False, None, None, "potential", True)
     assert np.array_equal(clipped, np.clip(V, 0, None))
+
+
+@pytest.mark.parametrize("dis,method,pad", [((64, 64), "fourier", None), ((48, 48), "fourier", (64, 64)),
+                                            ((128, 128), "fourier", None), ((48, 48), "bilinear", None),
+                                            ((32, 32), "bin", None), (None, "bilinear", (112, 112))])
+def test_resampling_maps_match_the_reference_stack_operations(dis, method, pad):
+    """Resampling and padding (phase_base_class.py:223-420): the drop-in
+    class's per-axis maps, applied as Re(Mx X My^T), equal the oracle's
+    restatement of the reference's per-pattern operations (FFT quadrant copy,
+    scipy zoom, block sums, np.pad) to float32 rounding."""
+    from py4dstem_amd.resample import axis_maps
+    rng = np.random.default_rng(8)
+    data = rng.uniform(0, 5, (2, 3, 96, 96)).astype(np.float32)
+    ref, fac = po.resample_datacube(data, dis, method, pad)
+    Mx, My, fac2 = axis_maps((96, 96), dis, method, pad)
+    mine = np.real(np.einsum("ij,abjk,lk->abil", Mx, data.astype(np.float64), My))
+    assert mine.shape == ref.shape and fac == fac2
+    assert np.max(np.abs(mine - ref)) <= 2e-6 * np.max(np.abs(ref))
+
+
+def test_resampling_closed_forms():
+    """Fourier resampling of a band-limited periodic pattern gives its samples
+    on the new grid times (Q/S)^2 (sums conserved); binning a constant c by b
+    gives b^2 c; zero padding keeps the sum and puts the pattern (S-Q)//2 in."""
+    Q, S = 48, 64
+    n = np.arange(Q)
+    f = lambda x, N: 1 + np.cos(2 * np.pi * 3 * x / N)[:, None] * np.sin(2 * np.pi * 5 * x / N)[None, :]
+    data = f(n, Q)[None, None].astype(np.float32)
+    out, fac = po.resample_datacube(data, (S, S), "fourier")
+    np.testing.assert_allclose(out[0, 0], f(np.arange(S), S) * (Q / S) ** 2, atol=1e-5)
+    assert fac == Q / S and out.sum() == pytest.approx(data.sum(), rel=1e-5)
+    c = np.full((1, 1, 9, 9), 0.5, np.float32)
+    b, fac = po.resample_datacube(c, (4, 4), "bin")
+    assert b.shape == (1, 1, 4, 4) and np.all(b == 2.0) and fac == 2.0
+    p, _ = po.resample_datacube(data, None, "bilinear", (S, S + 1))
+    assert p.shape == (1, 1, S, S + 1) and p.sum() == pytest.approx(data.sum(), rel=1e-6)
+    assert np.array_equal(p[0, 0, 8:8 + Q, 8:8 + Q], data[0, 0])
