@@ -1053,8 +1053,12 @@ template <int S, class Run> PTY_HD void position_correction_position(Run& run, c
 // pixel.  Per pixel the sum order is the batch order, as in the reference's
 // np.bincount; the float32 products are formed exactly as the reference does.
 // ---------------------------------------------------------------------------
-constexpr int GT_R = 4;                  // tile rows
-constexpr int GT_C = 64;                 // tile columns (512-byte patch-row reads)
+#ifndef PTY_GT_R
+#define PTY_GT_R 4
+#define PTY_GT_C 64
+#endif
+constexpr int GT_R = PTY_GT_R;           // tile rows
+constexpr int GT_C = PTY_GT_C;           // tile columns (512-byte patch-row reads)
 constexpr int GATHER_THREADS = GT_R * GT_C;
 constexpr int GATHER_CAND = 4;           // candidates tested per thread per segment
 constexpr int GATHER_SEG = GATHER_THREADS * GATHER_CAND;
