@@ -331,3 +331,23 @@ def test_production_iteration_matches_oracle(dev):
     assert rel(g.object, o.object) <= 1e-4
     assert rel(g.probe, o.probe) <= 1e-4
     assert err <= 1e-4
+
+
+@pytest.mark.parametrize("scan,batch", [((6, 7), 10), ((6, 7), 1), ((1, 1), None), ((5, 5), 25)])
+def test_ragged_and_degenerate_batches_match_oracle(dev, scan, batch):
+    """Batching edge cases against the oracle, probe update on: near-equal
+    ragged mini-batches (42 positions in batches of at most 10:
+    9, 9, 8, 8, 8 as utils.py:539-589 splits them), one-position batches, a
+    single scan position, and a batch equal to the scan.  Object, probe and
+    every per-iteration error <= 1e-4."""
+    from py4dstem_amd.synthetic import small_workload
+    say(f"batches: scan {scan} max_batch_size {batch}")
+    w = small_workload(seed=4, scan=scan, object_px=128)
+    recon = dict(w["reconstruct"], max_batch_size=batch, fix_probe=False, num_iter=3)
+    g, o = _run_pair(w, _init_object((128, 128), 21), recon)
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(g.probe, o.probe):.2e} "
+        f"error {_err_rel(g, o):.2e}")
+    assert len(g.error_iterations) == 3
+    assert rel(g.object, o.object) <= 1e-4
+    assert rel(g.probe, o.probe) <= 1e-4
+    assert _err_rel(g, o) <= 1e-4
