@@ -353,6 +353,19 @@ size_t pty_mixed_workspace_bytes(const pty_problem* problem, const pty_gd_params
 int pty_mixed_run(const pty_problem* problem, const pty_gd_params* params, int num_probes, const int* orders,
                   double* errors, void* workspace, size_t workspace_bytes, pty_stream_t stream);
 
+/* pty_mixed_run with position correction after every mini-batch's update
+ * (mixedstate_ptychography.py:942-956 with ptychographic_methods.py:
+ * 2071-2225): per mode the two perturbed transforms add p_x, p_y and |F_k|^2
+ * into per-pixel accumulators in mode order; the least squares then uses the
+ * mode sums and the intensity sqrt(sum_k |F_k|^2)^2 (:2736-2740).  The
+ * position parameters and constraints are those of pty_gd_run_positions.
+ * Workspace: pty_mixed_positions_workspace_bytes (adds [B][3][S][S] float32). */
+size_t pty_mixed_positions_workspace_bytes(const pty_problem* problem, const pty_gd_params* params,
+                                           int num_probes);
+int pty_mixed_run_positions(const pty_problem* problem, const pty_gd_params* params, int num_probes,
+                            const pty_position_params* positions, const int* orders, double* errors,
+                            void* workspace, size_t workspace_bytes, pty_stream_t stream);
+
 /* One batch of the mixed-state forward/adjoint terms: probe_hat [K][S][S]
  * (kernel layout); ps, chi [K][B][S][S] natural (mode-major); far [B][K][S*S]
  * far fields in the staging layout of pty_forward_adjoint_batch; err[b]. */

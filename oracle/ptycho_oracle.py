@@ -1023,22 +1023,36 @@ class OracleSingleslicePtychography:
         """Intensity residual and the two finite-difference intensity partials
         of ``_position_correction`` (ptychographic_methods.py:2117-2186):
         F = fft2(overlap), d = A^2 - |F|^2 and, for the object patch moved one
-        pixel along each axis, 2 Re((F - F_shifted) conj(F)).  Returns
-        (d, partial_x, partial_y), each [B, S*S]."""
+        pixel along each axis, 2 Re((F - F_shifted) conj(F)).  With probe
+        modes ([B, K, S, S] shifted probes and overlaps) the estimated
+        intensity is sqrt(sum_k |F_k|^2)^2 (:2736-2740) and the partials are
+        summed over the modes (:2187-2189).  Returns (d, partial_x,
+        partial_y), each [B, S*S]."""
         mode = self._mode
+        mixed = shifted_probes.ndim == 4
         rows, cols = patch_indices(positions_px, self._region_of_interest_shape, obj.shape)
         overlap_fft = _fft2(overlap, mode)
         overlap_fft_conj = np.conj(overlap_fft)
-        estimated_intensity = np.abs(overlap_fft) ** 2
+        if mixed:
+            estimated_intensity = np.sqrt(np.sum(np.abs(overlap_fft) ** 2, axis=1)) ** 2
+        else:
+            estimated_intensity = np.abs(overlap_fft) ** 2
         measured_intensity = amplitudes ** 2
         flat_shape = (measured_intensity.shape[0], -1)
         difference_intensity = (measured_intensity - estimated_intensity).reshape(flat_shape)
-        overlap_dx = shifted_probes * obj[(rows + 1) % obj.shape[0], cols]
-        overlap_dy = shifted_probes * obj[rows, (cols + 1) % obj.shape[1]]
+        patch_dx = obj[(rows + 1) % obj.shape[0], cols]
+        patch_dy = obj[rows, (cols + 1) % obj.shape[1]]
+        if mixed:
+            patch_dx, patch_dy = patch_dx[:, None], patch_dy[:, None]
+        overlap_dx = shifted_probes * patch_dx
+        overlap_dy = shifted_probes * patch_dy
         overlap_dx_fft = overlap_fft - _fft2(overlap_dx, mode)
         overlap_dy_fft = overlap_fft - _fft2(overlap_dy, mode)
         partial_dx = 2 * np.real(overlap_dx_fft * overlap_fft_conj)
         partial_dy = 2 * np.real(overlap_dy_fft * overlap_fft_conj)
+        if mixed:
+            partial_dx = partial_dx.sum(1)
+            partial_dy = partial_dy.sum(1)
         return (difference_intensity, partial_dx.reshape(flat_shape),
                 partial_dy.reshape(flat_shape))
 
@@ -1277,8 +1291,9 @@ class OracleSingleslicePtychography:
                         self._object, self._probe, op, sp, pos, ew, step_size,
                         normalization_min, fix_probe)
                 if not fix_positions:  # singleslice_ptychography.py:913-926
+                    overlap = sp * (op[:, None] if sp.ndim == 4 else op)
                     self._positions_px[idx] = self.position_correction(
-                        self._object, pos, sp, sp * op, self._amplitudes[idx],
+                        self._object, pos, sp, overlap, self._amplitudes[idx],
                         self._positions_px_initial[idx], positions_step_size,
                         max_position_update_distance, max_position_total_distance)
                 error += err

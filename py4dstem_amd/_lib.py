@@ -141,6 +141,11 @@ _SIGNATURES = {
                                              c_int]),
     "pty_mixed_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams), c_int,
                               c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_mixed_positions_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem),
+                                                       ctypes.POINTER(PtyGdParams), c_int]),
+    "pty_mixed_run_positions": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams), c_int,
+                                        ctypes.POINTER(PtyPositionParams), c_void_p, c_void_p,
+                                        c_void_p, c_size_t, c_void_p]),
     "pty_mixed_forward_adjoint_batch": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p,
                                                 c_int, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                                 c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -855,6 +855,14 @@ struct PosArgs {
   double max_update, max_total;    // those limits in pixels
   double* sums;                    // optional [B][5]: p_x^2, p_x p_y, p_y^2, p_x d, p_y d
   int* singular;                   // incremented per singular 2x2 system
+  // probe modes (ptychographic_methods.py:2187-2189, 2736-2740): `far`
+  // points at mode k of slot 0 and slot b's is far_modes slots further; the
+  // per-mode pass adds p_x, p_y and |F_k|^2 of every pixel into acc
+  // ([B][3][N1][T], staging order; acc_add = 0 for the first mode stores);
+  // far_modes 0 means 1
+  int far_modes;
+  float* acc;
+  int acc_add;
 };
 
 constexpr int POS_SUMS = 5;
@@ -931,7 +939,8 @@ template <int S> struct PosCorr {
     r.idx = idx;
     r.x0 = wrapi((int)rintf(a.positions[2 * idx]), a.Px);
     r.y0 = wrapi((int)rintf(a.positions[2 * idx + 1]), a.Py);
-    const cf* Fb = a.far + (size_t)b * S * S + tid;
+    const size_t modes = a.far_modes > 0 ? (size_t)a.far_modes : 1;  // 0: single mode
+    const cf* Fb = a.far + (size_t)b * modes * S * S + tid;
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.f[j] = Fb[(size_t)j * T];
   }
@@ -1010,12 +1019,43 @@ template <int S> struct PosCorr {
     }
     r.s[0] = s0; r.s[1] = s1; r.s[2] = s2; r.s[3] = s3; r.s[4] = s4;
   }
+  // probe modes: row pass B completes G_y; p_x, p_y and |F|^2 (abs, then
+  // squared, as the reference's mode sum) added into the accumulators in
+  // mode order, float32 as the reference's sum over the mode axis
+  template <class L> static PTY_HD void p_partial_y_acc(int tid, const PosArgs& a, int b, Regs& r, L lds,
+                                                        const cf* tw) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const int rr = F::row_line(tid), t = F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
+    P::template B<false>(t, r.v, tw);
+    float* acc = a.acc + (size_t)b * 3 * S * S + tid;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      const float py = partial(r.f[j], r.v[j]);
+      const float m = hypot_f(r.f[j].x, r.f[j].y);
+      const float i2 = m * m;
+      float* e = acc + (size_t)j * T;
+      if (a.acc_add) {
+        e[0] = e[0] + r.px[j];
+        e[S * S] = e[S * S] + py;
+        e[2 * S * S] = e[2 * S * S] + i2;
+      } else {
+        e[0] = r.px[j];
+        e[S * S] = py;
+        e[2 * S * S] = i2;
+      }
+    }
+  }
 };
 
 // The phase sequence of one position's correction (runner as for
 // fused_position; ``run.reduce_sums`` adds r.s over the work group and hands
 // the totals to thread 0).  Nine work-group barriers.
-template <int S, class Run> PTY_HD void position_correction_position(Run& run, const PosArgs& a, int b) {
+template <int S, class Run, bool ACC = false>
+PTY_HD void position_correction_position(Run& run, const PosArgs& a, int b) {
   using K = PosCorr<S>;
   using Regs = typename K::Regs;
   using FK = Fused<S>;
@@ -1035,6 +1075,10 @@ template <int S, class Run> PTY_HD void position_correction_position(Run& run, c
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_rowA_amps(tid, a, r, lds, tw); });
   run.wave_barrier();
+  if (ACC) {  // one probe mode: the sums follow once every mode is in
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_partial_y_acc(tid, a, b, r, lds, tw); });
+    return;
+  }
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_partial_y_sums(tid, r, lds, tw); });
   double tot[POS_SUMS];
   run.reduce_sums(tot);  // ends with a barrier; totals valid in thread 0

@@ -209,7 +209,7 @@ template <int S> struct GpuPosRunner {
   }
 };
 
-template <int S>
+template <int S, bool ACC = false>
 __global__ __launch_bounds__(Frame<S>::T) void k_position_correction(PosArgs a) {
   __shared__ cf lds_mem[PosCorr<S>::LDS_TOTAL];
   __shared__ cf tw[S];
@@ -218,7 +218,53 @@ __global__ __launch_bounds__(Frame<S>::T) void k_position_correction(PosArgs a) 
   Fused<S>::twiddles(tid, LdsPtr{tw});  // published by the setup barrier
   typename PosCorr<S>::Regs r;
   GpuPosRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red};
-  position_correction_position<S>(run, a, (int)blockIdx.x);
+  position_correction_position<S, GpuPosRunner<S>, ACC>(run, a, (int)blockIdx.x);
+}
+
+// probe modes: the least-squares sums of slot b from the mode-summed
+// accumulators (ptychographic_methods.py:2128-2196 with the mixed-state
+// intensity sqrt(sum_k |F_k|^2)^2 of :2736-2740), then the shared update
+template <int S>
+__global__ __launch_bounds__(Frame<S>::T) void k_position_solve(PosArgs a) {
+#pragma clang fp contract(off)
+  using F = Frame<S>;
+  constexpr int T = F::T, N1 = F::N1;
+  __shared__ double red[T / 64 * POS_SUMS];
+  const int b = (int)blockIdx.x, tid = threadIdx.x;
+  int idx = a.order ? a.order[b] : b;
+  idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
+  const float* A = a.amps + (size_t)idx * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
+  const float* acc = a.acc + (size_t)b * 3 * S * S + tid;
+  double s[POS_SUMS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < N1; ++j) {
+    const double px = acc[(size_t)j * T];
+    const double py = acc[(size_t)S * S + (size_t)j * T];
+    const float m = sqrt_rn(acc[(size_t)2 * S * S + (size_t)j * T]);
+    const double d = A[j] * A[j] - m * m;  // float32 difference
+    s[0] += px * px;
+    s[1] += px * py;
+    s[2] += py * py;
+    s[3] += px * d;
+    s[4] += py * d;
+  }
+#pragma unroll
+  for (int k = 0; k < POS_SUMS; ++k) {
+    double v = s[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((tid & 63) == 0) red[(tid >> 6) * POS_SUMS + k] = v;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double tot[POS_SUMS];
+    for (int k = 0; k < POS_SUMS; ++k) {
+      double v = 0.0;
+      for (int w = 0; w < T / 64; ++w) v += red[w * POS_SUMS + k];
+      tot[k] = v;
+    }
+    position_update(a, b, idx, tot);
+  }
 }
 
 // positions -= mean(positions) - com (phase/ptychographic_constraints.py:1292-1311)
@@ -1766,6 +1812,7 @@ PosArgs position_args(const pty_position_params& pp, const cf* ps, const cf* far
   a.clip_update = pp.clip_update; a.max_update = pp.max_update;
   a.clip_total = pp.clip_total; a.max_total = pp.max_total;
   a.sums = nullptr; a.singular = pp.singular;
+  a.far_modes = 1; a.acc = nullptr; a.acc_add = 0;
   return a;
 }
 
@@ -2073,11 +2120,11 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
 // workspace of the mixed-state loop: per-mode staging, far fields, scatter
 // outputs and probe partials for whole-batch launches
 struct MixedLayout {
-  size_t phat, ps, chi, far, err, grad, spsum, tilemax, total, bmax, ppart, pstride, cpart, shift, all;
+  size_t phat, ps, chi, far, err, grad, spsum, tilemax, total, bmax, ppart, pstride, cpart, shift, acc, all;
 };
 constexpr int MIXED_BLOCKS = 1024;
 
-MixedLayout mixed_layout(const pty_problem& p, const pty_gd_params& q, int K) {
+MixedLayout mixed_layout(const pty_problem& p, const pty_gd_params& q, int K, bool positions = false) {
   MixedLayout L{};
   const size_t S2 = (size_t)p.S * p.S, B = (size_t)q.max_batch, npix = (size_t)p.Px * p.Py;
   size_t o = 0;
@@ -2095,6 +2142,7 @@ MixedLayout mixed_layout(const pty_problem& p, const pty_gd_params& q, int K) {
   L.ppart = o; o = align256(o + K * L.pstride);
   L.cpart = o; o = align256(o + (size_t)CONSTRAINT_BLOCKS * 16);
   L.shift = o; o = align256(o + 8);
+  L.acc = o; o = align256(o + (positions ? 3 * B * S2 * 4 : 0));
   L.all = o;
   return L;
 }
@@ -2104,7 +2152,8 @@ MixedLayout mixed_layout(const pty_problem& p, const pty_gd_params& q, int K) {
 // [K][S][S]; the per-iteration probe constraint is the recentring by mode 0's
 // centre of mass (the orthogonalisation runs on the host between calls)
 int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, const int* orders, double* errors,
-              void* workspace, size_t workspace_bytes, hipStream_t stream) {
+              void* workspace, size_t workspace_bytes, hipStream_t stream,
+              const pty_position_params* pp = nullptr) {
   int rc = validate_problem(problem);
   if (rc) return rc;
   if (!params) return fail(PTY_E_ARG, "params is NULL");
@@ -2116,7 +2165,8 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
   if (q.num_iter < 0 || q.max_batch <= 0 || q.max_batch > p.N)
     return fail(PTY_E_ARG, "bad num_iter=%d / max_batch=%d for N=%d", q.num_iter, q.max_batch, p.N);
   if (!orders || !errors || !workspace) return fail(PTY_E_ARG, "null orders/errors/workspace");
-  const MixedLayout L = mixed_layout(p, q, K);
+  if ((rc = validate_position_params(pp, p.positions))) return rc;
+  const MixedLayout L = mixed_layout(p, q, K, pp != nullptr);
   if (workspace_bytes < L.all)
     return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.all);
   char* ws = (char*)workspace;
@@ -2186,6 +2236,25 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
             return rc;
         }
       }
+      if (pp) {  // position correction with the updated object (mixedstate_ptychography.py:942-956)
+        PosArgs base = position_args(*pp, ps, far, obj, p.Px, p.Py, p.amplitudes, p.N, ob, nb);
+        base.far_modes = K;
+        base.acc = (float*)(ws + L.acc);
+        for (int k = 0; k < K; ++k) {
+          PosArgs ak = base;
+          ak.ps = ps + (size_t)k * nb * S2;
+          ak.far = far + (size_t)k * S2;
+          ak.acc_add = k > 0;
+          if (p.S == 64)
+            hipLaunchKernelGGL((k_position_correction<64, true>), dim3(nb), dim3(Frame<64>::T), 0, stream, ak);
+          else
+            hipLaunchKernelGGL((k_position_correction<128, true>), dim3(nb), dim3(Frame<128>::T), 0, stream, ak);
+          if ((rc = check_launch("k_position_correction"))) return rc;
+        }
+        if (p.S == 64) hipLaunchKernelGGL(k_position_solve<64>, dim3(nb), dim3(Frame<64>::T), 0, stream, base);
+        else hipLaunchKernelGGL(k_position_solve<128>, dim3(nb), dim3(Frame<128>::T), 0, stream, base);
+        if ((rc = check_launch("k_position_solve"))) return rc;
+      }
       start += nb;
     }
     hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
@@ -2198,6 +2267,9 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
       if ((rc = launch_probe_com_modes(probe, K, p.S, (float*)(ws + L.shift), stream))) return rc;
     }
     if (!q.fix_probe && (rc = launch_probe_extra(probe, K, p.S, q, stream))) return rc;
+    if (pp && pp->center_of_mass) {
+      if ((rc = launch_positions_com(pp->positions, p.N, pp->initial_com, stream))) return rc;
+    }
   }
   return PTY_OK;
 }
@@ -2540,6 +2612,19 @@ size_t pty_mixed_workspace_bytes(const pty_problem* problem, const pty_gd_params
 int pty_mixed_run(const pty_problem* problem, const pty_gd_params* params, int num_probes, const int* orders,
                   double* errors, void* workspace, size_t workspace_bytes, pty_stream_t stream) {
   return mixed_run(problem, params, num_probes, orders, errors, workspace, workspace_bytes, st(stream));
+}
+
+size_t pty_mixed_positions_workspace_bytes(const pty_problem* problem, const pty_gd_params* params,
+                                           int num_probes) {
+  if (!problem || !params || params->max_batch <= 0 || num_probes < 1) return 0;
+  return mixed_layout(*problem, *params, num_probes, true).all;
+}
+
+int pty_mixed_run_positions(const pty_problem* problem, const pty_gd_params* params, int num_probes,
+                            const pty_position_params* positions, const int* orders, double* errors,
+                            void* workspace, size_t workspace_bytes, pty_stream_t stream) {
+  if (!positions) return fail(PTY_E_ARG, "position parameters are NULL");
+  return mixed_run(problem, params, num_probes, orders, errors, workspace, workspace_bytes, st(stream), positions);
 }
 
 int pty_mixed_forward_adjoint_batch(const void* probe_hat, int num_probes, const void* object, int Px, int Py,

@@ -263,10 +263,21 @@ def mixed_workspace_bytes(problem, params, num_probes):
                                                      int(num_probes)))
 
 
-def mixed_run(problem, params, num_probes, orders, errors, workspace):
+def mixed_run(problem, params, num_probes, orders, errors, workspace, position_params=None):
     import ctypes
-    _lib.call("pty_mixed_run", ctypes.byref(problem), ctypes.byref(params), int(num_probes),
-              ptr(orders), ptr(errors), ptr(workspace), workspace.numel(), stream_handle())
+    if position_params is None:
+        _lib.call("pty_mixed_run", ctypes.byref(problem), ctypes.byref(params), int(num_probes),
+                  ptr(orders), ptr(errors), ptr(workspace), workspace.numel(), stream_handle())
+    else:
+        _lib.call("pty_mixed_run_positions", ctypes.byref(problem), ctypes.byref(params),
+                  int(num_probes), ctypes.byref(position_params), ptr(orders), ptr(errors),
+                  ptr(workspace), workspace.numel(), stream_handle())
+
+
+def mixed_positions_workspace_bytes(problem, params, num_probes):
+    import ctypes
+    return int(_lib.load().pty_mixed_positions_workspace_bytes(
+        ctypes.byref(problem), ctypes.byref(params), int(num_probes)))
 
 
 def mixed_forward_adjoint(probe_hats, obj, positions, amplitudes_k, order):

@@ -135,9 +135,9 @@ class MixedstatePtychography(SingleslicePtychography):
                                reconstruction_parameter_c)  # the reference's ValueErrors first
             raise NotImplementedError("mixed-state projection-set methods are not supported on "
                                       "this path (gradient descent only)")
-        if not fix_positions:
-            raise NotImplementedError("position correction with several probe modes is not "
-                                      "supported on this path")
+        if not fix_positions and global_affine_transformation:
+            raise NotImplementedError("the global affine position constraint with several probe "
+                                      "modes is not supported on this path")
         if self._object_type == "potential":
             raise NotImplementedError("potential objects with several probe modes are not "
                                       "supported on this path")
@@ -146,7 +146,10 @@ class MixedstatePtychography(SingleslicePtychography):
             max_batch_size=max_batch_size, seed_random=seed_random, step_size=step_size,
             normalization_min=normalization_min, pure_phase_object=pure_phase_object,
             fix_probe_com=fix_probe_com, fix_probe=fix_probe,
-            fix_probe_aperture=fix_probe_aperture,
+            fix_probe_aperture=fix_probe_aperture, positions_step_size=positions_step_size,
+            fix_positions=fix_positions, fix_positions_com=fix_positions_com,
+            max_position_update_distance=max_position_update_distance,
+            max_position_total_distance=max_position_total_distance,
             constrain_probe_amplitude=constrain_probe_amplitude,
             constrain_probe_amplitude_relative_radius=constrain_probe_amplitude_relative_radius,
             constrain_probe_amplitude_relative_width=constrain_probe_amplitude_relative_width,
@@ -160,9 +163,10 @@ class MixedstatePtychography(SingleslicePtychography):
             progress_bar=progress_bar, reset=reset, device=device,
             clear_fft_cache=clear_fft_cache, object_type=object_type, _allocate=False)
         dev = run["orders"].device
-        run["workspace"] = torch.empty(
-            K.mixed_workspace_bytes(run["problem"], run["params"], self._num_probes),
-            dtype=torch.uint8, device=dev)
+        nbytes = (K.mixed_workspace_bytes if run["position_params"] is None
+                  else K.mixed_positions_workspace_bytes)(run["problem"], run["params"],
+                                                          self._num_probes)
+        run["workspace"] = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         run["orthogonalize"] = bool(orthogonalize_probe)
         if orthogonalize_probe:
             run["per_call"] = 1  # the host step follows every iteration
@@ -184,7 +188,8 @@ class MixedstatePtychography(SingleslicePtychography):
         while it < num_iter:
             n = min(per_call, num_iter - it)
             params.num_iter = n
-            K.mixed_run(problem, params, self._num_probes, orders_dev[it:], errors_dev[it:], ws)
+            K.mixed_run(problem, params, self._num_probes, orders_dev[it:], errors_dev[it:], ws,
+                        run["position_params"])
             if run["orthogonalize"]:
                 modes = orthogonalize_probe_modes(self._probe_dev.cpu().numpy())
                 self._probe_dev.copy_(torch.from_numpy(np.ascontiguousarray(modes)))

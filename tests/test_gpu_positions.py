@@ -202,3 +202,44 @@ def test_position_correction_reduces_scan_error(dev):
     e1 = scan_error()
     say(f"scan rms error {e0:.3f} px -> {e1:.3f} px")
     assert e1 < 0.5 * e0
+
+
+@pytest.mark.parametrize("variant", ["plain", "com", "clipped"])
+def test_mixed_state_position_correction_matches_oracle(dev, variant):
+    """MixedstatePtychography(num_probes=2, fix_positions=False): the
+    per-mode perturbed transforms summed over the modes before the least
+    squares, with the mixed-state intensity sqrt(sum_k |F_k|^2)^2
+    (ptychographic_methods.py:2071-2225, 2736-2740), against the oracle.
+    Same bounds as the single-mode test."""
+    from oracle import OracleMixedstatePtychography
+    from py4dstem_amd import MixedstatePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say(f"mixed state, fix_positions=False {variant}")
+    w = small_workload(seed=1)
+    first = dict(w["reconstruct"], max_batch_size=256, fix_probe=False, num_iter=3,
+                 orthogonalize_probe=False)
+    then = dict(first, num_iter=3, reset=False, fix_positions=False, positions_step_size=0.5,
+                **VARIANTS[variant])
+    init_obj = _init_object((256, 256), 17)
+    out = {}
+    for kind, cls in (("gpu", MixedstatePtychography), ("oracle", OracleMixedstatePtychography)):
+        p = cls(datacube=w["datacube"], num_probes=2,
+                **dict(w["init"], initial_object_guess=init_obj.copy()))
+        np.random.seed(5)
+        p.preprocess(**w["preprocess"])
+        np.random.seed(1234)
+        p.reconstruct(**first)
+        out[kind] = p.reconstruct(**then)
+    g, o = out["gpu"], out["oracle"]
+    err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
+    dg = g._positions_px - g._positions_px_initial
+    do = o._positions_px - o._positions_px_initial
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(np.asarray(g.probe), o.probe):.2e} "
+        f"error {err:.2e} positions {rel(dg, do):.2e} (max move {np.abs(do).max():.3f} px, "
+        f"max diff {np.abs(dg - do).max():.2e} px)")
+    assert np.abs(do).max() > 1e-3
+    assert rel(g.object, o.object) <= 1e-4
+    assert rel(np.asarray(g.probe), o.probe) <= 1e-4
+    assert err <= 1e-4
+    assert rel(dg, do) <= 1e-3
+    assert np.abs(dg - do).max() <= 1e-3

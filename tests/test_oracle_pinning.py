@@ -1021,3 +1021,27 @@ def test_resampling_closed_forms():
     p, _ = po.resample_datacube(data, None, "bilinear", (S, S + 1))
     assert p.shape == (1, 1, S, S + 1) and p.sum() == pytest.approx(data.sum(), rel=1e-6)
     assert np.array_equal(p[0, 0, 8:8 + Q, 8:8 + Q], data[0, 0])
+
+
+def test_mixed_position_partials_reduce_to_single_mode(tiny):
+    """Mixed-state position partials (ptychographic_methods.py:2187-2189,
+    2736-2740): with a second mode of zeros they equal the single-mode
+    partials (the intensity sqrt(|F|^2)^2 differs from |F|^2 by rounding
+    only); with two modes they are the sums of each mode's own partials."""
+    o, probe, obj, pos = tiny
+    rng = np.random.default_rng(9)
+    sp1, op, _, _ = o.forward(obj, probe, pos, np.ones((len(pos), 16, 16), np.float32))
+    amps = (np.abs(np.fft.fft2(sp1 * op)) * rng.uniform(0.9, 1.1, sp1.shape)).astype(np.float32)
+    obj2 = (obj * np.exp(0.05j * rng.standard_normal(obj.shape))).astype(np.complex64)
+    d1, px1, py1 = o.position_partials(obj2, pos, sp1, sp1 * op, amps)
+    sp = np.stack([sp1, np.zeros_like(sp1)], axis=1)
+    d, px, py = o.position_partials(obj2, pos, sp, sp * op[:, None], amps)
+    np.testing.assert_allclose(px, px1, rtol=1e-6, atol=1e-6 * np.abs(px1).max())
+    np.testing.assert_allclose(py, py1, rtol=1e-6, atol=1e-6 * np.abs(py1).max())
+    np.testing.assert_allclose(d, d1, atol=1e-5 * np.abs(d1).max())
+    sp2 = (0.5 * sp1 * np.exp(1j * 0.3)).astype(np.complex64)
+    sp = np.stack([sp1, sp2], axis=1)
+    d, px, py = o.position_partials(obj2, pos, sp, sp * op[:, None], amps)
+    _, px2, py2 = o.position_partials(obj2, pos, sp2, sp2 * op, amps)
+    np.testing.assert_allclose(px, px1 + px2, atol=1e-5 * np.abs(px1).max())
+    np.testing.assert_allclose(py, py1 + py2, atol=1e-5 * np.abs(py1).max())
