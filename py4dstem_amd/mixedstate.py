@@ -135,9 +135,6 @@ class MixedstatePtychography(SingleslicePtychography):
                                reconstruction_parameter_c)  # the reference's ValueErrors first
             raise NotImplementedError("mixed-state projection-set methods are not supported on "
                                       "this path (gradient descent only)")
-        if not fix_positions and global_affine_transformation:
-            raise NotImplementedError("the global affine position constraint with several probe "
-                                      "modes is not supported on this path")
         if self._object_type == "potential":
             raise NotImplementedError("potential objects with several probe modes are not "
                                       "supported on this path")
@@ -148,6 +145,7 @@ class MixedstatePtychography(SingleslicePtychography):
             fix_probe_com=fix_probe_com, fix_probe=fix_probe,
             fix_probe_aperture=fix_probe_aperture, positions_step_size=positions_step_size,
             fix_positions=fix_positions, fix_positions_com=fix_positions_com,
+            global_affine_transformation=global_affine_transformation,
             max_position_update_distance=max_position_update_distance,
             max_position_total_distance=max_position_total_distance,
             constrain_probe_amplitude=constrain_probe_amplitude,
@@ -188,11 +186,16 @@ class MixedstatePtychography(SingleslicePtychography):
         while it < num_iter:
             n = min(per_call, num_iter - it)
             params.num_iter = n
+            if run["affine"]:  # one iteration per call: orders drawn between the RANSAC fits
+                np.random.shuffle(run["shuffled"])
+                orders_dev[it].copy_(torch.from_numpy(run["shuffled"].astype(np.int32)))
             K.mixed_run(problem, params, self._num_probes, orders_dev[it:], errors_dev[it:], ws,
                         run["position_params"])
             if run["orthogonalize"]:
                 modes = orthogonalize_probe_modes(self._probe_dev.cpu().numpy())
                 self._probe_dev.copy_(torch.from_numpy(np.ascontiguousarray(modes)))
+            if run["affine"]:
+                self._affine_positions_constraint()
             it += n
             if run["store_iterations"]:
                 self.object_iterations.append(self._object_dev.cpu().numpy().copy())

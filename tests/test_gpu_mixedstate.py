@@ -165,8 +165,8 @@ def test_mixed_state_argument_errors(dev):
     assert p._probe.shape == (2, 64, 64)
     with pytest.raises(NotImplementedError):
         p.reconstruct(num_iter=1, reconstruction_method="DM_AP")
-    with pytest.raises(NotImplementedError):  # position correction runs; the affine fit does not
-        p.reconstruct(num_iter=1, fix_positions=False, global_affine_transformation=True,
-                      max_batch_size=16)
+    p.reconstruct(num_iter=1, fix_positions=False, global_affine_transformation=True,
+                  max_batch_size=16, progress_bar=False, reset=True)
+    assert np.isfinite(p.error)
     p.reconstruct(num_iter=2, max_batch_size=16, progress_bar=False, reset=True)
     assert np.isfinite(p.error) and len(p.error_iterations) == 2

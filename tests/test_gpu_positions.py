@@ -204,7 +204,7 @@ def test_position_correction_reduces_scan_error(dev):
     assert e1 < 0.5 * e0
 
 
-@pytest.mark.parametrize("variant", ["plain", "com", "clipped"])
+@pytest.mark.parametrize("variant", ["plain", "com", "clipped", "affine"])
 def test_mixed_state_position_correction_matches_oracle(dev, variant):
     """MixedstatePtychography(num_probes=2, fix_positions=False): the
     per-mode perturbed transforms summed over the modes before the least
