@@ -47,7 +47,21 @@ PINNING = {
                                  "Fourier phase and intensity kept",
     "probe_amplitude_constraint": "intensity kept; erf edge value 1/2 at the radius; identity "
                                   "inside for a narrow edge",
-    "position_partials": "brute-force DFT loops on tiny frames with np.roll'ed objects",
+    "position_partials": "brute-force DFT loops on tiny frames with np.roll'ed objects; probe "
+                         "modes: a zero second mode reproduces the single-mode partials, two "
+                         "modes give the sum of each mode's partials",
+    "object_constraints (gaussian)": "scipy.ndimage.gaussian_filter (the reference's call); "
+                                     "the GPU's summation order reproduces it bit for bit",
+    "object_constraints (potential)": "shrinkage leaves the masked mean at -shrinkage_rad; "
+                                      "positivity is clip(0)",
+    "object_butterworth_constraint": "constant plus plane wave: the wave scaled by the envelope "
+                                     "at its frequency (1/2 at the cut-off); zero cut-offs are "
+                                     "the identity",
+    "forward/adjoint (potential)": "V and exp(iV) give the same exit waves; the one-position "
+                                   "potential step is Re(-i conj(exp(iV)) dO_complex)",
+    "fourier_resample_4d / resample_datacube": "band-limited pattern resampled to its exact "
+                                               "samples x (Q/S)^2; binning and padding closed "
+                                               "forms",
     "position_correction": "exact recovery of a planted least-squares step; clipping "
                            "closed forms; sign/magnitude on a sub-pixel displaced scan",
     "positions_constraints": "scan centre of mass restored exactly; exact affine maps "
