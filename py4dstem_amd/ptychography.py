@@ -650,8 +650,11 @@ class SingleslicePtychography:
                                           "not supported on this path")
             if detector_fourier_mask is not None:  # ptychographic_methods.py:1686-1687
                 raise NotImplementedError()
+        tv = tv_denoise and tv_denoise_weight is not None
         unsupported = {
-            "tv_denoise_weight": tv_denoise and tv_denoise_weight is not None,
+            # the reference runs pylops' split Bregman solver here
+            # (ptychographic_constraints.py:233-256)
+            "tv_denoise_weight with object_type='potential'": tv and self._object_type == "potential",
             "fit_probe_aberrations": fit_probe_aberrations and not fix_probe,
             # the aperture constraint takes precedence (ptychographic_constraints.py:1127-1137)
             "constrain_probe_fourier_amplitude": (constrain_probe_fourier_amplitude
@@ -660,6 +663,12 @@ class SingleslicePtychography:
         bad = [k for k, v in unsupported.items() if v]
         if bad:
             raise NotImplementedError("not supported on this path: " + ", ".join(bad))
+        if tv:
+            # complex objects: TV denoising leaves the object as it is and only
+            # warns (ptychographic_constraints.py:224-231, called every
+            # iteration from :443-446)
+            warnings.warn("TV denoising is currently only supported for object_type=='potential'.",
+                          UserWarning)
 
         N = self._num_diffraction_patterns
         if self._verbose:

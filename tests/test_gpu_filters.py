@@ -163,3 +163,34 @@ def test_reconstruct_with_butterworth_matches_oracle(dev, mixed):
     assert rel(g.object, o.object) <= 1e-4
     assert rel(np.asarray(g.probe), o.probe) <= 1e-4
     assert err <= 1e-4
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_tv_denoise_complex_object_is_identity(dev, mixed):
+    """For complex objects the reference's TV step returns the object as it
+    is and warns (ptychographic_constraints.py:224-231): a reconstruction with
+    tv_denoise_weight set equals one without it, bit for bit."""
+    from py4dstem_amd import MixedstatePtychography
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say(f"tv_denoise_weight on a complex object, mixed={mixed}")
+    w = small_workload(seed=1, scan=(8, 8), object_px=128)
+    cls = MixedstatePtychography if mixed else SingleslicePtychography
+    out = []
+    for tv in (None, 0.1):
+        kw = dict(w["init"], num_probes=2) if mixed else w["init"]
+        p = cls(datacube=w["datacube"], **kw)
+        p.preprocess(**w["preprocess"])
+        recon = dict(num_iter=2, max_batch_size=16, fix_probe=False, progress_bar=False,
+                     tv_denoise_weight=tv, seed_random=3)
+        if mixed:
+            recon["orthogonalize_probe"] = False
+        if tv is None:
+            p.reconstruct(**recon)
+        else:
+            with pytest.warns(UserWarning, match="TV denoising"):
+                p.reconstruct(**recon)
+        out.append((p.object.copy(), np.asarray(p.probe).copy(), list(p.error_iterations)))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]

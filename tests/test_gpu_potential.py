@@ -90,6 +90,8 @@ def test_potential_default_init_and_errors(dev):
         p.reconstruct(num_iter=1, fix_positions=False, max_batch_size=16)
     with pytest.raises(NotImplementedError):
         p.reconstruct(num_iter=1, q_lowpass=1.0, max_batch_size=16)
+    with pytest.raises(NotImplementedError):  # pylops split Bregman
+        p.reconstruct(num_iter=1, tv_denoise_weight=0.1, max_batch_size=16)
     p.reconstruct(num_iter=2, max_batch_size=16, progress_bar=False, reset=True)
     assert np.isfinite(p.error) and len(p.error_iterations) == 2
     m = MixedstatePtychography(datacube=w["datacube"], num_probes=2, object_type="potential", **init)
