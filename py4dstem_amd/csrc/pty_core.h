@@ -1101,6 +1101,9 @@ PTY_HD void position_correction_position(Run& run, const PosArgs& a, int b) {
 #define PTY_GT_R 4
 #define PTY_GT_C 64
 #endif
+#ifndef PTY_GATHER_UNROLL
+#define PTY_GATHER_UNROLL 4
+#endif
 constexpr int GT_R = PTY_GT_R;           // tile rows
 constexpr int GT_C = PTY_GT_C;           // tile columns (512-byte patch-row reads)
 constexpr int GATHER_THREADS = GT_R * GT_C;
@@ -1248,12 +1251,14 @@ struct Gather {
     const int S = a.S;
     const size_t SS = (size_t)S * S;
     int h = 0;
-    // four hits per round: loads first, then the float64 adds in order
-    for (; h + 4 <= r.total; h += 4) {
-      cf p[4], c[4];
-      bool in[4];
+    // PTY_GATHER_UNROLL hits per round: loads first, then the float64 adds
+    // in order
+    constexpr int U = PTY_GATHER_UNROLL;
+    for (; h + U <= r.total; h += U) {
+      cf p[U], c[U];
+      bool in[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         const GatherHit e = list.ld(h + u);
         const int w = welem(row, col, e, S, a.Px, a.Py);
         in[u] = w >= 0;
@@ -1262,7 +1267,7 @@ struct Gather {
         c[u] = a.chi[o];
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         if (in[u]) {
           const cf g = cconjmul(p[u], c[u]);
           r.re += (double)g.x;

@@ -2761,8 +2761,8 @@ int pty_multi_run(const pty_multi_problem* problems, int count, int num_iter, in
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (ready) (void)hipEventDestroy(ready);
-  if (done) hipEventDestroy(done);
-  if (cs) hipStreamDestroy(cs);
+  if (done) (void)hipEventDestroy(done);
+  if (cs) (void)hipStreamDestroy(cs);
   return rc;
 }
 
