@@ -15,7 +15,8 @@
  *  - Every call is asynchronous on `stream` (a hipStream_t, NULL = default
  *    stream) and returns 0 on success or a negative PTY_E* code; the message
  *    is available from pty_last_error() on the calling thread.
- *  - Frame edge S must be 64 or 128 (square frames).  Objects need Px >= S
+ *  - Frame edge S must be 32, 64 or 128 (square frames; the batched runner
+ *    pty_multi_run and the network entries take 64 or 128).  Objects need Px >= S
  *    and Py >= S.  "Kernel layout" amplitudes / probe spectra are stored in
  *    the FFT's permuted frequency order (see pty_permute_frames).
  */

@@ -159,11 +159,14 @@ template <> PTY_HD void dft<8, false>(cf* x) { dft8<false>(x); }
 template <> PTY_HD void dft<8, true>(cf* x) { dft8<true>(x); }
 template <> PTY_HD void dft<16, false>(cf* x) { dft16<false>(x); }
 template <> PTY_HD void dft<16, true>(cf* x) { dft16<true>(x); }
+template <> PTY_HD void dft<4, false>(cf* x) { dft4<false>(x[0], x[1], x[2], x[3]); }
+template <> PTY_HD void dft<4, true>(cf* x) { dft4<true>(x[0], x[1], x[2], x[3]); }
 
 // ---------------------------------------------------------------------------
 // frame geometry
 // ---------------------------------------------------------------------------
 template <int S> struct Geo;
+template <> struct Geo<32> { static constexpr int N1 = 8, N2 = 4; };
 template <> struct Geo<64> { static constexpr int N1 = 8, N2 = 8; };
 template <> struct Geo<128> { static constexpr int N1 = 16, N2 = 8; };
 

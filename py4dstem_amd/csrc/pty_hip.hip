@@ -1222,20 +1222,23 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_amplitude(cf* __restrict_
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-bool frame_supported(int S) { return S == 64 || S == 128; }
+bool frame_supported(int S) { return S == 32 || S == 64 || S == 128; }
 
 int launch_fused(const FusedArgs& a, int S, hipStream_t stream) {
   const int grid = a.B - a.b0;
   if (grid <= 0) return PTY_OK;
   if (a.object_pot) {  // potential object (no detector mask: the caller rejects the pair)
     if (a.fmask) return fail(PTY_E_UNSUPPORTED, "detector masks with potential objects are not supported");
-    if (S == 64) hipLaunchKernelGGL((k_fused<64, false, true>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+    if (S == 32) hipLaunchKernelGGL((k_fused<32, false, true>), dim3(grid), dim3(Frame<32>::T), 0, stream, a);
+    else if (S == 64) hipLaunchKernelGGL((k_fused<64, false, true>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
     else hipLaunchKernelGGL((k_fused<128, false, true>), dim3(grid), dim3(Frame<128>::T), 0, stream, a);
   } else if (a.fmask) {
-    if (S == 64) hipLaunchKernelGGL((k_fused<64, true>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+    if (S == 32) hipLaunchKernelGGL((k_fused<32, true>), dim3(grid), dim3(Frame<32>::T), 0, stream, a);
+    else if (S == 64) hipLaunchKernelGGL((k_fused<64, true>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
     else hipLaunchKernelGGL((k_fused<128, true>), dim3(grid), dim3(Frame<128>::T), 0, stream, a);
   } else {
-    if (S == 64) hipLaunchKernelGGL((k_fused<64, false>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+    if (S == 32) hipLaunchKernelGGL((k_fused<32, false>), dim3(grid), dim3(Frame<32>::T), 0, stream, a);
+    else if (S == 64) hipLaunchKernelGGL((k_fused<64, false>), dim3(grid), dim3(Frame<64>::T), 0, stream, a);
     else hipLaunchKernelGGL((k_fused<128, false>), dim3(grid), dim3(Frame<128>::T), 0, stream, a);
   }
   return check_launch("k_fused");
@@ -1244,7 +1247,8 @@ int launch_fused(const FusedArgs& a, int S, hipStream_t stream) {
 int launch_mixed(const FusedArgs& a, int S, hipStream_t stream) {
   const int grid = a.B - a.b0;
   if (grid <= 0) return PTY_OK;
-  if (S == 64) hipLaunchKernelGGL(k_mixed<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+  if (S == 32) hipLaunchKernelGGL(k_mixed<32>, dim3(grid), dim3(Frame<32>::T), 0, stream, a);
+  else if (S == 64) hipLaunchKernelGGL(k_mixed<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
   else hipLaunchKernelGGL(k_mixed<128>, dim3(grid), dim3(Frame<128>::T), 0, stream, a);
   return check_launch("k_mixed");
 }
@@ -1252,14 +1256,18 @@ int launch_mixed(const FusedArgs& a, int S, hipStream_t stream) {
 int launch_projection(const FusedArgs& a, int S, hipStream_t stream) {
   const int grid = a.B - a.b0;
   if (grid <= 0) return PTY_OK;
-  if (S == 64) hipLaunchKernelGGL(k_projection<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
+  if (S == 32) hipLaunchKernelGGL(k_projection<32>, dim3(grid), dim3(Frame<32>::T), 0, stream, a);
+  else if (S == 64) hipLaunchKernelGGL(k_projection<64>, dim3(grid), dim3(Frame<64>::T), 0, stream, a);
   else hipLaunchKernelGGL(k_projection<128>, dim3(grid), dim3(Frame<128>::T), 0, stream, a);
   return check_launch("k_projection");
 }
 
 int launch_fft(const cf* in, cf* out, int batch, int S, bool inv, int natural, hipStream_t stream) {
   if (batch <= 0) return PTY_OK;
-  if (S == 64) {
+  if (S == 32) {
+    if (inv) hipLaunchKernelGGL((k_fft2<32, true>), dim3(batch), dim3(Frame<32>::T), 0, stream, in, out, natural);
+    else hipLaunchKernelGGL((k_fft2<32, false>), dim3(batch), dim3(Frame<32>::T), 0, stream, in, out, natural);
+  } else if (S == 64) {
     if (inv) hipLaunchKernelGGL((k_fft2<64, true>), dim3(batch), dim3(Frame<64>::T), 0, stream, in, out, natural);
     else hipLaunchKernelGGL((k_fft2<64, false>), dim3(batch), dim3(Frame<64>::T), 0, stream, in, out, natural);
   } else {
@@ -1294,7 +1302,10 @@ int launch_object_update_fft(cf* obj, const cf* grad, const float* spsum, const 
                              int Py, float step, float nmin, const cf* probe, cf* phat, int S, hipStream_t stream) {
   const long long npix = (long long)Px * Py;
   const float c1 = (float)(1.0 - (double)nmin), c2 = nmin;
-  if (S == 64)
+  if (S == 32)
+    hipLaunchKernelGGL(k_object_update_fft<32>, dim3(1 + grid_for(npix, Frame<32>::T)), dim3(Frame<32>::T), 0, stream,
+                       obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, probe, phat);
+  else if (S == 64)
     hipLaunchKernelGGL(k_object_update_fft<64>, dim3(1 + grid_for(npix, Frame<64>::T)), dim3(Frame<64>::T), 0, stream,
                        obj, grad, spsum, tilemax, ntiles, npix, step, c1, c2, probe, phat);
   else
@@ -1743,7 +1754,8 @@ int launch_constraints_potential(float* V, int Px, int Py, const unsigned char* 
 }
 
 int launch_probe_com(cf* probe, int S, hipStream_t stream) {
-  if (S == 64) hipLaunchKernelGGL(k_probe_com<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe);
+  if (S == 32) hipLaunchKernelGGL(k_probe_com<32>, dim3(1), dim3(Frame<32>::T), 0, stream, probe);
+  else if (S == 64) hipLaunchKernelGGL(k_probe_com<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe);
   else hipLaunchKernelGGL(k_probe_com<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe);
   return check_launch("k_probe_com");
 }
@@ -1755,7 +1767,9 @@ int launch_probe_extra(cf* probe, int K, int S, const pty_gd_params& q, hipStrea
   int rc = PTY_OK;
   if (q.fix_probe_aperture) {
     if (!q.probe_aperture) return fail(PTY_E_ARG, "fix_probe_aperture needs the initial aperture");
-    if (S == 64) hipLaunchKernelGGL(k_probe_aperture<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe,
+    if (S == 32) hipLaunchKernelGGL(k_probe_aperture<32>, dim3(1), dim3(Frame<32>::T), 0, stream, probe,
+                                    q.probe_aperture);
+    else if (S == 64) hipLaunchKernelGGL(k_probe_aperture<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe,
                                     q.probe_aperture);
     else hipLaunchKernelGGL(k_probe_aperture<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe,
                             q.probe_aperture);
@@ -1763,7 +1777,8 @@ int launch_probe_extra(cf* probe, int K, int S, const pty_gd_params& q, hipStrea
   }
   if (q.constrain_probe_amplitude) {
     const double rad = q.probe_amplitude_radius, wid = q.probe_amplitude_width;
-    if (S == 64) hipLaunchKernelGGL(k_probe_amplitude<64>, dim3(K), dim3(Frame<64>::T), 0, stream, probe, rad, wid);
+    if (S == 32) hipLaunchKernelGGL(k_probe_amplitude<32>, dim3(K), dim3(Frame<32>::T), 0, stream, probe, rad, wid);
+    else if (S == 64) hipLaunchKernelGGL(k_probe_amplitude<64>, dim3(K), dim3(Frame<64>::T), 0, stream, probe, rad, wid);
     else hipLaunchKernelGGL(k_probe_amplitude<128>, dim3(K), dim3(Frame<128>::T), 0, stream, probe, rad, wid);
     if ((rc = check_launch("k_probe_amplitude"))) return rc;
   }
@@ -1772,18 +1787,21 @@ int launch_probe_extra(cf* probe, int K, int S, const pty_gd_params& q, hipStrea
 
 // every one of K modes shifted by mode 0's centre of mass (shift: 2 floats of scratch)
 int launch_probe_com_modes(cf* probe, int K, int S, float* shift, hipStream_t stream) {
-  if (S == 64) hipLaunchKernelGGL(k_probe_com_measure<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe, shift);
+  if (S == 32) hipLaunchKernelGGL(k_probe_com_measure<32>, dim3(1), dim3(Frame<32>::T), 0, stream, probe, shift);
+  else if (S == 64) hipLaunchKernelGGL(k_probe_com_measure<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe, shift);
   else hipLaunchKernelGGL(k_probe_com_measure<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe, shift);
   int rc = check_launch("k_probe_com_measure");
   if (rc) return rc;
-  if (S == 64) hipLaunchKernelGGL(k_probe_shift_modes<64>, dim3(K), dim3(Frame<64>::T), 0, stream, probe, shift);
+  if (S == 32) hipLaunchKernelGGL(k_probe_shift_modes<32>, dim3(K), dim3(Frame<32>::T), 0, stream, probe, shift);
+  else if (S == 64) hipLaunchKernelGGL(k_probe_shift_modes<64>, dim3(K), dim3(Frame<64>::T), 0, stream, probe, shift);
   else hipLaunchKernelGGL(k_probe_shift_modes<128>, dim3(K), dim3(Frame<128>::T), 0, stream, probe, shift);
   return check_launch("k_probe_shift_modes");
 }
 
 int launch_position_correction(const PosArgs& a, int S, hipStream_t stream) {
   if (a.B <= 0) return PTY_OK;
-  if (S == 64) hipLaunchKernelGGL(k_position_correction<64>, dim3(a.B), dim3(Frame<64>::T), 0, stream, a);
+  if (S == 32) hipLaunchKernelGGL(k_position_correction<32>, dim3(a.B), dim3(Frame<32>::T), 0, stream, a);
+  else if (S == 64) hipLaunchKernelGGL(k_position_correction<64>, dim3(a.B), dim3(Frame<64>::T), 0, stream, a);
   else hipLaunchKernelGGL(k_position_correction<128>, dim3(a.B), dim3(Frame<128>::T), 0, stream, a);
   return check_launch("k_position_correction");
 }
@@ -1866,7 +1884,7 @@ WsLayout layout(const pty_problem& p, const pty_gd_params& q, bool positions = f
 
 int validate_problem(const pty_problem* p) {
   if (!p) return fail(PTY_E_ARG, "problem is NULL");
-  if (!frame_supported(p->S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported (64 or 128)", p->S);
+  if (!frame_supported(p->S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported (32, 64 or 128)", p->S);
   if (p->Px < p->S || p->Py < p->S)
     return fail(PTY_E_ARG, "object %dx%d smaller than the %dx%d frame", p->Px, p->Py, p->S, p->S);
   if (p->N <= 0) return fail(PTY_E_ARG, "no scan positions");
@@ -2245,13 +2263,15 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
           ak.ps = ps + (size_t)k * nb * S2;
           ak.far = far + (size_t)k * S2;
           ak.acc_add = k > 0;
-          if (p.S == 64)
+          if (p.S == 32) hipLaunchKernelGGL((k_position_correction<32, true>), dim3(nb), dim3(Frame<32>::T), 0, stream, ak);
+          else if (p.S == 64)
             hipLaunchKernelGGL((k_position_correction<64, true>), dim3(nb), dim3(Frame<64>::T), 0, stream, ak);
           else
             hipLaunchKernelGGL((k_position_correction<128, true>), dim3(nb), dim3(Frame<128>::T), 0, stream, ak);
           if ((rc = check_launch("k_position_correction"))) return rc;
         }
-        if (p.S == 64) hipLaunchKernelGGL(k_position_solve<64>, dim3(nb), dim3(Frame<64>::T), 0, stream, base);
+        if (p.S == 32) hipLaunchKernelGGL(k_position_solve<32>, dim3(nb), dim3(Frame<32>::T), 0, stream, base);
+        else if (p.S == 64) hipLaunchKernelGGL(k_position_solve<64>, dim3(nb), dim3(Frame<64>::T), 0, stream, base);
         else hipLaunchKernelGGL(k_position_solve<128>, dim3(nb), dim3(Frame<128>::T), 0, stream, base);
         if ((rc = check_launch("k_position_solve"))) return rc;
       }
@@ -2295,7 +2315,8 @@ int pty_permute_frames(const float* in, float* out, int count, int S, int direct
   const long long total = (long long)count * S * S;
   if (total == 0) return PTY_OK;
   const int to_kernel = direction == 0;
-  if (S == 64) hipLaunchKernelGGL(k_permute<64>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
+  if (S == 32) hipLaunchKernelGGL(k_permute<32>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
+  else if (S == 64) hipLaunchKernelGGL(k_permute<64>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
   else hipLaunchKernelGGL(k_permute<128>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
   return check_launch("k_permute");
 }
@@ -2494,7 +2515,8 @@ int pty_loss_gradients(const void* object, const void* probe, int Px, int Py, co
     if (grad_probe) {
       a.chi_in = chi;
       a.pgrad_out = pg;
-      if (S == 64) hipLaunchKernelGGL(k_probe_grad<64>, dim3(nb), dim3(Frame<64>::T), 0, stream, a);
+      if (S == 32) hipLaunchKernelGGL(k_probe_grad<32>, dim3(nb), dim3(Frame<32>::T), 0, stream, a);
+      else if (S == 64) hipLaunchKernelGGL(k_probe_grad<64>, dim3(nb), dim3(Frame<64>::T), 0, stream, a);
       else hipLaunchKernelGGL(k_probe_grad<128>, dim3(nb), dim3(Frame<128>::T), 0, stream, a);
       if ((rc = check_launch("k_probe_grad"))) return rc;
       const int nchunk = probe_nchunk(nb);
@@ -2531,7 +2553,8 @@ int pty_simulate_intensities(const void* probe_hat, const void* object, int Px, 
   FusedArgs a{};
   a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions;
   a.order = nullptr; a.N = N; a.B = N; a.Px = Px; a.Py = Py; a.b0 = 0; a.sim_out = out;
-  if (S == 64) hipLaunchKernelGGL(k_simulate<64>, dim3(N), dim3(Frame<64>::T), 0, st(stream), a);
+  if (S == 32) hipLaunchKernelGGL(k_simulate<32>, dim3(N), dim3(Frame<32>::T), 0, st(stream), a);
+  else if (S == 64) hipLaunchKernelGGL(k_simulate<64>, dim3(N), dim3(Frame<64>::T), 0, st(stream), a);
   else hipLaunchKernelGGL(k_simulate<128>, dim3(N), dim3(Frame<128>::T), 0, st(stream), a);
   return check_launch("k_simulate");
 }

@@ -13,7 +13,7 @@ import torch
 from . import _lib
 from ._lib import PtyGdParams, PtyProblem, c_void_p, ptr, stream_handle
 
-SUPPORTED_FRAMES = (64, 128)
+SUPPORTED_FRAMES = (32, 64, 128)
 
 
 def _req(t, dtype, shape=None, name="tensor"):
@@ -28,6 +28,10 @@ def _req(t, dtype, shape=None, name="tensor"):
     return t
 
 
+# radix split N1 x N2 of each frame size (N1 along a row pass, N2 lanes per line)
+FRAME_RADIX = {32: (8, 4), 64: (8, 8), 128: (16, 8)}
+
+
 def check_frame(S):
     if S not in SUPPORTED_FRAMES:
         raise NotImplementedError(f"frame size {S} not supported (compiled for {SUPPORTED_FRAMES})")
@@ -36,7 +40,7 @@ def check_frame(S):
 def kernel_index(S):
     """Natural frequency index held at each kernel-layout position."""
     check_frame(S)
-    N1, N2 = (16, 8) if S == 128 else (8, 8)
+    N1, N2 = FRAME_RADIX[S]
     p = np.arange(S)
     return p // N2 + N1 * (p % N2)
 
@@ -312,7 +316,7 @@ def far_to_kernel_layout(far):
     kernel layout (element j of row-pass thread t -> row t // (S/N1),
     column N1 * (t % (S/N1)) + j)."""
     S = far.shape[-1]
-    N1 = 16 if S == 128 else 8
+    N1 = FRAME_RADIX[S][0]
     x = far.reshape(far.shape[0], N1, S, S // N1)  # [b][j][row][q], t = row * (S/N1) + q
     return x.permute(0, 2, 3, 1).reshape(far.shape[0], S, S).contiguous()
 

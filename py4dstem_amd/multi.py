@@ -37,6 +37,10 @@ class MultiPtychography:
     def _runs(self, num_iter, kwargs):
         runs = []
         for p, kw in zip(self.items, kwargs):
+            if int(p._region_of_interest_shape[0]) not in (64, 128):
+                raise NotImplementedError("the batched runner takes 64 or 128 px frames; "
+                                          f"{int(p._region_of_interest_shape[0])} px frames run "
+                                          "through SingleslicePtychography.reconstruct()")
             if getattr(p, "_num_probes", 1) > 1:
                 raise NotImplementedError("mixed-state reconstructions run through their own "
                                           "reconstruct(), not the batched runner")

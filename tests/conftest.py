@@ -54,10 +54,24 @@ def small_case():
     return w, o
 
 
+@pytest.fixture(scope="session")
+def small_case32():
+    """The same kind of workload with 32 px frames (16x16 scan, 128^2 object,
+    0.4 A pixels) plus a preprocessed oracle instance."""
+    from oracle import OracleSingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+
+    w = small_workload(seed=2, scan=(16, 16), S=32, object_px=128)
+    init = dict(w["init"], initial_object_guess=w["init"]["initial_object_guess"].copy())
+    o = OracleSingleslicePtychography(datacube=w["datacube"], **init)
+    o.preprocess(**w["preprocess"])
+    return w, o
+
+
 def far_staging_to_kernel(far):
     """(B, S, S) far field from the fused pass's staging layout (element j of
     row-pass thread t at [j][t]) to the kernel layout (NumPy)."""
     S = far.shape[-1]
-    N1 = 16 if S == 128 else 8
+    N1 = {32: 8, 64: 8, 128: 16}[S]
     x = far.reshape(far.shape[0], N1, S, S // N1)
     return np.ascontiguousarray(x.transpose(0, 2, 3, 1).reshape(far.shape[0], S, S))

@@ -188,7 +188,8 @@ int emu_fused(int S, const float* probe_hat, const float* object, int Px, int Py
   a.order = order;
   a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0;
   a.ps_out = (cf*)ps; a.chi_out = (cf*)chi; a.err = err; a.fwd_out = (cf*)fwd_out;
-  if (S == 64) run_fused<64>(a);
+  if (S == 32) run_fused<32>(a);
+  else if (S == 64) run_fused<64>(a);
   else if (S == 128) run_fused<128>(a);
   else return -1;
   return 0;
@@ -252,14 +253,16 @@ extern "C" {
 int emu_fft2_perm(int S, int inverse, const float* in, float* out) {
   const cf* i = (const cf*)in;
   cf* o = (cf*)out;
-  if (S == 64) { if (inverse) fft_inv<64>(i, o); else fft_fwd<64>(i, o); }
+  if (S == 32) { if (inverse) fft_inv<32>(i, o); else fft_fwd<32>(i, o); }
+  else if (S == 64) { if (inverse) fft_inv<64>(i, o); else fft_fwd<64>(i, o); }
   else if (S == 128) { if (inverse) fft_inv<128>(i, o); else fft_fwd<128>(i, o); }
   else return -1;
   return 0;
 }
 
 int emu_kidx(int S, int p) {
-  if (S == 64) return Frame<64>::kidx(p);
+  if (S == 32) return Frame<32>::kidx(p);
+  else if (S == 64) return Frame<64>::kidx(p);
   if (S == 128) return Frame<128>::kidx(p);
   return -1;
 }
@@ -306,7 +309,8 @@ int emu_simulate(int S, const float* probe_hat, const float* object, int Px, int
   FusedArgs a{};
   a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions;
   a.N = N; a.B = N; a.Px = Px; a.Py = Py; a.sim_out = out;
-  if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < N; ++b) simulate_position<64>(run, a, b); }
+  if (S == 32) { EmuRunner<32> run(&a); for (int b = 0; b < N; ++b) simulate_position<32>(run, a, b); }
+  else if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < N; ++b) simulate_position<64>(run, a, b); }
   else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < N; ++b) simulate_position<128>(run, a, b); }
   else return -1;
   return 0;
@@ -320,7 +324,8 @@ int emu_mixed(int S, const float* probe_hat, int K, const float* object, int Px,
   a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions; a.amps = amps;
   a.order = order; a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0; a.K = K;
   a.ps_out = (cf*)ps; a.chi_out = (cf*)chi; a.err = err; a.fwd_out = (cf*)far;
-  if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) mixed_position<64>(run, a, b); }
+  if (S == 32) { EmuRunner<32> run(&a); for (int b = 0; b < B; ++b) mixed_position<32>(run, a, b); }
+  else if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) mixed_position<64>(run, a, b); }
   else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < B; ++b) mixed_position<128>(run, a, b); }
   else return -1;
   return 0;
@@ -337,7 +342,8 @@ int emu_projection(int S, const float* probe_hat, const float* object, int Px, i
   a.ps_out = (cf*)ps; a.chi_in = (const cf*)exits; a.chi_out = (cf*)exits; a.err = err;
   a.proj_a = (float)pa; a.proj_b = (float)pb; a.proj_c = (float)pc;
   a.proj_x = (float)(1.0 - pa - pb); a.proj_y = (float)(1.0 - pc); a.exit_valid = exit_valid;
-  if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) projection_position<64>(run, a, b); }
+  if (S == 32) { EmuRunner<32> run(&a); for (int b = 0; b < B; ++b) projection_position<32>(run, a, b); }
+  else if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) projection_position<64>(run, a, b); }
   else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < B; ++b) projection_position<128>(run, a, b); }
   else return -1;
   return 0;
@@ -356,7 +362,11 @@ int emu_position_correction(int S, const float* ps, const float* far, const floa
   a.clip_update = clip_update; a.max_update = max_update;
   a.clip_total = clip_total; a.max_total = max_total;
   a.sums = sums; a.singular = singular;
-  if (S == 64) {
+  if (S == 32) {
+    EmuRunner<32, PosCorr<32>::Regs> run(nullptr);
+    for (int b = 0; b < B; ++b) position_correction_position<32>(run, a, b);
+  }
+  else if (S == 64) {
     EmuRunner<64, PosCorr<64>::Regs> run(nullptr);
     for (int b = 0; b < B; ++b) position_correction_position<64>(run, a, b);
   } else if (S == 128) {
@@ -375,7 +385,8 @@ int emu_probe_grad(int S, const float* chi, const float* object, int Px, int Py,
   a.object = (const cf*)object; a.positions = positions; a.order = order;
   a.N = N; a.B = B; a.Px = Px; a.Py = Py;
   a.chi_in = (const cf*)chi; a.pgrad_out = (cf*)out;
-  if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) probe_grad_position<64>(run, a, b); }
+  if (S == 32) { EmuRunner<32> run(&a); for (int b = 0; b < B; ++b) probe_grad_position<32>(run, a, b); }
+  else if (S == 64) { EmuRunner<64> run(&a); for (int b = 0; b < B; ++b) probe_grad_position<64>(run, a, b); }
   else if (S == 128) { EmuRunner<128> run(&a); for (int b = 0; b < B; ++b) probe_grad_position<128>(run, a, b); }
   else return -1;
   return 0;

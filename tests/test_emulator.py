@@ -11,11 +11,17 @@ import oracle.ptycho_oracle as po
 from tests.conftest import cptr, far_staging_to_kernel, rel
 
 
+@pytest.fixture(params=[64, 32], ids=["S64", "S32"])
+def frame_case(request):
+    """The 64 px and the 32 px small workloads."""
+    return request.getfixturevalue("small_case" if request.param == 64 else "small_case32")
+
+
 def kidx(emu, S):
     return np.array([emu.emu_kidx(S, p) for p in range(S)])
 
 
-@pytest.mark.parametrize("S", [64, 128])
+@pytest.mark.parametrize("S", [32, 64, 128])
 def test_fft_passes_match_numpy_and_are_race_free(emu, S):
     emu.emu_reset()
     rng = np.random.default_rng(S)
@@ -33,9 +39,9 @@ def test_fft_passes_match_numpy_and_are_race_free(emu, S):
     assert emu.emu_violations() == 0, emu.emu_first_violation()
 
 
-def test_fused_position_matches_oracle_forward(emu, small_case):
-    w, o = small_case
-    S = 64
+def test_fused_position_matches_oracle_forward(emu, frame_case):
+    w, o = frame_case
+    S = o._amplitudes.shape[-1]
     rng = np.random.default_rng(4)
     obj = (o._object * np.exp(1j * 0.1 * rng.standard_normal(o._object.shape))).astype(np.complex64)
     probe = o._probe.astype(np.complex64)
@@ -63,9 +69,9 @@ def test_fused_position_matches_oracle_forward(emu, small_case):
     assert rel(farn, F) < 1e-6
 
 
-def test_fused_position_edge_wrap_and_half_pixels(emu):
+@pytest.mark.parametrize("S", [32, 64])
+def test_fused_position_edge_wrap_and_half_pixels(emu, S):
     """Positions on the object edges (periodic wrap) and exact .5 fractions."""
-    S = 64
     rng = np.random.default_rng(8)
     Px, Py = 64, 70  # object as small as the frame
     obj = np.exp(1j * rng.uniform(0, 0.5, (Px, Py))).astype(np.complex64)
@@ -138,11 +144,11 @@ def test_scatter_multi_segment_order(emu, small_case):
     np.testing.assert_array_equal(grad, po.scatter_bincount(np.conj(ps), pos[order], (S, S), shape))
 
 
-def test_probe_gradient_pass_matches_oracle(emu, small_case):
+def test_probe_gradient_pass_matches_oracle(emu, frame_case):
     """Backward probe pass: fft2(conj(O_b) chi_b) conj(T_b) per position;
     summed and inverse transformed it is the oracle's true probe adjoint."""
-    _, o = small_case
-    S = 64
+    _, o = frame_case
+    S = o._amplitudes.shape[-1]
     rng = np.random.default_rng(13)
     obj = (o._object * np.exp(1j * 0.1 * rng.standard_normal(o._object.shape))).astype(np.complex64)
     B = 10
@@ -248,7 +254,7 @@ def test_scatter_of_32px_stitching_windows(emu):
 def _position_batch(emu, o, seed=4, B=12):
     """Fused pass of a batch, the oracle's update of the object, and the
     inputs of the position-correction phases."""
-    S = 64
+    S = o._amplitudes.shape[-1]
     rng = np.random.default_rng(seed)
     obj = (o._object * np.exp(1j * 0.3 * rng.standard_normal(o._object.shape))).astype(np.complex64)
     probe = o._probe.astype(np.complex64)
@@ -287,10 +293,10 @@ def _emu_positions(emu, d, clip_update=0, max_update=0.0, clip_total=0, max_tota
     return pos, sums, int(singular[0])
 
 
-def test_position_correction_phases_match_oracle(emu, small_case):
+def test_position_correction_phases_match_oracle(emu, frame_case):
     """Position-correction phases (two shifted-object FFTs, float64 sums,
     2x2 solve) against the oracle's _position_correction on one batch."""
-    _, o = small_case
+    _, o = frame_case
     d = _position_batch(emu, o)
     order = d["order"]
     pos, sums, singular = _emu_positions(emu, d)
@@ -338,11 +344,11 @@ def test_position_correction_clips_match_oracle(emu, small_case):
 
 @pytest.mark.parametrize("abc,valid", [((-1.0, 1, 2.0), 1), ((1 - 2 * 0.6, 0.6, 2), 1),
                                        ((0, 1, 2), 0)])
-def test_projection_phases_match_oracle(emu, small_case, abc, valid):
+def test_projection_phases_match_oracle(emu, frame_case, abc, valid):
     """Projection-set exit-wave step (DM, RAAR, SUPERFLIP's first iteration)
     against the oracle's _projection_sets_fourier_projection, race-free."""
-    _, o = small_case
-    S = 64
+    _, o = frame_case
+    S = o._amplitudes.shape[-1]
     rng = np.random.default_rng(21)
     obj = (o._object * np.exp(1j * 0.2 * rng.standard_normal(o._object.shape))).astype(np.complex64)
     probe = o._probe.astype(np.complex64)
@@ -371,11 +377,11 @@ def test_projection_phases_match_oracle(emu, small_case, abc, valid):
 
 
 @pytest.mark.parametrize("K", [1, 3])
-def test_mixed_state_phases_match_oracle(emu, small_case, K):
+def test_mixed_state_phases_match_oracle(emu, frame_case, K):
     """Mixed-state phases (K probe modes, staged far fields, common amplitude
     factor) against the oracle's mixed forward, race-free."""
-    _, o = small_case
-    S = 64
+    _, o = frame_case
+    S = o._amplitudes.shape[-1]
     rng = np.random.default_rng(31 + K)
     obj = (o._object * np.exp(1j * 0.2 * rng.standard_normal(o._object.shape))).astype(np.complex64)
     np.random.seed(5)

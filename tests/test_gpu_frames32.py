@@ -1,0 +1,103 @@
+"""32 px diffraction patterns (e.g. 128 px frames binned by 4) through the
+same kernels as 64 and 128 px ones: radix 8 x 4 transforms, one work group of
+128 threads per position.  Whole reconstructions against the oracle:
+object, probe and every per-iteration error <= 1e-4 (relative L2 for arrays); position updates within 1e-3 px."""
+
+import numpy as np
+import pytest
+import scipy.ndimage
+
+from oracle import OracleMixedstatePtychography, OracleSingleslicePtychography
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    from py4dstem_amd import _lib
+    _lib.load()
+    return torch.device("cuda", 0)
+
+
+def _init_object(shape, seed):
+    rng = np.random.default_rng(seed)
+    return np.exp(0.02j * scipy.ndimage.gaussian_filter(rng.standard_normal(shape), 3)
+                  / 0.05).astype(np.complex64)
+
+
+CASES = {
+    "gd-probe": dict(max_batch_size=64, fix_probe=False),
+    "gd-full-batch": dict(max_batch_size=None, fix_probe=True),
+    "positions": dict(max_batch_size=64, fix_probe=False, fix_positions=False),
+    "mixed-k2": dict(max_batch_size=64, fix_probe=False, orthogonalize_probe=False),
+    "dm_ap": dict(max_batch_size=None, fix_probe=False, reconstruction_method="DM_AP"),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_reconstruct_32px_matches_oracle(dev, case):
+    from py4dstem_amd import MixedstatePtychography
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say(f"32 px frames: {case}")
+    w = small_workload(seed=2, scan=(16, 16), S=32, object_px=128)
+    mixed = case.startswith("mixed")
+    recon = dict(w["reconstruct"], num_iter=4, **CASES[case])
+    init_obj = _init_object((128, 128), 7)
+    out = {}
+    pairs = ((("gpu", MixedstatePtychography), ("oracle", OracleMixedstatePtychography)) if mixed
+             else (("gpu", SingleslicePtychography), ("oracle", OracleSingleslicePtychography)))
+    for kind, cls in pairs:
+        kw = dict(w["init"], initial_object_guess=init_obj.copy())
+        if mixed:
+            kw["num_probes"] = 2
+        p = cls(datacube=w["datacube"], **kw)
+        np.random.seed(5)
+        p.preprocess(**w["preprocess"])
+        np.random.seed(1234)
+        out[kind] = p.reconstruct(**recon)
+    g, o = out["gpu"], out["oracle"]
+    assert g.object.shape == (128, 128) and np.asarray(g.probe).shape[-1] == 32
+    err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(np.asarray(g.probe), o.probe):.2e} "
+        f"error {err:.2e}")
+    assert rel(g.object, o.object) <= 1e-4
+    assert rel(np.asarray(g.probe), o.probe) <= 1e-4
+    assert err <= 1e-4
+    if case == "positions":
+        # position updates (pixels), as in the 64/128 px position tests
+        dg = np.asarray(g._positions_px) - np.asarray(g._positions_px_initial)
+        do = o._positions_px - o._positions_px_initial
+        say(f"positions: max move {np.abs(do).max():.3f} px, max diff {np.abs(dg - do).max():.2e} px")
+        assert np.abs(do).max() > 1e-3
+        assert np.abs(dg - do).max() <= 1e-3
+
+
+def test_fft2_32px_matches_numpy(dev):
+    from py4dstem_amd import kernels as K
+    rng = np.random.default_rng(9)
+    x = (rng.standard_normal((5, 32, 32)) + 1j * rng.standard_normal((5, 32, 32))).astype(np.complex64)
+    t = torch.from_numpy(x).to(dev)
+    f = K.fft2(t).cpu().numpy()
+    assert rel(f, np.fft.fft2(x.astype(np.complex128))) < 5e-7
+    b = K.fft2(torch.from_numpy(f).to(dev), inverse=True).cpu().numpy()
+    assert rel(b, x) < 5e-7
+
+
+def test_batched_runner_rejects_32px(dev):
+    from py4dstem_amd.multi import MultiPtychography
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    w = small_workload(seed=2, scan=(8, 8), S=32, object_px=96)
+    p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+    p.preprocess(**w["preprocess"])
+    with pytest.raises(NotImplementedError):
+        MultiPtychography([p]).reconstruct(2)
