@@ -200,11 +200,11 @@ def test_staged_scatter_is_bitwise_identical(emu, small_case):
         np.testing.assert_array_equal(t, outs[0][2])
 
 
-def test_simulation_pass_matches_forward_model(emu, small_case):
+def test_simulation_pass_matches_forward_model(emu, frame_case):
     """Forward-model pass: |fft2(P_shift_b O_b)|^2 in natural order, equal to
     the oracle's forward model (and race-free)."""
-    _, o = small_case
-    S = 64
+    _, o = frame_case
+    S = o._amplitudes.shape[-1]
     rng = np.random.default_rng(15)
     obj = (o._object * np.exp(1j * 0.2 * rng.standard_normal(o._object.shape))).astype(np.complex64)
     pos = o._positions_px[:6].copy()

@@ -39,6 +39,7 @@ CASES = {
     "positions": dict(max_batch_size=64, fix_probe=False, fix_positions=False),
     "mixed-k2": dict(max_batch_size=64, fix_probe=False, orthogonalize_probe=False),
     "dm_ap": dict(max_batch_size=None, fix_probe=False, reconstruction_method="DM_AP"),
+    "potential": dict(max_batch_size=64, fix_probe=False),
 }
 
 
@@ -59,6 +60,9 @@ def test_reconstruct_32px_matches_oracle(dev, case):
         kw = dict(w["init"], initial_object_guess=init_obj.copy())
         if mixed:
             kw["num_probes"] = 2
+        if case == "potential":
+            kw["object_type"] = "potential"
+            kw["initial_object_guess"] = (np.angle(init_obj) + 0.05).astype(np.float32)
         p = cls(datacube=w["datacube"], **kw)
         np.random.seed(5)
         p.preprocess(**w["preprocess"])
