@@ -68,6 +68,12 @@ def small_case32():
     return w, o
 
 
+@pytest.fixture(params=[64, 32], ids=["S64", "S32"])
+def frame_case(request):
+    """The 64 px and the 32 px small workloads."""
+    return request.getfixturevalue("small_case" if request.param == 64 else "small_case32")
+
+
 def far_staging_to_kernel(far):
     """(B, S, S) far field from the fused pass's staging layout (element j of
     row-pass thread t at [j][t]) to the kernel layout (NumPy)."""

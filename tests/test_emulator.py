@@ -11,12 +11,6 @@ import oracle.ptycho_oracle as po
 from tests.conftest import cptr, far_staging_to_kernel, rel
 
 
-@pytest.fixture(params=[64, 32], ids=["S64", "S32"])
-def frame_case(request):
-    """The 64 px and the 32 px small workloads."""
-    return request.getfixturevalue("small_case" if request.param == 64 else "small_case32")
-
-
 def kidx(emu, S):
     return np.array([emu.emu_kidx(S, p) for p in range(S)])
 

@@ -42,10 +42,10 @@ def _tensors(o, obj, probe, dev):
     return O, P, pos, amps
 
 
-def test_autograd_matches_oracle_adjoint_small(dev, small_case):
+def test_autograd_matches_oracle_adjoint_small(dev, frame_case):
     from py4dstem_amd.autograd import ptychographic_loss
-    say("autograd small vs oracle")
-    _, o = small_case
+    _, o = frame_case
+    say(f"autograd small vs oracle, {o._amplitudes.shape[-1]} px frames")
     rng = np.random.default_rng(31)
     obj = (o._object * np.exp(1j * 0.1 * rng.standard_normal(o._object.shape))).astype(np.complex64)
     probe = o._probe.astype(np.complex64)
