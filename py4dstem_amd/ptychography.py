@@ -243,8 +243,6 @@ class SingleslicePtychography:
         if self._datacube is None:
             raise ValueError("The preprocess() method requires a DataCube. "
                              "Please run ptycho.attach_datacube(DataCube) first.")
-        if region_of_interest_shape is not None:
-            raise NotImplementedError("region_of_interest_shape (resampled exit waves) is not supported")
         reshape = (diffraction_intensities_shape is not None
                    or padded_diffraction_intensities_shape is not None)
         if reshape and (self._vacuum_probe_intensity is not None or dp_mask is not None):
@@ -288,6 +286,14 @@ class SingleslicePtychography:
             raise NotImplementedError("only square diffraction patterns are supported")
         S = int(shape[0])
         K.check_frame(S)
+        if region_of_interest_shape is not None and tuple(
+                int(v) for v in np.ravel(region_of_interest_shape)) != (S, S):
+            # the reference resamples every exit wave between the two shapes
+            # (ptychographic_methods.py:1603-1631); at the patterns' own shape
+            # that order-1 zoom by 1 returns its input, so the run is the one
+            # without it
+            raise NotImplementedError("region_of_interest_shape different from the diffraction "
+                                      "pattern shape (resampled exit waves) is not supported")
         if self._semiangle_cutoff_pixels:
             self._semiangle_cutoff = self._semiangle_cutoff_pixels * self._angular_sampling[0]
 

@@ -31,7 +31,7 @@ def dev():
     return torch.device("cuda", 0)
 
 
-@pytest.mark.parametrize("S", [64, 128])
+@pytest.mark.parametrize("S", [32, 64, 128])
 def test_shift_amplitudes_bitwise(dev, S):
     from py4dstem_amd import kernels as K
     say(f"shift amplitudes S={S}")
@@ -170,3 +170,26 @@ def test_resampled_preprocess_matches_oracle(dev, dis, method, pad):
     assert rel(g._probe, o._probe) <= 1e-6
     g.reconstruct(num_iter=1, max_batch_size=16, progress_bar=False)
     assert np.isfinite(g.error)
+
+
+def test_region_of_interest_at_pattern_shape_is_plain_run(dev):
+    """region_of_interest_shape equal to the patterns' shape: the reference's
+    exit-wave resampling (ptychographic_methods.py:1603-1631) is then an
+    order-1 zoom by 1, i.e. its input, so the reconstruction equals the one
+    without it bit for bit; another shape raises NotImplementedError."""
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say("region_of_interest_shape at the pattern shape")
+    w = small_workload(seed=1, scan=(8, 8), object_px=128)
+    out = []
+    for roi in (None, (64, 64)):
+        p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+        p.preprocess(**w["preprocess"], region_of_interest_shape=roi)
+        p.reconstruct(num_iter=2, max_batch_size=16, fix_probe=False, seed_random=3,
+                      progress_bar=False)
+        out.append((p.object.copy(), np.asarray(p.probe).copy(), list(p.error_iterations)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
+    p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+    with pytest.raises(NotImplementedError):
+        p.preprocess(**w["preprocess"], region_of_interest_shape=(48, 48))
