@@ -333,6 +333,19 @@ int pty_projection_run(const pty_problem* problem, const pty_gd_params* params,
                        const pty_projection_params* projection, const int* order, double* errors,
                        void* workspace, size_t workspace_bytes, pty_stream_t stream);
 
+/* pty_projection_run with position correction (fix_positions=False;
+ * singleslice_ptychography.py:913-926 after the projection-set adjoint,
+ * ptychographic_methods.py:2071-2225): the far field of P_shift * O from the
+ * iteration's forward object and probe, the updated object, the same
+ * position parameters and centre-of-mass constraint as
+ * pty_gd_run_positions.  Workspace: pty_projection_positions_workspace_bytes
+ * (whole-batch P_shift, chi and far-field staging). */
+size_t pty_projection_positions_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);
+int pty_projection_run_positions(const pty_problem* problem, const pty_gd_params* params,
+                                 const pty_projection_params* projection, const pty_position_params* positions,
+                                 const int* order, double* errors, void* workspace, size_t workspace_bytes,
+                                 pty_stream_t stream);
+
 /* The exit-wave step of one batch (slots 0..B-1 of order; exit_waves[k] is
  * slot k): ps[b] = P_shift (natural), err[b], exit waves advanced in place. */
 int pty_projection_batch(const void* probe_hat, const void* object, int Px, int Py,

@@ -155,6 +155,12 @@ _SIGNATURES = {
     "pty_projection_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams),
                                    ctypes.POINTER(PtyProjectionParams), c_void_p, c_void_p,
                                    c_void_p, c_size_t, c_void_p]),
+    "pty_projection_positions_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem),
+                                                            ctypes.POINTER(PtyGdParams)]),
+    "pty_projection_run_positions": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams),
+                                             ctypes.POINTER(PtyProjectionParams),
+                                             ctypes.POINTER(PtyPositionParams), c_void_p, c_void_p,
+                                             c_void_p, c_size_t, c_void_p]),
     "pty_projection_batch": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p,
                                      c_void_p, c_int, c_int, ctypes.POINTER(PtyProjectionParams),
                                      c_void_p, c_void_p, c_void_p]),

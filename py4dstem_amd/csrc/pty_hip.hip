@@ -2041,7 +2041,8 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
 // exit waves advanced in place, object and probe replaced by their normalised
 // adjoints, then the same constraints as gradient descent
 int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_projection_params* jp,
-             const int* order, double* errors, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+             const int* order, double* errors, void* workspace, size_t workspace_bytes, hipStream_t stream,
+             const pty_position_params* pp = nullptr) {
   int rc = validate_problem(problem);
   if (rc) return rc;
   if (!params || !jp) return fail(PTY_E_ARG, "params are NULL");
@@ -2054,7 +2055,11 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
   if (!order || !errors || !workspace || !jp->exit_waves) return fail(PTY_E_ARG, "null order/errors/workspace/exit waves");
   if (p.fourier_mask) return fail(PTY_E_UNSUPPORTED, "a detector Fourier mask is not defined for projection sets");
   if (!aligned8(jp->exit_waves)) return fail(PTY_E_ARG, "exit waves must be 8-byte aligned");
-  const WsLayout L = layout(p, q, false, true);
+  if ((rc = validate_position_params(pp, p.positions))) return rc;
+  const bool correct = pp != nullptr;
+  // position correction needs P_shift and fft2(P_shift * O) of every position
+  // from the object before the update: whole-batch ps, chi and far buffers
+  const WsLayout L = correct ? layout(p, q, true, false) : layout(p, q, false, true);
   if (workspace_bytes < L.total)
     return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.total);
   char* ws = (char*)workspace;
@@ -2067,6 +2072,8 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
   double* ppart = (double*)(ws + L.ppart);
   double* cpart = (double*)(ws + L.cpart);
   double* runbuf = (double*)(ws + L.run);
+  cf* chi = (cf*)(ws + L.chi);
+  cf* far = (cf*)(ws + L.far);
   cf* exits = (cf*)jp->exit_waves;
   const int stage = stage_size(q);
   cf* obj = (cf*)p.object;
@@ -2115,14 +2122,28 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
       }
     }
     valid = 1;
+    hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
+    if ((rc = check_launch("k_error_reduce"))) return rc;
+    if (correct) {
+      // the overlap's far field with the object and probe of this iteration's
+      // forward pass (singleslice_ptychography.py:913-926 passes `overlap`);
+      // the gradient-descent pass forms it (its chi and err are not used)
+      FusedArgs a{};
+      a.probe_hat = phat; a.object = obj; a.positions = p.positions; a.amps = p.amplitudes;
+      a.order = order; a.N = p.N; a.B = p.N; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
+      a.ps_out = ps; a.chi_out = chi; a.err = err; a.fwd_out = far;
+      if ((rc = launch_fused(a, p.S, stream))) return rc;
+    }
     if ((rc = launch_object_update(obj, grad, spsum, tilemax, ntiles, p.Px, p.Py, 0.f, q.normalization_min, stream,
                                    1)))
       return rc;
     if (!q.fix_probe) {
       if ((rc = launch_probe_apply(probe, ppart, p.S, p.N, 0.f, q.normalization_min, stream, 1))) return rc;
     }
-    hipLaunchKernelGGL(k_error_reduce, dim3(1), dim3(1024), 0, stream, err, p.N, norm, errors + it);
-    if ((rc = check_launch("k_error_reduce"))) return rc;
+    if (correct) {  // after the adjoint, with the updated object
+      const PosArgs pa = position_args(*pp, ps, far, obj, p.Px, p.Py, p.amplitudes, p.N, order, p.N);
+      if ((rc = launch_position_correction(pa, p.S, stream))) return rc;
+    }
     if ((rc = launch_object_filters(obj, p.Px, p.Py, q, grad, cpart, stream))) return rc;
     if ((rc = launch_constraints(obj, p.Px, p.Py, p.fov_mask_inverse, q.shrinkage_rad, q.pure_phase_object, cpart,
                                  stream)))
@@ -2131,6 +2152,9 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
       if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
     }
     if (!q.fix_probe && (rc = launch_probe_extra(probe, 1, p.S, q, stream))) return rc;
+    if (correct && pp->center_of_mass) {
+      if ((rc = launch_positions_com(pp->positions, p.N, pp->initial_com, stream))) return rc;
+    }
   }
   return PTY_OK;
 }
@@ -2605,6 +2629,19 @@ int pty_projection_run(const pty_problem* problem, const pty_gd_params* params,
                        const pty_projection_params* projection, const int* order, double* errors,
                        void* workspace, size_t workspace_bytes, pty_stream_t stream) {
   return proj_run(problem, params, projection, order, errors, workspace, workspace_bytes, st(stream));
+}
+
+size_t pty_projection_positions_workspace_bytes(const pty_problem* problem, const pty_gd_params* params) {
+  if (!problem || !params || params->max_batch <= 0) return 0;
+  return layout(*problem, *params, true, false).total;
+}
+
+int pty_projection_run_positions(const pty_problem* problem, const pty_gd_params* params,
+                                 const pty_projection_params* projection, const pty_position_params* positions,
+                                 const int* order, double* errors, void* workspace, size_t workspace_bytes,
+                                 pty_stream_t stream) {
+  if (!positions) return fail(PTY_E_ARG, "position params are NULL");
+  return proj_run(problem, params, projection, order, errors, workspace, workspace_bytes, st(stream), positions);
 }
 
 int pty_projection_batch(const void* probe_hat, const void* object, int Px, int Py, const float* positions,

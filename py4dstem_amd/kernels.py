@@ -245,6 +245,20 @@ def projection_run(problem, params, proj_params, order, errors, workspace):
               workspace.numel(), stream_handle())
 
 
+def projection_positions_workspace_bytes(problem, params):
+    import ctypes
+    return int(_lib.load().pty_projection_positions_workspace_bytes(ctypes.byref(problem),
+                                                                    ctypes.byref(params)))
+
+
+def projection_run_positions(problem, params, proj_params, position_params, order, errors,
+                             workspace):
+    import ctypes
+    _lib.call("pty_projection_run_positions", ctypes.byref(problem), ctypes.byref(params),
+              ctypes.byref(proj_params), ctypes.byref(position_params), ptr(order), ptr(errors),
+              ptr(workspace), workspace.numel(), stream_handle())
+
+
 def projection_batch(probe_hat, obj, positions, amplitudes_k, order, proj_params):
     """Exit-wave step of the projection sets for the slots of ``order``
     (exit waves of proj_params advanced in place); returns (P_shift, err)."""

@@ -651,9 +651,9 @@ class SingleslicePtychography:
                     "Stochastic object/probe updating is inconsistent with 'DM_AP', 'RAAR', "
                     "'RRR', and 'SUPERFLIP'. Use reconstruction_method='GD' or set "
                     "max_batch_size=None.")
-            if not fix_positions:
-                raise NotImplementedError("position correction with projection-set methods is "
-                                          "not supported on this path")
+            if not fix_positions and global_affine_transformation:
+                raise NotImplementedError("the global affine position constraint with "
+                                          "projection-set methods is not supported on this path")
             if detector_fourier_mask is not None:  # ptychographic_methods.py:1686-1687
                 raise NotImplementedError()
         tv = tv_denoise and tv_denoise_weight is not None
@@ -800,7 +800,7 @@ class SingleslicePtychography:
             proj_params = K.projection_params(self._exit_waves, *projection,
                                               exit_valid=self._exit_valid)
             nbytes = K.projection_workspace_bytes(problem, params)
-        elif not fix_positions:
+        if not fix_positions:
             # position correction (singleslice_ptychography.py:913-926) and the
             # centre-of-mass constraint, which the reference applies when its
             # own flag arrives False (fix_positions_com and not fix_positions
@@ -816,8 +816,9 @@ class SingleslicePtychography:
                 else max_position_total_distance / norm,
                 center_of_mass=not fix_positions_com,
                 initial_com=self._positions_px_initial.mean(0), singular=singular)
-            nbytes = K.gd_positions_workspace_bytes(problem, params)
-        else:
+            nbytes = (K.gd_positions_workspace_bytes(problem, params) if projection is None
+                      else K.projection_positions_workspace_bytes(problem, params))
+        elif projection is None:
             nbytes = K.gd_workspace_bytes(problem, params)
         ws = torch.empty(nbytes, dtype=torch.uint8, device=dev) if _allocate else None
         orders_dev = torch.from_numpy(orders).to(dev)
@@ -852,8 +853,12 @@ class SingleslicePtychography:
                 np.random.shuffle(run["shuffled"])
                 orders_dev[it].copy_(torch.from_numpy(run["shuffled"].astype(np.int32)))
             if run["projection_params"] is not None:
-                K.projection_run(problem, params, run["projection_params"], orders_dev[0],
-                                 errors_dev[it:], ws)
+                if pos_params is None:
+                    K.projection_run(problem, params, run["projection_params"], orders_dev[0],
+                                     errors_dev[it:], ws)
+                else:
+                    K.projection_run_positions(problem, params, run["projection_params"],
+                                               pos_params, orders_dev[0], errors_dev[it:], ws)
                 run["projection_params"].exit_valid = 1
                 self._exit_valid = True
             elif pos_params is None:

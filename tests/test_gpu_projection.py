@@ -159,8 +159,47 @@ def test_projection_argument_errors(dev):
                       reconstruction_parameter_a=0.1)
     with pytest.raises(ValueError):
         p.reconstruct(num_iter=1, reconstruction_method="ePIE")
+    with pytest.raises(NotImplementedError):
+        p.reconstruct(num_iter=1, reconstruction_method="DM_AP", fix_positions=False,
+                      global_affine_transformation=True)
     # a gradient-descent run afterwards drops the projection state
     p.reconstruct(num_iter=1, reconstruction_method="SUPERFLIP", progress_bar=False, reset=True)
     assert p._exit_waves is not None
     p.reconstruct(num_iter=1, progress_bar=False, reset=False)
     assert p._exit_waves is None
+
+
+@pytest.mark.parametrize("method,com", [("DM_AP", False), ("RAAR", True)])
+def test_projection_sets_with_position_correction_match_oracle(dev, method, com):
+    """Projection sets with fix_positions=False: the positions are corrected
+    after the projection-set adjoint from the forward overlap's far field and
+    the updated object (singleslice_ptychography.py:913-926), as in the
+    oracle; object, probe, errors <= 1e-4 and position updates within
+    1e-3 px."""
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say(f"{method} with position correction, fix_positions_com={com}")
+    w = small_workload(seed=1)
+    # two iterations at step 0.1: the oracle's own complex64/complex128 gap
+    # stays at ~1e-6 (object) and ~5e-5 px (positions); longer runs of these
+    # methods with moving positions are chaotic
+    recon = dict(w["reconstruct"], max_batch_size=None, fix_probe=False, num_iter=2,
+                 fix_positions=False, fix_positions_com=com, positions_step_size=0.1,
+                 **METHODS[method])
+    init_obj = _init_object((256, 256), 17)
+    out = {}
+    for kind, cls in (("gpu", SingleslicePtychography), ("oracle", OracleSingleslicePtychography)):
+        p = cls(datacube=w["datacube"], **dict(w["init"], initial_object_guess=init_obj.copy()))
+        p.preprocess(**w["preprocess"])
+        out[kind] = p.reconstruct(**recon)
+    g, o = out["gpu"], out["oracle"]
+    err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
+    dg = np.asarray(g._positions_px) - np.asarray(g._positions_px_initial)
+    do = o._positions_px - o._positions_px_initial
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(g.probe, o.probe):.2e} error {err:.2e} "
+        f"max move {np.abs(do).max():.3f} px, max diff {np.abs(dg - do).max():.2e} px")
+    assert np.abs(do).max() > 1e-3
+    assert rel(g.object, o.object) <= 1e-4
+    assert rel(g.probe, o.probe) <= 1e-4
+    assert err <= 1e-4
+    assert np.abs(dg - do).max() <= 1e-3
