@@ -11,7 +11,10 @@
  * Conventions
  *  - Complex arrays are complex64: interleaved float32 (re, im), 8-byte aligned.
  *  - All buffers are device memory owned by the caller; the library keeps no
- *    pointer after a call returns and allocates nothing.
+ *    pointer after a call returns and allocates no device memory.
+ *    pty_multi_run is the one entry that creates runtime objects (a stream,
+ *    two events, a graph) and waits for its work before returning; every
+ *    other entry is asynchronous and creates none.
  *  - Every call is asynchronous on `stream` (a hipStream_t, NULL = default
  *    stream) and returns 0 on success or a negative PTY_E* code; the message
  *    is available from pty_last_error() on the calling thread.

@@ -77,7 +77,25 @@ PINNING = {
     "OracleMixedstatePtychography.forward/adjoint": "K=1 equals the single-mode path; data "
                                                     "from the mixed model is a fixed point; FP64 "
                                                     "finite differences of the mixed loss",
-    "reconstruct": "composition of the above; c64 vs c128 sensitivity measured",
+    "bilinear_resample": "zoom by 1 returns the input values; the float64 divisor promotes "
+                         "complex64 to complex128 (NumPy >= 2 scalar rules)",
+    "solve_com_relative_rotation": "curl-free (gradient) CoM fields rotated by a known angle, "
+                                   "plain and transposed: the search returns that angle and "
+                                   "transpose with zero curl; a radial field rotated by 90 "
+                                   "degrees for the divergence branch; the forced-rotation "
+                                   "transpose decision on the same fields",
+    "reconstruct": "zero object: first error = sum A^2 / (mean_I N) = 1; object c*truth: "
+                   "first error = (1 - c)^2 (pins the error normalisation); noise-free truth "
+                   "with fixed probe and |O| = 1, zero phase outside the field: every error "
+                   "~0 and the object passes the constraint sequence unchanged; one "
+                   "iteration equals the separately pinned forward/adjoint/constraint "
+                   "functions composed by hand",
+    "adjoint (step normalisation)": "one position: normalization_min = 0 gives step*chi/P "
+                                    "and step*chi/O_patch, = 1 gives step*conj(P)chi/max|P|^2 "
+                                    "and step*conj(O)chi/max|O|^2; several positions at "
+                                    "normalization_min = 1: the object step is "
+                                    "-step/(2 S^2 max overlap) times the FP64 finite-difference "
+                                    "gradient of the loss",
 }
 
 
@@ -592,6 +610,91 @@ def affine_ransac(positions0, positions1, origin=(0, 0), translation_allowed=Tru
     return transform, best_fitness
 
 
+def bilinear_resample(array, output_size):
+    """``bilinear_resample`` (phase/utils.py:2345-2428) with the arguments of
+    its calls in the forward projection: vectorised order-1 ``zoom`` with
+    mode 'grid-wrap' and grid_mode=True over the last two axes, then division
+    by the product of the scale factors as a float64 array (which promotes a
+    complex64 input to complex128 under NumPy >= 2)."""
+    size = np.array(array.shape)
+    input_size = size[-2:].copy()
+    output_size = np.array(output_size)
+    scale_output = tuple(output_size / input_size)
+    scale_output = (1,) * (size.size - input_size.size) + scale_output
+    array = scipy.ndimage.zoom(array, scale_output, order=1, mode="grid-wrap", grid_mode=True)
+    return array / np.array(scale_output).prod()
+
+
+def _rotated_com(nx, ny, theta, transposed):
+    """CoM field rotated by theta, from the transposed intensities when
+    ``transposed`` (phase_base_class.py:892-899, 907-914)."""
+    if transposed:
+        return (np.cos(theta) * ny - np.sin(theta) * nx,
+                np.sin(theta) * ny + np.cos(theta) * nx)
+    return (np.cos(theta) * nx - np.sin(theta) * ny,
+            np.sin(theta) * nx + np.cos(theta) * ny)
+
+
+def _mean_abs_curl(cx, cy):
+    """phase_base_class.py:904-906: mean |d cy/dx - d cx/dy| with central
+    differences on the interior (last two axes)."""
+    gxy = cx[..., 1:-1, 2:] - cx[..., 1:-1, :-2]
+    gyx = cy[..., 2:, 1:-1] - cy[..., :-2, 1:-1]
+    return np.mean(np.abs(gyx - gxy), axis=(-2, -1))
+
+
+def _mean_abs_div(cx, cy):
+    """phase_base_class.py:900-903: mean |d cx/dx + d cy/dy| on the interior."""
+    gxx = cx[..., 2:, 1:-1] - cx[..., :-2, 1:-1]
+    gyy = cy[..., 1:-1, 2:] - cy[..., 1:-1, :-2]
+    return np.mean(np.abs(gxx + gyy), axis=(-2, -1))
+
+
+def solve_com_relative_rotation(com_normalized_x, com_normalized_y, rotation_angles_deg=None,
+                                maximize_divergence=False, force_com_rotation=None,
+                                force_com_transpose=None):
+    """``_solve_for_center_of_mass_relative_rotation`` (phase_base_class.py:
+    769-1345, plots and summaries omitted).  Four branches: rotation and
+    transpose forced (:868-882); rotation forced, transpose decided by
+    comparing the two curls/divergences at that angle (:884-938); transpose
+    forced, rotation the argmin curl / argmax divergence over the angles
+    (:940-1067); both searched, the transposed field winning only when
+    strictly better (:1069-1190).  Angles are float32 degrees
+    (np.arange(-89, 90, 1) by default).  Returns (rotation_rad, transpose,
+    com_x, com_y) with the corrected CoM of :1265-1283."""
+    nx, ny = com_normalized_x, com_normalized_y
+    score = _mean_abs_div if maximize_divergence else _mean_abs_curl
+    if rotation_angles_deg is None:
+        rotation_angles_deg = np.arange(-89.0, 90.0, 1.0)
+    if force_com_rotation is not None:
+        rotation = np.deg2rad(force_com_rotation)
+        if force_com_transpose is not None:
+            transpose = force_com_transpose
+        else:
+            plain = score(*_rotated_com(nx, ny, rotation, False))
+            flipped = score(*_rotated_com(nx, ny, rotation, True))
+            transpose = flipped > plain if maximize_divergence else flipped < plain
+    else:
+        angles_deg = np.asarray(rotation_angles_deg, dtype=np.float32)
+        angles_rad = np.deg2rad(angles_deg)[:, None, None]
+        flat_angles = np.squeeze(angles_rad)
+        pick = np.argmax if maximize_divergence else np.argmin
+        if force_com_transpose is not None:
+            transpose = force_com_transpose
+            s = score(*_rotated_com(nx[None], ny[None], angles_rad, transpose))
+            rotation = flat_angles[pick(s)]
+        else:
+            s_plain = score(*_rotated_com(nx[None], ny[None], angles_rad, False))
+            s_flip = score(*_rotated_com(nx[None], ny[None], angles_rad, True))
+            i, j = pick(s_plain), pick(s_flip)
+            plain_wins = (s_plain[i] >= s_flip[j]) if maximize_divergence else (
+                s_plain[i] <= s_flip[j])
+            transpose = not plain_wins
+            rotation = flat_angles[j] if transpose else flat_angles[i]
+    com_x, com_y = _rotated_com(nx, ny, rotation, transpose)
+    return rotation, bool(transpose), com_x, com_y
+
+
 class _Cal:
     def __init__(self, R_size, R_units, Q_size, Q_units):
         self._r = (R_size, R_units)
@@ -737,11 +840,15 @@ class OracleSingleslicePtychography:
             com_fy = np.asarray(force_com_shifts[1], dtype=np.float32)
         self._com_fitted_x, self._com_fitted_y = com_fx, com_fy
 
-        # rotation (phase_base_class.py:769-1345) -- forced branch only
-        if force_com_rotation is None or force_com_transpose is None:
-            raise NotImplementedError("oracle covers force_com_rotation/transpose only")
-        self._rotation_best_rad = np.deg2rad(force_com_rotation)
-        self._rotation_best_transpose = force_com_transpose
+        # normalised CoM (phase_base_class.py:749-757) and the rotation /
+        # transpose search (phase_base_class.py:769-1345)
+        self._com_normalized_x = np.nan_to_num(com_mx - com_fx) * self._reciprocal_sampling[0]
+        self._com_normalized_y = np.nan_to_num(com_my - com_fy) * self._reciprocal_sampling[1]
+        (self._rotation_best_rad, self._rotation_best_transpose, self._com_x,
+         self._com_y) = solve_com_relative_rotation(
+            self._com_normalized_x, self._com_normalized_y,
+            kwargs.get("rotation_angles_deg"), kwargs.get("maximize_divergence", False),
+            force_com_rotation, force_com_transpose)
 
         # normalisation (phase_base_class.py:1347-1460)
         mean_intensity = 0
@@ -758,6 +865,13 @@ class OracleSingleslicePtychography:
         self._mean_diffraction_intensity = mean_intensity
         self._num_diffraction_patterns = diff.shape[0]
         self._amplitudes_shape = np.array(diff.shape[-2:])
+        # singleslice_ptychography.py:431-436: a given region of interest turns
+        # on the far-field resampling in the forward projection; only the
+        # patterns' own shape is restated here
+        roi = kwargs.get("region_of_interest_shape")
+        self._resample_exit_waves = roi is not None
+        if roi is not None and tuple(int(v) for v in np.ravel(roi)) != tuple(diff.shape[-2:]):
+            raise NotImplementedError("oracle: region_of_interest_shape other than the patterns'")
         self._region_of_interest_shape = np.array(diff.shape[-2:])
 
         # positions (phase_base_class.py:1811-1903)
@@ -876,6 +990,9 @@ class OracleSingleslicePtychography:
             object_patches = np.exp(1j * object_patches)
         overlap = shifted_probes * object_patches
         fourier_overlap = _fft2(overlap, mode)
+        resample = getattr(self, "_resample_exit_waves", False)
+        if resample:  # ptychographic_methods.py:1603-1611
+            fourier_overlap = bilinear_resample(fourier_overlap, self._amplitudes_shape)
         if fourier_mask is not None:
             fourier_overlap *= fourier_mask
         farfield = np.abs(fourier_overlap)
@@ -884,6 +1001,8 @@ class OracleSingleslicePtychography:
         modified = modified - fourier_overlap
         if fourier_mask is not None:
             modified *= fourier_mask
+        if resample:  # ptychographic_methods.py:1620-1629
+            modified = bilinear_resample(modified, self._region_of_interest_shape)
         exit_waves = _ifft2(modified, mode)
         return shifted_probes, object_patches, exit_waves, error
 

@@ -61,10 +61,11 @@ PTY_HD float frsq(float x) {
 
 // correctly rounded float32 square root: the hardware estimate r is moved to
 // a neighbour when x lies beyond the midpoint between them (midpoints squared
-// exactly in float64)
+// exactly in float64).  NaN stays NaN (as NumPy's sqrt); negative inputs,
+// which only rounding could produce here, give 0.
 PTY_HD float sqrt_rn(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  if (!(x > 0.f)) return x == 0.f ? x : 0.f;
+  if (!(x > 0.f)) return x < 0.f ? 0.f : x;
   const float r = __builtin_amdgcn_sqrtf(x);
   const float lo = __uint_as_float(__float_as_uint(r) - 1u);
   const float hi = __uint_as_float(__float_as_uint(r) + 1u);
@@ -74,7 +75,7 @@ PTY_HD float sqrt_rn(float x) {
   if (xd > m_hi * m_hi) return hi;
   return r;
 #else
-  return x > 0.f ? std::sqrt(x) : 0.f;
+  return x < 0.f ? 0.f : std::sqrt(x);
 #endif
 }
 
@@ -481,7 +482,8 @@ template <int S> struct Fused {
       const float m2 = cabs2(f);
       cf g;
       float mag;
-      if (m2 > 0.f) {
+      // |F| = 0 takes angle 0 (NumPy's angle(0)); a NaN far field stays NaN
+      if (m2 != 0.f) {
         const float inv = frsq(m2);
         mag = m2 * inv;
         g = PROJ ? mkcf(amp * (f.x * inv), amp * (f.y * inv))

@@ -1148,7 +1148,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_aperture(cf* __restrict__
     for (int j = 0; j < F::N1; ++j) {
       const cf f = r.v[j];
       const float m2 = cabs2(f);
-      const cf u = m2 > 0.f ? cscale(f, frsq(m2)) : mkcf(1.f, 0.f);  // exp(i angle F)
+      const cf u = m2 != 0.f ? cscale(f, frsq(m2)) : mkcf(1.f, 0.f);  // exp(i angle F); NaN stays NaN
       r.v[j] = cscale(u, ap[j]);
     }
     Pass<S>::template B<true>(t, r.v, tw);
@@ -2075,7 +2075,9 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
   cf* chi = (cf*)(ws + L.chi);
   cf* far = (cf*)(ws + L.far);
   cf* exits = (cf*)jp->exit_waves;
-  const int stage = stage_size(q);
+  // with position correction the layout holds whole-batch staging and no
+  // running sums, so every launch covers all N positions
+  const int stage = correct ? p.N : stage_size(q);
   cf* obj = (cf*)p.object;
   cf* probe = (cf*)p.probe;
   const int ntiles = num_tiles(p.Px, p.Py);

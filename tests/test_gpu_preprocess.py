@@ -173,23 +173,80 @@ def test_resampled_preprocess_matches_oracle(dev, dis, method, pad):
 
 
 def test_region_of_interest_at_pattern_shape_is_plain_run(dev):
-    """region_of_interest_shape equal to the patterns' shape: the reference's
-    exit-wave resampling (ptychographic_methods.py:1603-1631) is then an
-    order-1 zoom by 1, i.e. its input, so the reconstruction equals the one
-    without it bit for bit; another shape raises NotImplementedError."""
+    """region_of_interest_shape equal to the patterns' shape: the reference
+    then resamples the far field (ptychographic_methods.py:1603-1631) by an
+    order-1 zoom by 1, which keeps the values, divided by a float64 scale
+    product that moves its projection to complex128.  The drop-in runs the
+    plain complex64 projection: bit for bit its run without the setting, and
+    within 1e-4 of the oracle's run with the resampling branch (object,
+    probe, every error); another shape raises NotImplementedError."""
     from py4dstem_amd.ptychography import SingleslicePtychography
     from py4dstem_amd.synthetic import small_workload
     say("region_of_interest_shape at the pattern shape")
     w = small_workload(seed=1, scan=(8, 8), object_px=128)
+    rng = np.random.default_rng(17)
+    init_obj = np.exp(0.02j * scipy.ndimage.gaussian_filter(rng.standard_normal((128, 128)), 3)
+                      / 0.05).astype(np.complex64)
+    recon = dict(num_iter=2, max_batch_size=16, fix_probe=False, seed_random=3)
     out = []
     for roi in (None, (64, 64)):
-        p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
+        p = SingleslicePtychography(datacube=w["datacube"],
+                                    **dict(w["init"], initial_object_guess=init_obj.copy()))
         p.preprocess(**w["preprocess"], region_of_interest_shape=roi)
-        p.reconstruct(num_iter=2, max_batch_size=16, fix_probe=False, seed_random=3,
-                      progress_bar=False)
+        p.reconstruct(progress_bar=False, **recon)
         out.append((p.object.copy(), np.asarray(p.probe).copy(), list(p.error_iterations)))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2]
+    o = po.OracleSingleslicePtychography(datacube=w["datacube"],
+                                         **dict(w["init"], initial_object_guess=init_obj.copy()))
+    o.preprocess(**w["preprocess"], region_of_interest_shape=(64, 64))
+    assert o._resample_exit_waves
+    o.reconstruct(**recon)
+    err = float(np.max(np.abs(np.array(out[1][2]) / np.array(o.error_iterations) - 1)))
+    say(f"vs oracle with resampling: object {rel(out[1][0], o.object):.2e} "
+        f"probe {rel(out[1][1], o.probe):.2e} error {err:.2e}")
+    assert rel(out[1][0], o.object) <= 1e-4
+    assert rel(out[1][1], o.probe) <= 1e-4
+    assert err <= 1e-4
     p = SingleslicePtychography(datacube=w["datacube"], **w["init"])
     with pytest.raises(NotImplementedError):
         p.preprocess(**w["preprocess"], region_of_interest_shape=(48, 48))
+
+
+def test_centred_positions_and_default_object_match_oracle(dev):
+    """center_positions_in_fov=True (singleslice_ptychography.py:467-478) with
+    the default object size (ptychographic_methods.py:40-79) and a raster
+    scan from the calibration: positions, object shape, probe and FOV mask
+    equal the oracle's preprocessing, and a short reconstruction from that
+    state matches it to 1e-4."""
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say("centred positions, default object size")
+    w = small_workload(seed=6, scan=(10, 9), object_px=160)
+    init = {k: v for k, v in w["init"].items()
+            if k not in ("initial_object_guess", "initial_scan_positions")}
+    pre = dict(w["preprocess"], center_positions_in_fov=True)
+    g = SingleslicePtychography(datacube=w["datacube"], **init)
+    g.preprocess(**pre)
+    o = po.OracleSingleslicePtychography(datacube=w["datacube"], **init)
+    o.preprocess(**pre)
+    assert g._object.shape == o._object.shape
+    np.testing.assert_array_equal(g._positions_px, o._positions_px)
+    com = o._positions_px.mean(0)
+    np.testing.assert_allclose(com, np.array(o._object.shape) / 2, atol=1e-3)
+    np.testing.assert_array_equal(g._object_fov_mask, o._object_fov_mask)
+    assert rel(g._probe, o._probe) <= 1e-6
+    recon = dict(num_iter=2, max_batch_size=30, fix_probe=False, seed_random=2)
+    g.reconstruct(progress_bar=False, **recon)
+    o.reconstruct(**recon)
+    err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(g.probe, o.probe):.2e} error {err:.2e}")
+    assert err <= 1e-4
+    assert rel(g.probe, o.probe) <= 1e-4
+    # flat start: the object is held to the oracle's own complex64/complex128
+    # gap as in the other flat-start tests
+    o128 = po.OracleSingleslicePtychography(datacube=w["datacube"], mode="c128", **init)
+    o128.preprocess(**pre)
+    o128.reconstruct(**recon)
+    gap = rel(o.object, o128.object)
+    assert rel(g.object, o.object) <= max(1e-4, 3 * gap)

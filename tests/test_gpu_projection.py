@@ -203,3 +203,32 @@ def test_projection_sets_with_position_correction_match_oracle(dev, method, com)
     assert rel(g.probe, o.probe) <= 1e-4
     assert err <= 1e-4
     assert np.abs(dg - do).max() <= 1e-3
+
+
+def test_projection_positions_stage_setting_is_bitwise_neutral(dev):
+    """DM_AP with position correction and a stage of 64 positions per launch
+    (1024 positions) equals the default whole-batch run bit for bit: with
+    position correction every launch covers all positions, whatever the
+    stage setting, because the workspace then holds whole-batch staging and
+    no running sums."""
+    from py4dstem_amd.ptychography import SingleslicePtychography
+    from py4dstem_amd.synthetic import small_workload
+    say("DM_AP with position correction: set_stage(64) vs default")
+    w = small_workload(seed=1)
+    recon = dict(w["reconstruct"], max_batch_size=None, fix_probe=False, num_iter=2,
+                 fix_positions=False, positions_step_size=0.1, **METHODS["DM_AP"])
+    init_obj = _init_object((256, 256), 17)
+    out = []
+    for stage in (None, 64):
+        p = SingleslicePtychography(datacube=w["datacube"],
+                                    **dict(w["init"], initial_object_guess=init_obj.copy()))
+        p.preprocess(**w["preprocess"])
+        if stage is not None:
+            p.set_stage(stage)
+        out.append(p.reconstruct(**recon))
+    a, b = out
+    assert a._num_diffraction_patterns > 64
+    np.testing.assert_array_equal(a.object, b.object)
+    np.testing.assert_array_equal(a.probe, b.probe)
+    np.testing.assert_array_equal(np.asarray(a._positions_px), np.asarray(b._positions_px))
+    assert a.error_iterations == b.error_iterations
