@@ -53,18 +53,59 @@ def algorithmic_bytes_per_update(S, Px, Py, B):
     return 20 * S * S + 16 * (Px * Py + S * S) / B
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
+COUNTERS = os.path.join(ROOT, "profiles", "counters_gd_minibatch_scan128_frame128.md")
+
+
+def measured_traffic(key):
+    """HBM bytes of `key` (a kernel name: per launch; "minibatch": all kernels
+    of one 1024-position mini-batch) from the committed counter summary
     (FETCH_SIZE / WRITE_SIZE passes scaled by a copy-kernel calibration), or
     None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "counters_hot_kernels_scan128_frame128.md")
     try:
-        for line in open(path):
-            if line.startswith(f"traffic_bytes_per_launch {kernel} = "):
+        for line in open(COUNTERS):
+            if line.startswith(f"traffic_bytes {key} = "):
                 return int(line.rsplit("=", 1)[1])
     except OSError:
         pass
     return None
+
+
+def learned_init_summary(seeds, max_iter):
+    """Time to object NMSE <= 1e-3 from the learned and from the flat start
+    (learned_init.time_to_nmse) on held-out seeds at the production shapes;
+    value = total flat seconds / total learned seconds."""
+    from py4dstem_amd import learned_init as li
+
+    net = li.ConvNet.load()
+    per = {}
+    for s in seeds:
+        t = time.perf_counter()
+        r = li.time_to_nmse(li.evaluation_workload(s), net, max_iter=max_iter)
+        per[s] = r
+        log(f"learned-init seed {s} ({time.perf_counter() - t:.1f} s): "
+            + "; ".join(f"{k} nmse0 {r[k]['nmse'][0]:.2e} iterations {r[k]['iterations']} "
+                        f"seconds {r[k]['seconds']}" for k in ("flat", "learned")))
+    def total(kind):
+        v = [per[s][kind]["seconds"] for s in seeds]
+        return None if any(x is None for x in v) else sum(v)
+    t_flat, t_learned = total("flat"), total("learned")
+    return {
+        "metric": "time to object NMSE<=1e-3, flat start / learned start",
+        "value": round(t_flat / t_learned, 3) if (t_flat and t_learned) else None,
+        "unit": "x", "target": 2.0,
+        "flat_seconds": t_flat, "learned_seconds": t_learned,
+        "workload": "held-out Gaussian-atom objects of the initialiser's distribution, 128x128 "
+                    "scan at 0.4 A, 128 px frames, 720x720 object, 1e6 e/A^2, mini-batch 1024, "
+                    "probe update; seconds = initialiser + iterations to target x per-iteration "
+                    "time of one uninterrupted multi-iteration reconstruct()",
+        "per_seed": {str(s): {k: {"iterations": per[s][k]["iterations"],
+                                  "seconds": per[s][k]["seconds"],
+                                  "init_seconds": round(per[s][k]["init_seconds"], 5),
+                                  "s_per_iteration": round(per[s][k]["s_per_iteration"], 5),
+                                  "nmse_start": per[s][k]["nmse"][0],
+                                  "nmse_after_1": per[s][k]["nmse"][1]}
+                              for k in ("flat", "learned")} for s in seeds},
+    }
 
 
 def cpu_model():
@@ -189,12 +230,18 @@ def bench_many(args):
 
 
 def bench_kernels(args):
-    """Hot kernels of one production mini-batch (1024 positions, 128 px frames,
-    720x720 object) launched `--kernel-reps` times each, plus a plain copy of
-    256 MiB: the workload for hardware-counter passes.  The copy calibrates
-    FETCH_SIZE / WRITE_SIZE against a known byte count."""
+    """The production gradient-descent loop on a 1024-position-per-batch
+    problem (128x128 raster at 4 px, 128 px frames, 720x720 object, uniform
+    random amplitudes, mini-batches of 1024 in seeded order, probe update),
+    `--kernel-reps` iterations of pty_gd_run, plus a plain copy of 256 MiB:
+    the workload for hardware-counter passes.  Every kernel of a mini-batch
+    (fused pass, scatter + probe partial sums, probe reduction and step,
+    object step with the probe's transform) appears 16 times per iteration;
+    the copy calibrates FETCH_SIZE / WRITE_SIZE against a known byte
+    count."""
     import torch
 
+    from py4dstem_amd import _lib
     from py4dstem_amd import kernels as K
     from py4dstem_amd.preprocess import center_positions, real_space_sampling, scan_positions_in_pixels
     from py4dstem_amd.probe import electron_wavelength_angstrom
@@ -210,34 +257,36 @@ def bench_kernels(args):
     pos = center_positions(pos.astype(np.float32), (P, P))
     probe = scaled_probe(300e3, S, sampling, 20.0, 2.0, 1.6e5, defocus=200.0).astype(np.complex64)
     obj = np.exp(1j * rng.uniform(0, 0.3, (P, P))).astype(np.complex64)
-    amps = rng.uniform(0, 20, (n * n, S, S)).astype(np.float32)
     o = torch.from_numpy(obj).to(dev)
     pr = torch.from_numpy(probe).to(dev)
     posd = torch.from_numpy(pos).to(dev)
-    ampd = torch.from_numpy(amps).to(dev)
-    del amps
-    order = torch.from_numpy(rng.permutation(n * n)[:B].astype(np.int32)).to(dev)
-    phat = K.probe_spectrum(pr)
+    ampd = torch.rand((n * n, S, S), generator=torch.Generator(device=dev).manual_seed(0),
+                      device=dev) * 20.0
+    N = n * n
+    reps = args.kernel_reps
+    orders = torch.from_numpy(np.stack([rng.permutation(N) for _ in range(reps)]).astype(np.int32)).to(dev)
+    errors = torch.zeros(reps, dtype=torch.float64, device=dev)
+    problem = K.make_problem(o, pr, posd, ampd, None, 20.0 * 20.0 / 3 * S * S, None)
+    params = _lib.PtyGdParams()
+    params.num_iter, params.max_batch, params.fix_probe, params.fix_probe_com = reps, B, 0, 1
+    params.step_size, params.normalization_min = 0.5, 1.0
+    ws = torch.empty(K.gd_workspace_bytes(problem, params), dtype=torch.uint8, device=dev)
     src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
-    for _ in range(args.kernel_reps):
+    for _ in range(reps):
         dst.copy_(src)
-    for _ in range(args.kernel_reps):
-        ps, chi, err = K.forward_adjoint(phat, o, posd, ampd, order)
-    for _ in range(args.kernel_reps):
-        K.scatter(ps, chi, posd, order, (P, P))
-    for _ in range(args.kernel_reps):
-        K.probe_update(pr.clone(), chi, o, posd, order, 0.5, 1.0)
+    K.gd_run(problem, params, orders, errors, ws)
     torch.cuda.synchronize()
     print(json.dumps({
         "copy_bytes": 2 * src.numel() * 4,
-        "k_fused_alg_bytes": B * (4 + 8 + 8 + 8) * S * S,
-        "k_gather_alg_bytes": B * 16 * S * S + P * P * 12,
-        "k_probe_partial_alg_bytes": B * 8 * S * S,
-        "note": "k_fused: amplitudes read, object patch read, P_shift and chi written; "
-                "k_gather: P_shift and chi read, gradient and overlap sum written; "
-                "k_probe_partial: chi read (object patch reads hit the caches)",
+        "minibatches": reps * (N // B),
+        "k_fused_alg_bytes": B * (4 + 8) * S * S + 8 * S * S,
+        "minibatch_alg_bytes": round(algorithmic_bytes_per_update(S, P, P, B) * B),
+        "errors_finite": bool(torch.isfinite(errors).all().item()),
+        "note": "k_fused algorithmic: amplitudes and object patch read, probe spectrum; "
+                "mini-batch algorithmic: 20 S^2 per position plus the object and probe "
+                "read and written once",
     }), flush=True)
 
 
@@ -549,6 +598,9 @@ def main():
                     default="production")
     ap.add_argument("--probe-modes", type=int, default=4)
     ap.add_argument("--problems", type=int, default=64)
+    ap.add_argument("--learned-seeds", type=int, default=2,
+                    help="held-out seeds for the time-to-NMSE sub-object of the default line "
+                         "(0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); the multi-GPU "
@@ -624,8 +676,15 @@ def main():
     del ps, bufs
     k_avg = e0.elapsed_time(e1) * 1e-3 / args.kernel_reps
     per_update = algorithmic_bytes_per_update(S, Px, Py, B)
-    achieved = per_update * B / k_avg / 1e9
-    log(f"k_fused avg {k_avg * 1e6:.1f} us per launch of {B} positions")
+    # the fused pass's own algorithmic bytes: amplitudes and object patch
+    # read per position, the probe spectrum once (its P_shift / chi staging
+    # is not algorithmic: the scatter and probe sums read it back)
+    fused_bytes = B * (4 + 8) * S * S + 8 * S * S
+    achieved = fused_bytes / k_avg / 1e9
+    iter_achieved = pos_per_s * per_update / 1e9
+    log(f"k_fused avg {k_avg * 1e6:.1f} us per launch of {B} positions: "
+        f"{achieved:,.0f} GB/s; iteration level {iter_achieved:,.0f} GB/s")
+    production = (S, B, Px, Py) == (128, 1024, 720, 720)
 
     if args.cpu_seconds > 0:
         cpu_rate, sample = cpu_baseline(p, B, args.cpu_seconds)
@@ -659,9 +718,16 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("k_fused") if (S, B) == (128, 1024) else None,
+            "traffic": measured_traffic("k_fused") if production else None,
             "kernel_avg_us": round(k_avg * 1e6, 2),
-            "bytes_per_update": per_update,
+            "bytes_per_launch": fused_bytes,
+            "iteration": {
+                "achieved": round(iter_achieved, 1),
+                "frac": round(iter_achieved / HBM_PEAK_GBS, 4),
+                "bytes_per_update": per_update,
+                "traffic_per_minibatch": measured_traffic("minibatch") if production else None,
+                "algorithmic_per_minibatch": round(per_update * B),
+            },
         },
         "cpu_baseline": {
             "value": round(cpu_rate, 2) if cpu_rate is not None else None,
@@ -671,7 +737,13 @@ def main():
             "sample": sample,
         },
     }
-    print(json.dumps(out), flush=True)
+    if args.learned_seeds > 0:
+        try:
+            out["learned_init"] = learned_init_summary(list(range(2001, 2001 + args.learned_seeds)),
+                                                       max_iter=4)
+        except Exception as exc:  # reported, never hides the main line
+            out["learned_init"] = {"error": f"{type(exc).__name__}: {exc}"}
+    print(json.dumps(out, default=float), flush=True)
 
 
 if __name__ == "__main__":

@@ -23,7 +23,7 @@ from ._lib import ptr, stream_handle
 
 def _loss_call(obj, probe, positions, amplitudes_k, order, chunk, scale, want_obj, want_probe):
     S = probe.shape[-1]
-    K.check_frame(S)
+    K.check_frame(S, lds_only=True)
     Px, Py = obj.shape
     N = order.numel()
     ws_bytes = _lib.load().pty_loss_workspace_bytes(S, Px, Py, N, chunk)
@@ -52,10 +52,12 @@ class PtychographicLoss(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         obj, probe, positions, amplitudes_k, order = ctx.saved_tensors
-        scale = float(grad_out)
-        _, go, gp = _loss_call(obj, probe, positions, amplitudes_k, order, ctx.chunk, scale,
+        _, go, gp = _loss_call(obj, probe, positions, amplitudes_k, order, ctx.chunk, 1.0,
                                ctx.needs_input_grad[0], ctx.needs_input_grad[1])
-        return go, gp, None, None, None, None
+        # the incoming gradient stays on the device (no host synchronisation)
+        g = grad_out.to(torch.float32)
+        return (go * g if go is not None else None), (gp * g if gp is not None else None), \
+            None, None, None, None
 
 
 def ptychographic_loss(obj, probe, positions, amplitudes_k, order=None, chunk=4096):

@@ -27,6 +27,7 @@
 #else
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #define PTY_HD inline
 #endif
 
@@ -170,6 +171,11 @@ template <int S> struct Geo;
 template <> struct Geo<32> { static constexpr int N1 = 8, N2 = 4; };
 template <> struct Geo<64> { static constexpr int N1 = 8, N2 = 8; };
 template <> struct Geo<128> { static constexpr int N1 = 16, N2 = 8; };
+// 256 px frames are transformed by line passes through global memory (no
+// LDS-resident frame); their "kernel layout" is the natural order, which
+// N1 = S, N2 = 1 expresses (kidx(p) = p).  Only the layout helpers are used
+// at this size.
+template <> struct Geo<256> { static constexpr int N1 = 256, N2 = 1; };
 
 template <int S> struct Frame {
   static constexpr int N1 = Geo<S>::N1;
@@ -204,6 +210,38 @@ template <int S> struct Frame {
   static PTY_HD int far_index(int t, int j) { return j * T + t; }
 };
 
+// element i of an array whose base address is the same for the whole work
+// group (a frame of the batch, the object): the byte offset stays a 32-bit
+// value, so the GPU access is scalar base + 32-bit vector offset instead of
+// 64-bit address arithmetic per element.  Every such array is below 4 GiB
+// from its base (frames are 512 KiB at most; objects below 23,000^2 pixels).
+template <class T> PTY_HD T& elem(T* base, unsigned i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T));
+}
+template <class T> PTY_HD const T& elem(const T* base, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
+}
+
+// bit patterns between float32 and uint32 (LDS tables of offsets)
+PTY_HD float bits_f32(unsigned u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __uint_as_float(u);
+#else
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+#endif
+}
+PTY_HD unsigned f32_bits(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __float_as_uint(f);
+#else
+  unsigned u;
+  std::memcpy(&u, &f, 4);
+  return u;
+#endif
+}
+
 PTY_HD int wrapi(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
@@ -216,12 +254,16 @@ template <int S> struct Pass {
   using F = Frame<S>;
   static constexpr int N1 = F::N1, N2 = F::N2, R = F::R;
 
-  // tw[m] = exp(-2 pi i m / S)
+  // tw: the twiddle table of Fused<S>::twiddles, W^(n2 k1) = exp(-2 pi i n2 k1 / S)
+  // stored twice: at n2 * N1 + k1 (the A pass reads a thread's N1 values at
+  // one base address) and at S + k1 * N2 + n2 (the B' pass reads N2 values
+  // at one base), so every twiddle load is base + immediate offset
   template <bool INV> static PTY_HD void A(int n2, cf* v, const cf* tw) {
     if (!INV) {
       dft<N1, false>(v);
+      const cf* w = tw + n2 * N1;
 #pragma unroll
-      for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+      for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], w[k1]);
     } else {
       dft<N1, true>(v);
     }
@@ -232,9 +274,9 @@ template <int S> struct Pass {
       cf* u = v + g * N2;
       dft<N2, INV>(u);
       if (INV) {
-        const int k1 = R * t + g;
+        const cf* w = tw + S + (R * t + g) * N2;  // k1 = R t + g
 #pragma unroll
-        for (int n2 = 1; n2 < N2; ++n2) u[n2] = cmul(u[n2], cconj(tw[n2 * k1]));
+        for (int n2 = 1; n2 < N2; ++n2) u[n2] = cmul(u[n2], cconj(w[n2]));
       }
     }
   }
@@ -313,16 +355,24 @@ template <int S> struct Fused {
   // extra LDS words after the frame: exp(-2 pi i k f / S) ramp factors for
   // rows (k in [0,S)) then columns
   static constexpr int RAMP = F::LDS_WORDS;
-  static constexpr int LDS_TOTAL = F::LDS_WORDS + 2 * S;
+  // then S words holding the object row offsets row(i) * Py of the window's
+  // rows i (as float bit patterns in .x), written once per position
+  static constexpr int ROWS = F::LDS_WORDS + 2 * S;
+  static constexpr int LDS_TOTAL = F::LDS_WORDS + 3 * S;
 
-  // twiddle table into LDS (tid < S)
+  // twiddle table into LDS (tid < 2 S): W^(n2 k1) at n2 * N1 + k1 for the A
+  // passes and at S + k1 * N2 + n2 for the B' passes (n2 < N2, k1 < N1;
+  // n2 k1 < S), each the float32 rounding of the float64 value
+  static constexpr int TW_WORDS = 2 * S;
   template <class L> static PTY_HD void twiddles(int tid, L tw) {
-    if ((unsigned)tid < (unsigned)S) {
+    if ((unsigned)tid < (unsigned)(2 * S)) {
+      const int u = tid < S ? tid : tid - S;
+      const int m = tid < S ? (u / N1) * (u % N1) : (u / N2) * (u % N2);
 #if defined(__HIP_DEVICE_COMPILE__)
       double sn, cs;
-      sincospi(-2.0 * tid / S, &sn, &cs);
+      sincospi(-2.0 * m / S, &sn, &cs);
 #else
-      double cs = std::cos(-2.0 * M_PI * tid / S), sn = std::sin(-2.0 * M_PI * tid / S);
+      double cs = std::cos(-2.0 * M_PI * m / S), sn = std::sin(-2.0 * M_PI * m / S);
 #endif
       tw.st(tid, mkcf((float)cs, (float)sn));
     }
@@ -372,6 +422,12 @@ template <int S> struct Fused {
       const float frac = tid < S ? px - rx : py - ry;
       // (-2 pi kf) * (-frac) is the exact negation: conj of the forward ramp
       lds.st(RAMP + tid, ramp1(k, conj_ramp ? -frac : frac));
+    }
+    if (tid < S) {  // periodic row of window row `tid`, times the row length
+      int row = r.x0 + F::fidx(tid);
+      row += row < 0 ? a.Px : 0;
+      row -= row >= a.Px ? a.Px : 0;
+      lds.st(ROWS + tid, mkcf(bits_f32((unsigned)row * (unsigned)a.Py), 0.f));
     }
   }
 
@@ -446,13 +502,21 @@ template <int S> struct Fused {
   static PTY_HD void p_exitwave_colA(int tid, const FusedArgs& a, int b, Regs& r, L lds, const cf* tw) {
     p_colAinv_to_regs(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
-    cf* ps_b = a.ps_out ? a.ps_out + (size_t)b * S * S : nullptr;
+    if (a.ps_out) {
+      cf* ps_b = a.ps_out + (size_t)b * S * S;
+#pragma unroll
+      for (int m = 0; m < N1; ++m) elem(ps_b, (n2 + N2 * m) * S + cc) = r.v[m];
+    }
+    int col = r.y0 + F::fidx(cc);
+    col += col < 0 ? a.Py : 0;
+    col -= col >= a.Py ? a.Py : 0;
 #pragma unroll
     for (int m = 0; m < N1; ++m) {
-      const int i = n2 + N2 * m;
-      if (ps_b) ps_b[i * S + cc] = r.v[m];
-      const unsigned o = object_offset(a, r, i, cc);
-      r.v[m] = cmul(r.v[m], POT ? expi(a.object_pot[o]) : a.object[o]);
+      // row offset from the set-up table: window row n2 + N2 m
+      const unsigned o = f32_bits(lds.ld(ROWS + n2 + N2 * m).x) + (unsigned)col;
+      PTY_CHECK(o < (unsigned)a.Px * (unsigned)a.Py && o == object_offset(a, r, n2 + N2 * m, cc),
+                "object index out of range");
+      r.v[m] = cmul(r.v[m], POT ? expi(elem(a.object_pot, o)) : elem(a.object, o));
     }
     P::template A<false>(n2, r.v, tw);
 #pragma unroll
@@ -470,12 +534,12 @@ template <int S> struct Fused {
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
-    cf* far = a.fwd_out ? a.fwd_out + (size_t)b * S * S + tid : nullptr;
+    cf* far = a.fwd_out ? a.fwd_out + (size_t)b * S * S : nullptr;
     const float* fm = MASK ? a.fmask + rr * S + N1 * t : nullptr;
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
-      if (far) far[(size_t)j * T] = r.v[j];
+      if (far) elem(far, j * T + tid) = r.v[j];
       const float mj = MASK ? fm[j] : 1.f;
       const cf f = MASK ? cscale(r.v[j], mj) : r.v[j];
       const float amp = r.amp[j];
@@ -671,7 +735,7 @@ template <int S> struct Fused {
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
     cf* chi_b = a.chi_out + (size_t)b * S * S;
 #pragma unroll
-    for (int m = 0; m < N1; ++m) chi_b[(n2 + N2 * m) * S + cc] = r.v[m];
+    for (int m = 0; m < N1; ++m) elem(chi_b, (n2 + N2 * m) * S + cc) = r.v[m];
   }
 };
 

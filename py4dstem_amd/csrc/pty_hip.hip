@@ -83,6 +83,7 @@ template <int S> struct GpuRunner {
   template <class Fn> __device__ __forceinline__ void phase(Fn fn) {
     int t = tid;
     asm volatile("" : "+v"(t));
+    __builtin_assume(t >= 0 && t < T);  // lets index math use unsigned shifts
     fn(t, r, lds, tw);
   }
   __device__ __forceinline__ void barrier() { __syncthreads(); }
@@ -104,7 +105,7 @@ template <int S> struct GpuRunner {
 template <int S, bool MASK = false, bool POT = false>
 __device__ __forceinline__ void fused_body(const FusedArgs& a, int b) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[Frame<S>::T / 64];
   const int tid = threadIdx.x;
   // threads [2S, 3S) are idle in the setup phase; its closing barrier
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_fused(FusedArgs a) {
 template <int S>
 __global__ __launch_bounds__(Frame<S>::T) void k_mixed(FusedArgs a) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[Frame<S>::T / 64];
   const int tid = threadIdx.x;
   Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the first barrier
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_mixed(FusedArgs a) {
 template <int S>
 __global__ __launch_bounds__(Frame<S>::T) void k_projection(FusedArgs a) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[Frame<S>::T / 64];
   const int tid = threadIdx.x;
   Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_projection(FusedArgs a) {
 template <int S>
 __global__ __launch_bounds__(Frame<S>::T) void k_probe_grad(FusedArgs a) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   const int tid = threadIdx.x;
   Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
   typename Fused<S>::Regs r;
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_grad(FusedArgs a) {
 template <int S>
 __global__ __launch_bounds__(Frame<S>::T) void k_simulate(FusedArgs a) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   const int tid = threadIdx.x;
   Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
   typename Fused<S>::Regs r;
@@ -184,6 +185,7 @@ template <int S> struct GpuPosRunner {
   template <class Fn> __device__ __forceinline__ void phase(Fn fn) {
     int t = tid;
     asm volatile("" : "+v"(t));
+    __builtin_assume(t >= 0 && t < T);  // lets index math use unsigned shifts
     fn(t, r, lds, tw);
   }
   __device__ __forceinline__ void barrier() { __syncthreads(); }
@@ -212,7 +214,7 @@ template <int S> struct GpuPosRunner {
 template <int S, bool ACC = false>
 __global__ __launch_bounds__(Frame<S>::T) void k_position_correction(PosArgs a) {
   __shared__ cf lds_mem[PosCorr<S>::LDS_TOTAL];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[Frame<S>::T / 64 * POS_SUMS];
   const int tid = threadIdx.x;
   Fused<S>::twiddles(tid, LdsPtr{tw});  // published by the setup barrier
@@ -360,7 +362,7 @@ __device__ __forceinline__ void fft2_body(const cf* __restrict__ in, cf* __restr
   using F = Frame<S>;
   using K = Fused<S>;
   __shared__ cf lds_mem[F::LDS_WORDS];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   const int tid = threadIdx.x;
   K::twiddles(tid, LdsPtr{tw});
   __syncthreads();
@@ -902,7 +904,7 @@ template <int S> __device__ __forceinline__ void probe_com_body(cf* __restrict__
   using K = Fused<S>;
   constexpr int T = F::T;
   __shared__ cf lds_mem[F::LDS_WORDS];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[3][T / 64];
   __shared__ float shift[2];
   const int tid = threadIdx.x;
@@ -1025,7 +1027,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_shift_modes(cf* __restric
   using F = Frame<S>;
   using K = Fused<S>;
   __shared__ cf lds_mem[F::LDS_WORDS];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   const int tid = threadIdx.x;
   K::twiddles(tid, LdsPtr{tw});
   LdsPtr lds{lds_mem};
@@ -1114,7 +1116,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_aperture(cf* __restrict__
   using K = Fused<S>;
   constexpr int T = F::T;
   __shared__ cf lds_mem[F::LDS_WORDS];
-  __shared__ cf tw[S];
+  __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[T / 64];
   __shared__ float norm;
   const int tid = threadIdx.x;
@@ -1222,11 +1224,17 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_amplitude(cf* __restrict_
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
+#include "pty_256.inc"
+
+// frames whose transforms run in one work group with the frame in LDS
 bool frame_supported(int S) { return S == 32 || S == 64 || S == 128; }
+// frames of the gradient-descent path: also 256 (line passes, pty_256.inc)
+bool frame_supported_gd(int S) { return frame_supported(S) || S == 256; }
 
 int launch_fused(const FusedArgs& a, int S, hipStream_t stream) {
   const int grid = a.B - a.b0;
   if (grid <= 0) return PTY_OK;
+  if (S == 256) return launch_fused256(a, stream);
   if (a.object_pot) {  // potential object (no detector mask: the caller rejects the pair)
     if (a.fmask) return fail(PTY_E_UNSUPPORTED, "detector masks with potential objects are not supported");
     if (S == 32) hipLaunchKernelGGL((k_fused<32, false, true>), dim3(grid), dim3(Frame<32>::T), 0, stream, a);
@@ -1264,6 +1272,7 @@ int launch_projection(const FusedArgs& a, int S, hipStream_t stream) {
 
 int launch_fft(const cf* in, cf* out, int batch, int S, bool inv, int natural, hipStream_t stream) {
   if (batch <= 0) return PTY_OK;
+  if (S == 256) return launch_fft256(in, out, batch, inv, stream);  // natural order in and out
   if (S == 32) {
     if (inv) hipLaunchKernelGGL((k_fft2<32, true>), dim3(batch), dim3(Frame<32>::T), 0, stream, in, out, natural);
     else hipLaunchKernelGGL((k_fft2<32, false>), dim3(batch), dim3(Frame<32>::T), 0, stream, in, out, natural);
@@ -1753,7 +1762,13 @@ int launch_constraints_potential(float* V, int Px, int Py, const unsigned char* 
   return check_launch("k_constraint_apply_potential");
 }
 
-int launch_probe_com(cf* probe, int S, hipStream_t stream) {
+// probe recentring; 256 px frames need `tmp` (one frame) and `shift` (2
+// floats) of device scratch
+int launch_probe_com(cf* probe, int S, hipStream_t stream, cf* tmp = nullptr, float* shift = nullptr) {
+  if (S == 256) {
+    if (!tmp || !shift) return fail(PTY_E_UNSUPPORTED, "the 256 px probe recentring needs a workspace (pty_gd_run)");
+    return launch_probe_com256(probe, tmp, shift, stream);
+  }
   if (S == 32) hipLaunchKernelGGL(k_probe_com<32>, dim3(1), dim3(Frame<32>::T), 0, stream, probe);
   else if (S == 64) hipLaunchKernelGGL(k_probe_com<64>, dim3(1), dim3(Frame<64>::T), 0, stream, probe);
   else hipLaunchKernelGGL(k_probe_com<128>, dim3(1), dim3(Frame<128>::T), 0, stream, probe);
@@ -1765,6 +1780,8 @@ int launch_probe_com(cf* probe, int S, hipStream_t stream) {
 // every mode
 int launch_probe_extra(cf* probe, int K, int S, const pty_gd_params& q, hipStream_t stream) {
   int rc = PTY_OK;
+  if (S == 256 && (q.fix_probe_aperture || q.constrain_probe_amplitude))
+    return fail(PTY_E_UNSUPPORTED, "probe aperture/amplitude constraints take frames of 32, 64 or 128 px");
   if (q.fix_probe_aperture) {
     if (!q.probe_aperture) return fail(PTY_E_ARG, "fix_probe_aperture needs the initial aperture");
     if (S == 32) hipLaunchKernelGGL(k_probe_aperture<32>, dim3(1), dim3(Frame<32>::T), 0, stream, probe,
@@ -1884,7 +1901,8 @@ WsLayout layout(const pty_problem& p, const pty_gd_params& q, bool positions = f
 
 int validate_problem(const pty_problem* p) {
   if (!p) return fail(PTY_E_ARG, "problem is NULL");
-  if (!frame_supported(p->S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported (32, 64 or 128)", p->S);
+  if (!frame_supported_gd(p->S))
+    return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported (32, 64, 128 or 256)", p->S);
   if (p->Px < p->S || p->Py < p->S)
     return fail(PTY_E_ARG, "object %dx%d smaller than the %dx%d frame", p->Px, p->Py, p->S, p->S);
   if (p->N <= 0) return fail(PTY_E_ARG, "no scan positions");
@@ -1923,6 +1941,8 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
   const bool pot = p.object_type == PTY_OBJECT_POTENTIAL;
   if (pot && (correct || p.fourier_mask || q.butterworth))
     return fail(PTY_E_UNSUPPORTED, "potential objects: no position correction, detector mask or Butterworth filter");
+  if (p.S == 256 && (correct || (!q.fix_probe && (q.fix_probe_aperture || q.constrain_probe_amplitude))))
+    return fail(PTY_E_UNSUPPORTED, "256 px frames: no position correction or probe aperture/amplitude constraint");
   const WsLayout L = layout(p, q, correct);
   if (workspace_bytes < L.total)
     return fail(PTY_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, L.total);
@@ -1949,7 +1969,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
   // complex object with probe update: the object step carries the fft2 of
   // the updated probe (k_object_update_fft), so phat stays current across
   // the mini-batches of an iteration
-  const bool fft_with_update = !q.fix_probe && !pot;
+  const bool fft_with_update = !q.fix_probe && !pot && p.S != 256;
   for (int it = 0; it < q.num_iter; ++it) {
     const int* order_it = orders + (size_t)it * p.N;
     int start = 0;
@@ -2025,7 +2045,7 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
                                  stream)))
       return rc;
     if (!q.fix_probe && q.fix_probe_com) {
-      if ((rc = launch_probe_com(probe, p.S, stream))) return rc;
+      if ((rc = launch_probe_com(probe, p.S, stream, grad, (float*)cpart))) return rc;
     }
     if (!q.fix_probe && (rc = launch_probe_extra(probe, 1, p.S, q, stream))) return rc;
     if (!q.fix_probe) phat_valid = false;  // the recentring and constraints moved the probe
@@ -2048,6 +2068,8 @@ int proj_run(const pty_problem* problem, const pty_gd_params* params, const pty_
   if (!params || !jp) return fail(PTY_E_ARG, "params are NULL");
   const pty_problem& p = *problem;
   const pty_gd_params& q = *params;
+  if (!frame_supported(p.S))
+    return fail(PTY_E_UNSUPPORTED, "projection sets and probe modes take frames of 32, 64 or 128 px, not %d", p.S);
   if (p.object_type != PTY_OBJECT_COMPLEX)
     return fail(PTY_E_UNSUPPORTED, "potential objects run through pty_gd_run (gradient descent)");
   if (q.num_iter < 0 || q.max_batch != p.N)
@@ -2203,6 +2225,8 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
   if (!params) return fail(PTY_E_ARG, "params is NULL");
   const pty_problem& p = *problem;
   const pty_gd_params& q = *params;
+  if (!frame_supported(p.S))
+    return fail(PTY_E_UNSUPPORTED, "projection sets and probe modes take frames of 32, 64 or 128 px, not %d", p.S);
   if (p.object_type != PTY_OBJECT_COMPLEX)
     return fail(PTY_E_UNSUPPORTED, "potential objects run through pty_gd_run (gradient descent)");
   if (K < 1 || K > 64) return fail(PTY_E_ARG, "num_probes=%d out of range 1..64", K);
@@ -2329,26 +2353,27 @@ int pty_version(void) { return 10000; }
 const char* pty_last_error(void) { return g_err.c_str(); }
 
 int pty_fft2_c64(const void* in, void* out, int batch, int S, int inverse, pty_stream_t stream) {
-  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!frame_supported_gd(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (batch < 0 || !in || !out) return fail(PTY_E_ARG, "bad fft2 arguments");
   if (in == out) return fail(PTY_E_ARG, "fft2 must not run in place");
   return launch_fft((const cf*)in, (cf*)out, batch, S, inverse != 0, 1, st(stream));
 }
 
 int pty_permute_frames(const float* in, float* out, int count, int S, int direction, pty_stream_t stream) {
-  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!frame_supported_gd(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (count < 0 || !in || !out || in == out) return fail(PTY_E_ARG, "bad permute arguments");
   const long long total = (long long)count * S * S;
   if (total == 0) return PTY_OK;
   const int to_kernel = direction == 0;
-  if (S == 32) hipLaunchKernelGGL(k_permute<32>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
+  if (S == 256) hipLaunchKernelGGL(k_permute<256>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
+  else if (S == 32) hipLaunchKernelGGL(k_permute<32>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
   else if (S == 64) hipLaunchKernelGGL(k_permute<64>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
   else hipLaunchKernelGGL(k_permute<128>, dim3(grid_for(total, 256, 65536)), dim3(256), 0, st(stream), in, out, total, to_kernel);
   return check_launch("k_permute");
 }
 
 int pty_probe_spectrum(const void* probe, void* probe_hat, int S, pty_stream_t stream) {
-  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!frame_supported_gd(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (!probe || !probe_hat || probe == probe_hat) return fail(PTY_E_ARG, "bad probe spectrum arguments");
   return launch_fft((const cf*)probe, (cf*)probe_hat, 1, S, false, 0, st(stream));
 }
@@ -2356,7 +2381,7 @@ int pty_probe_spectrum(const void* probe, void* probe_hat, int S, pty_stream_t s
 int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px, int Py, const float* positions,
                               int N, const float* amplitudes, const int* order, int B, int S, void* ps, void* chi,
                               double* err, void* far, pty_stream_t stream) {
-  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!frame_supported_gd(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (Px < S || Py < S) return fail(PTY_E_ARG, "object %dx%d smaller than frame %d", Px, Py, S);
   if (N <= 0 || B < 0) return fail(PTY_E_ARG, "bad N=%d or B=%d", N, B);
   if (!probe_hat || !object || !positions || !amplitudes || !order || !ps || !chi || !err)
@@ -2375,7 +2400,7 @@ int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px,
 int pty_scatter_patches(const void* ps, const void* chi, const float* positions, int N, const int* order, int B,
                         int S, int Px, int Py, void* grad, float* spsum, float* tilemax, pty_stream_t stream) {
   // 32 x 32 windows are the learned initialiser's stitching patches
-  if (S != 32 && !frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!frame_supported_gd(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (Px < S || Py < S || N <= 0 || B < 0) return fail(PTY_E_ARG, "bad scatter sizes");
   if (!ps || !chi || !positions || !order || !grad || !spsum || !tilemax) return fail(PTY_E_ARG, "null buffer in scatter");
   GatherArgs a{};
@@ -2400,7 +2425,7 @@ int pty_scatter_num_tiles(int Px, int Py) { return gather_num_tiles(Px, Py); }
 int pty_probe_update(void* probe, const void* chi, const void* object, int Px, int Py, const float* positions, int N,
                      const int* order, int B, int S, float step_size, float normalization_min, void* partials,
                      pty_stream_t stream) {
-  if (!frame_supported(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
+  if (!frame_supported_gd(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (!probe || !chi || !object || !positions || !order || !partials || B <= 0 || N <= 0)
     return fail(PTY_E_ARG, "bad probe update arguments");
   return launch_probe_update((cf*)probe, (const cf*)chi, (const cf*)object, Px, Py, positions, N, order, B, S,

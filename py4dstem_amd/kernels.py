@@ -13,7 +13,10 @@ import torch
 from . import _lib
 from ._lib import PtyGdParams, PtyProblem, c_void_p, ptr, stream_handle
 
-SUPPORTED_FRAMES = (32, 64, 128)
+SUPPORTED_FRAMES = (32, 64, 128, 256)
+# frames transformed inside one work group with the frame in LDS; 256 px
+# frames run gradient descent through line passes in global memory
+LDS_FRAMES = (32, 64, 128)
 
 
 def _req(t, dtype, shape=None, name="tensor"):
@@ -28,13 +31,17 @@ def _req(t, dtype, shape=None, name="tensor"):
     return t
 
 
-# radix split N1 x N2 of each frame size (N1 along a row pass, N2 lanes per line)
-FRAME_RADIX = {32: (8, 4), 64: (8, 8), 128: (16, 8)}
+# radix split N1 x N2 of each frame size (N1 along a row pass, N2 lanes per
+# line); (S, 1) makes the 256 px kernel layout the natural order
+FRAME_RADIX = {32: (8, 4), 64: (8, 8), 128: (16, 8), 256: (256, 1)}
 
 
-def check_frame(S):
+def check_frame(S, lds_only=False):
     if S not in SUPPORTED_FRAMES:
         raise NotImplementedError(f"frame size {S} not supported (compiled for {SUPPORTED_FRAMES})")
+    if lds_only and S not in LDS_FRAMES:
+        raise NotImplementedError(f"frame size {S}: this operation takes frames of {LDS_FRAMES} px "
+                                  "(256 px frames run gradient descent only)")
 
 
 def kernel_index(S):
@@ -264,7 +271,7 @@ def projection_batch(probe_hat, obj, positions, amplitudes_k, order, proj_params
     (exit waves of proj_params advanced in place); returns (P_shift, err)."""
     import ctypes
     S = probe_hat.shape[-1]
-    check_frame(S)
+    check_frame(S, lds_only=True)
     B = order.numel()
     Px, Py = obj.shape
     ps = torch.empty((B, S, S), dtype=torch.complex64, device=obj.device)
@@ -303,7 +310,7 @@ def mixed_forward_adjoint(probe_hats, obj, positions, amplitudes_k, order):
     spectra [K, S, S] (kernel layout); returns (P_shift [K, B, S, S],
     chi [K, B, S, S], err [B], far [B, K, S, S] staging layout)."""
     Kp, S = probe_hats.shape[0], probe_hats.shape[-1]
-    check_frame(S)
+    check_frame(S, lds_only=True)
     _req(probe_hats, torch.complex64, (Kp, S, S), "probe_hats")
     B = order.numel()
     Px, Py = obj.shape
@@ -320,7 +327,7 @@ def mixed_forward_adjoint(probe_hats, obj, positions, amplitudes_k, order):
 def probe_com_modes(probe):
     """Shift every mode of probe [K, S, S] (in place) by mode 0's centre of mass."""
     Kp, S = probe.shape[0], probe.shape[-1]
-    check_frame(S)
+    check_frame(S, lds_only=True)
     shift = torch.empty(2, dtype=torch.float32, device=probe.device)
     _lib.call("pty_probe_com_modes", ptr(probe), Kp, S, ptr(shift), stream_handle())
 
@@ -330,6 +337,8 @@ def far_to_kernel_layout(far):
     kernel layout (element j of row-pass thread t -> row t // (S/N1),
     column N1 * (t % (S/N1)) + j)."""
     S = far.shape[-1]
+    if S == 256:  # staged in natural order
+        return far.contiguous()
     N1 = FRAME_RADIX[S][0]
     x = far.reshape(far.shape[0], N1, S, S // N1)  # [b][j][row][q], t = row * (S/N1) + q
     return x.permute(0, 2, 3, 1).reshape(far.shape[0], S, S).contiguous()

@@ -138,6 +138,9 @@ class MixedstatePtychography(SingleslicePtychography):
         if self._object_type == "potential":
             raise NotImplementedError("potential objects with several probe modes are not "
                                       "supported on this path")
+        if int(self._region_of_interest_shape[0]) not in K.LDS_FRAMES:
+            raise NotImplementedError("several probe modes take frames of 32, 64 or 128 px on "
+                                      "this path")
         run = self._prepare_run(
             num_iter=num_iter, reconstruction_method=reconstruction_method,
             max_batch_size=max_batch_size, seed_random=seed_random, step_size=step_size,

@@ -652,9 +652,6 @@ class SingleslicePtychography:
                     "Stochastic object/probe updating is inconsistent with 'DM_AP', 'RAAR', "
                     "'RRR', and 'SUPERFLIP'. Use reconstruction_method='GD' or set "
                     "max_batch_size=None.")
-            if not fix_positions and global_affine_transformation:
-                raise NotImplementedError("the global affine position constraint with "
-                                          "projection-set methods is not supported on this path")
             if detector_fourier_mask is not None:  # ptychographic_methods.py:1686-1687
                 raise NotImplementedError()
         tv = tv_denoise and tv_denoise_weight is not None
@@ -678,6 +675,17 @@ class SingleslicePtychography:
                           UserWarning)
 
         N = self._num_diffraction_patterns
+        S = int(self._region_of_interest_shape[0])
+        if S not in K.LDS_FRAMES:
+            # 256 px frames: gradient descent through line passes (no LDS frame)
+            for what, used in (("projection-set methods", projection is not None),
+                               ("position correction", not fix_positions),
+                               ("fix_probe_aperture", fix_probe_aperture and not fix_probe),
+                               ("constrain_probe_amplitude",
+                                constrain_probe_amplitude and not fix_probe)):
+                if used:
+                    raise NotImplementedError(f"{what} with {S} px frames is not supported on "
+                                              "this path (gradient descent only)")
         if self._verbose:
             first = (f"Performing {num_iter} iterations using a {self._object_type} object type, "
                      f"with the {reconstruction_method} algorithm, with normalization_min: "
@@ -850,7 +858,9 @@ class SingleslicePtychography:
         while it < num_iter:
             n = min(per_call, num_iter - it)
             params.num_iter = n
-            if run["affine"]:
+            if run["affine"] and run["projection_params"] is None:
+                # gradient descent shuffles before every iteration; the
+                # projection sets keep the scan order
                 np.random.shuffle(run["shuffled"])
                 orders_dev[it].copy_(torch.from_numpy(run["shuffled"].astype(np.int32)))
             if run["projection_params"] is not None:

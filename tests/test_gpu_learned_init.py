@@ -232,3 +232,24 @@ def test_judge_keeps_flat_start_when_learned_is_worse(dev):
     assert not np.array_equal(q._object_initial, flat)
     q.reconstruct(num_iter=1, reset=True, seed_random=0, **w["reconstruct"])
     assert np.isfinite(q.error)
+
+
+@pytest.mark.parametrize("seed", [2001, 2003])
+def test_learned_start_ablation_at_production_shapes(dev, seed):
+    """Ablation on held-out objects of the initialiser's distribution at the
+    production shapes (128x128 raster at 0.4 A, 128 px frames, 720x720
+    object, 1e6 e/A^2, mini-batch 1024, probe update): the judged learned
+    start has a lower object NMSE than the flat start, and reaches NMSE <=
+    1e-3 in no more iterations than the flat start (both within 6)."""
+    from py4dstem_amd.learned_init import ConvNet, evaluation_workload, time_to_nmse
+    say(f"learned vs flat start, held-out seed {seed}")
+    w = evaluation_workload(seed)
+    r = time_to_nmse(w, ConvNet.load(), max_iter=6)
+    fl, le = r["flat"], r["learned"]
+    say(f"flat: nmse {[f'{v:.2e}' for v in fl['nmse'][:3]]} iterations {fl['iterations']} "
+        f"seconds {fl['seconds']}; learned: nmse {[f'{v:.2e}' for v in le['nmse'][:3]]} "
+        f"iterations {le['iterations']} seconds {le['seconds']} used {le['init_info']['used_learned']}")
+    assert le["init_info"]["used_learned"]
+    assert le["nmse"][0] < fl["nmse"][0]
+    assert le["iterations"] is not None and fl["iterations"] is not None
+    assert le["iterations"] <= fl["iterations"]

@@ -149,3 +149,52 @@ def test_gpu_simulator_matches_numpy(dev, S):
     got = simulate_intensities_gpu(obj, probe, pos, chunk=16).cpu().numpy()
     ref = simulate_intensities(obj.astype(np.complex128), probe.astype(np.complex64).astype(np.complex128), pos)
     assert rel(got, ref) < 1e-5
+
+
+def test_sixty_four_problem_mix_batched_equals_sequential_and_oracle(dev):
+    """The full batched workload: 64 seeded problems (square scans of 32 to
+    96 positions per side, half 64 px and half 128 px frames, defocus 100 to
+    400 A, full-batch updates), 3 iterations.  Batched (one graph replay per
+    iteration) equals the 64 sequential reconstruct() calls bit for bit, and
+    the smallest 64 px and the smallest 128 px problem match the oracle:
+    probe and every error <= 1e-4, object <= max(1e-4, 3 x the oracle's own
+    complex64/complex128 gap) as for every flat start."""
+    from oracle import OracleSingleslicePtychography
+    from py4dstem_amd.synthetic import many_workloads
+    say("simulate the 64-problem mix")
+    ws = many_workloads(count=64, seed=0, num_iter=3, on_gpu=True)
+    shapes = [w["datacube"].data.shape for w in ws]
+    assert sorted({s[-1] for s in shapes}) == [64, 128]
+    assert min(s[0] for s in shapes) >= 32 and max(s[0] for s in shapes) <= 96
+    say("64 sequential reconstructions")
+    seq = _sequential(ws)
+    say("64 batched reconstructions (graph)")
+    bat = _batched(ws, True, num_iter=3)
+    _assert_same(bat, seq)
+    picks = [min((i for i, s in enumerate(shapes) if s[-1] == S), key=lambda i: shapes[i][0])
+             for S in (64, 128)]
+    for i in picks:
+        w = ws[i]
+        say(f"problem {i}: scan {shapes[i][:2]}, {shapes[i][-1]} px, vs oracle")
+        runs = {}
+        for mode in ("c64", "c128"):
+            o = OracleSingleslicePtychography(datacube=w["datacube"], mode=mode, **w["init"])
+            o.preprocess(**w["preprocess"])
+            # the full-batch shuffles come from the global generator, in the
+            # order the sequential pass drew them
+            np.random.seed(1234)
+            for w_prev in ws[:i]:
+                for _ in range(3):
+                    np.random.shuffle(np.arange(w_prev["datacube"].data.shape[0]
+                                                * w_prev["datacube"].data.shape[1]))
+            o.reconstruct(**{k: v for k, v in w["reconstruct"].items() if k != "progress_bar"})
+            runs[mode] = o
+        o, o128 = runs["c64"], runs["c128"]
+        obj, probe, errs = seq[i]
+        gap = rel(o.object, o128.object)
+        err = float(np.max(np.abs(np.array(errs) / np.array(o.error_iterations) - 1)))
+        say(f"  object {rel(obj, o.object):.2e} (gap {gap:.2e}) probe {rel(probe, o.probe):.2e} "
+            f"error {err:.2e}")
+        assert rel(obj, o.object) <= max(1e-4, 3 * gap)
+        assert rel(probe, o.probe) <= 1e-4
+        assert err <= 1e-4
