@@ -89,11 +89,17 @@ def learned_init_summary(seeds, max_iter):
         v = [per[s][kind]["seconds"] for s in seeds]
         return None if any(x is None for x in v) else sum(v)
     t_flat, t_learned = total("flat"), total("learned")
+    # seeds whose flat start is not yet at the target (there the initialiser
+    # can save iterations; elsewhere it can only add its own time)
+    miss = [s for s in seeds if per[s]["flat"]["iterations"] and per[s]["learned"]["seconds"]]
+    ratios = sorted(per[s]["flat"]["seconds"] / per[s]["learned"]["seconds"] for s in miss)
     return {
         "metric": "time to object NMSE<=1e-3, flat start / learned start",
         "value": round(t_flat / t_learned, 3) if (t_flat and t_learned) else None,
         "unit": "x", "target": 2.0,
         "flat_seconds": t_flat, "learned_seconds": t_learned,
+        "where_flat_start_misses_target": {"seeds": miss,
+                                           "ratios": [round(r, 3) for r in ratios]},
         "workload": "held-out Gaussian-atom objects of the initialiser's distribution, 128x128 "
                     "scan at 0.4 A, 128 px frames, 720x720 object, 1e6 e/A^2, mini-batch 1024, "
                     "probe update; seconds = initialiser + iterations to target x per-iteration "
@@ -170,10 +176,11 @@ def bench_many(args):
     frames = sorted({int(p._region_of_interest_shape[0]) for p in items})
     m = MultiPtychography(items)
     t = time.perf_counter()
-    batch, stage, timings = m.tune(use_cache=False)
-    log(f"autotuner: mini-batch {batch}, stage {stage} ({time.perf_counter() - t:.1f} s) {timings}")
-    kws = [dict(seed_random=w["reconstruct"]["seed_random"], max_batch_size=batch, fix_probe=False,
-                step_size=0.5, normalization_min=1, reset=True) for w in ws]
+    batches, stage, timings = m.tune(use_cache=False)
+    log(f"autotuner: mini-batches {sorted(set(batches))}, stage {stage} "
+        f"({time.perf_counter() - t:.1f} s) {timings}")
+    kws = [dict(seed_random=w["reconstruct"]["seed_random"], max_batch_size=b, fix_probe=False,
+                step_size=0.5, normalization_min=1, reset=True) for w, b in zip(ws, batches)]
 
     def sequential():
         np.random.seed(7)
@@ -219,7 +226,8 @@ def bench_many(args):
             "workload": f"{len(items)} independent GD ptychography reconstructions, square scans "
                         "32..96 per side, frames " + "/".join(map(str, frames)) +
                         f" px, {num_iter} iterations, batched in one graph replay per iteration",
-            "problems": len(items), "total_positions": npos, "mini_batch": batch, "stage": stage,
+            "problems": len(items), "total_positions": npos,
+            "mini_batch": {"min": min(batches), "max": max(batches)}, "stage": stage,
             "autotune_s_per_candidate": timings,
         },
         "sequential": {"value": round(rate_seq, 1), "seconds": round(t_seq, 3)},
@@ -475,6 +483,11 @@ def bench_projection(args):
     N = p._num_diffraction_patterns
     recon = dict(w["reconstruct"], progress_bar=False, max_batch_size=None,
                  reconstruction_method="DM_AP", reconstruction_parameter=0.5)
+    if args.correct_positions:
+        # position correction after every projection-set update; the
+        # corrected positions need the forward overlap's far field, one more
+        # whole-batch fused pass per iteration
+        recon.update(fix_positions=False, positions_step_size=0.1)
     if args.warmup:
         p.reconstruct(**dict(recon, num_iter=args.warmup))
     torch.cuda.synchronize()
@@ -524,7 +537,8 @@ def bench_projection(args):
         "config": {
             "workload": f"single-slice ptychography, DM_AP(0.5) projection sets, "
                         f"{args.scan}x{args.scan} scan, {S}x{S} px patterns, {Px}x{Py} object, "
-                        f"all positions per update, probe update",
+                        f"all positions per update, probe update"
+                        + (", position correction" if args.correct_positions else ""),
             "scan_positions": N, "frame_px": S, "object_px": [Px, Py],
         },
         "roofline": {
@@ -598,7 +612,9 @@ def main():
                     default="production")
     ap.add_argument("--probe-modes", type=int, default=4)
     ap.add_argument("--problems", type=int, default=64)
-    ap.add_argument("--learned-seeds", type=int, default=2,
+    ap.add_argument("--correct-positions", action="store_true",
+                    help="--workload projection: with position correction (fix_positions=False)")
+    ap.add_argument("--learned-seeds", type=int, default=4,
                     help="held-out seeds for the time-to-NMSE sub-object of the default line "
                          "(0 skips it)")
     args = ap.parse_args()

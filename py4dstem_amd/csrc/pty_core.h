@@ -420,8 +420,12 @@ template <int S> struct Fused {
     if (tid < 2 * S) {
       const int k = tid < S ? tid : tid - S;
       const float frac = tid < S ? px - rx : py - ry;
-      // (-2 pi kf) * (-frac) is the exact negation: conj of the forward ramp
-      lds.st(RAMP + tid, ramp1(k, conj_ramp ? -frac : frac));
+      // (-2 pi kf) * (-frac) is the exact negation: conj of the forward ramp.
+      // The shifted probe's 1/S^2 (ifft2 normalisation) rides on the row
+      // factors: a power of two, so scaling before the transform is exact
+      // and the result is bit for bit the scaled-after one.
+      const cf e = ramp1(k, conj_ramp ? -frac : frac);
+      lds.st(RAMP + tid, (tid < S && !conj_ramp) ? cscale(e, 1.0f / (float)(S * S)) : e);
     }
     if (tid < S) {  // periodic row of window row `tid`, times the row length
       int row = r.x0 + F::fidx(tid);
@@ -471,15 +475,20 @@ template <int S> struct Fused {
 #pragma unroll
     for (int j = 0; j < N1; ++j) lds.st(F::at(N1 * t + j, cc), r.v[j]);
   }
-  // column pass A' leaving the natural-order result in registers
-  template <class L> static PTY_HD void p_colAinv_to_regs(int tid, Regs& r, L lds, const cf* tw) {
+  // column pass A' leaving the natural-order result in registers, times
+  // 1/S^2 unless the input already carries it (SCALE = false: the shifted
+  // probe, whose row ramp factors hold the 1/S^2)
+  template <class L, bool SCALE = true>
+  static PTY_HD void p_colAinv_to_regs(int tid, Regs& r, L lds, const cf* tw) {
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
 #pragma unroll
     for (int m = 0; m < N1; ++m) r.v[m] = lds.ld(F::at(n2 + N2 * m, cc));
     P::template A<true>(n2, r.v, tw);
-    const float s = 1.0f / (float)(S * S);
+    if (SCALE) {
+      const float s = 1.0f / (float)(S * S);
 #pragma unroll
-    for (int m = 0; m < N1; ++m) r.v[m] = cscale(r.v[m], s);
+      for (int m = 0; m < N1; ++m) r.v[m] = cscale(r.v[m], s);
+    }
   }
 
   // periodic object index of window element (i, j); x0/y0 already wrapped
@@ -500,7 +509,7 @@ template <int S> struct Fused {
   // POT: the object is a potential V and the patch exp(i V)
   template <class L, bool POT = false>
   static PTY_HD void p_exitwave_colA(int tid, const FusedArgs& a, int b, Regs& r, L lds, const cf* tw) {
-    p_colAinv_to_regs(tid, r, lds, tw);
+    p_colAinv_to_regs<L, false>(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
     if (a.ps_out) {
       cf* ps_b = a.ps_out + (size_t)b * S * S;
@@ -577,7 +586,7 @@ template <int S> struct Fused {
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #endif
-    p_colAinv_to_regs(tid, r, lds, tw);
+    p_colAinv_to_regs<L, false>(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
     cf* ps_b = a.ps_out ? a.ps_out + (size_t)b * S * S : nullptr;
     const cf* ex_b = a.chi_in + (size_t)b * S * S;

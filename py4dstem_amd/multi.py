@@ -102,30 +102,41 @@ class MultiPtychography:
         return json.dumps(sorted((int(p._region_of_interest_shape[0]), int(p._num_diffraction_patterns),
                                   tuple(int(v) for v in p._object_shape)) for p in self.items))
 
-    def tune(self, batch_candidates=(256, 512, 1024), stage_candidates=(0, 256), iters=2,
-             reconstruct_kwargs=None, use_cache=True):
-        """Pick the mini-batch size (when the caller leaves it free) and the
-        stage size by timing `iters` iterations of every candidate on copies
-        of the problems' current state.  Returns (batch, stage, timings)."""
-        key = "multi:" + self.signature()
+    def tune(self, batch_candidates=(256, 512, 1024), split_candidates=(2, 4, 8),
+             stage_candidates=(0, 256), iters=2, reconstruct_kwargs=None, use_cache=True):
+        """Pick every problem's mini-batch size (when the caller leaves it
+        free) and positions per fused launch by timing `iters` iterations of
+        each candidate on copies of the problems' current state, all problems
+        together as reconstruct() runs them.  Mini-batch candidates are either
+        one size for every problem (batch_candidates) or a split of every
+        problem into the same number of mini-batches (split_candidates: a
+        problem of N positions gets ceil(N / k)), which keeps the problems'
+        launch sequences the same length.  Returns (per-problem batch sizes,
+        stage, timings); the choice is cached in a JSON file keyed by the
+        problems' signature."""
+        key = "multi2:" + self.signature()
         if use_cache:
             hit = autotune.lookup(key)
             if hit:
                 self._apply(hit["batch"], hit["stage"])
-                return hit["batch"], hit["stage"], hit.get("timings", {})
+                return list(hit["batch"]), hit["stage"], hit.get("timings", {})
         kw = dict(reconstruct_kwargs or {})
         saved = [(p._object_dev.clone() if hasattr(p, "_object_dev") else None,
                   p._probe_dev.clone() if hasattr(p, "_probe_dev") else None,
                   list(getattr(p, "error_iterations", [])), hasattr(p, "error")) for p in self.items]
+        sizes = [int(p._num_diffraction_patterns) for p in self.items]
+        batches = {f"b{b}": [min(b, n) for n in sizes] for b in batch_candidates}
+        batches.update({f"k{k}": [-(-n // k) for n in sizes] for k in split_candidates})
 
         def trial(cand):
-            batch, stage = cand
-            self._apply(batch, stage)
-            runs = self._runs(iters, [dict(kw, max_batch_size=batch, reset=True)] * len(self.items))
+            name, stage = cand
+            self._apply(batches[name], stage)
+            runs = self._runs(iters, [dict(kw, max_batch_size=b, reset=True) for b in self.batch])
             self.run_prepared(runs, iters)
             torch.cuda.synchronize()
 
-        cands = [(b, s) for b in batch_candidates for s in stage_candidates if s == 0 or s < b]
+        cands = [(name, s) for name in batches for s in stage_candidates
+                 if s == 0 or s < max(batches[name])]
         best, timings = autotune.best_of(cands, trial, repeats=1)
         for p, (o, pr, errs, had) in zip(self.items, saved):
             for attr, val in (("_object_dev", o), ("_probe_dev", pr)):
@@ -136,12 +147,14 @@ class MultiPtychography:
             p.error_iterations = errs
             if not had and hasattr(p, "error"):
                 del p.error
-        self._apply(*best)
-        tim = {f"{b}/{s}": t for (b, s), t in timings.items()}
-        autotune.remember(key, {"batch": best[0], "stage": best[1], "timings": tim})
-        return best[0], best[1], tim
+        self._apply(batches[best[0]], best[1])
+        tim = {f"{n}/{s}": t for (n, s), t in timings.items()}
+        autotune.remember(key, {"batch": self.batch, "stage": best[1], "choice": best[0],
+                                "timings": tim})
+        return list(self.batch), best[1], tim
 
     def _apply(self, batch, stage):
-        self.batch = batch
+        """batch: one size for every problem or a list, one per problem."""
+        self.batch = list(batch) if isinstance(batch, (list, tuple)) else [batch] * len(self.items)
         for p in self.items:
             p.set_stage(stage)

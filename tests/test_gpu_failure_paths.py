@@ -3,8 +3,13 @@ through the bound entry points.
 
 * All-zero patterns: A = 0 gives chi = -psi and err = sum |F|^2
   (phase/ptychographic_methods.py:1576-1635).
-* Zero far field: an object that is zero over whole windows gives F = 0
-  there, where NumPy's angle(0) = 0 makes chi = ifft2(A).
+* Zero far field: an object that is zero over a whole window gives F = 0
+  there.  The GPU takes angle(0) = 0 there (chi = ifft2(A)).  NumPy's
+  exp(1j angle(F)) at an exact zero follows the sign bits pocketfft leaves on
+  it (angle pi at -0 + 0j), which depend on the signs of the zeros it was
+  given, so the reference's value there is an artefact of its FFT's
+  operation order that no other FFT reproduces; the oracle comparison below
+  therefore keeps every far field nonzero.
 * NaN counts: NumPy carries NaN through sqrt(max(I, 0)), the mean intensity,
   the error and every object pixel the NaN pattern's window touches; the GPU
   path must give the same NaN pattern and the same finite values.
@@ -58,16 +63,17 @@ def _pair(w, init_obj, recon, probe=None, pre=None):
     return out["gpu"], out["oracle"]
 
 
-def test_zero_patterns_and_zero_far_field_match_oracle(dev):
-    """Two all-zero patterns and an object that is zero over a block wider
-    than a window: 3 iterations with mini-batches and probe update, object,
-    probe and every error <= 1e-4 against the oracle."""
+def test_zero_patterns_match_oracle(dev):
+    """Two all-zero patterns (A = 0: chi = -psi, err = sum |F|^2) and an
+    object that is zero over part of the field: 3 iterations with
+    mini-batches and probe update, object, probe and every error <= 1e-4
+    against the oracle."""
     from py4dstem_amd.synthetic import small_workload
     w = small_workload(seed=2, scan=(10, 10), object_px=160)
     w["datacube"].data[3, 4] = 0.0
     w["datacube"].data[7, 1] = 0.0
     init = _start((160, 160))
-    init[10:90, 10:90] = 0.0          # windows of the first scan rows see F = 0
+    init[20:60, 20:60] = 0.0          # zero over part of some windows, never a whole one
     recon = dict(w["reconstruct"], num_iter=3, max_batch_size=25, fix_probe=False)
     g, o = _pair(w, init, recon)
     err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
@@ -79,8 +85,11 @@ def test_zero_patterns_and_zero_far_field_match_oracle(dev):
 
 
 def test_zero_pattern_and_zero_patch_single_batch_terms(dev):
-    """The fused kernel's terms for one batch against the oracle's forward:
-    a zero pattern gives chi = -psi; a zero patch gives chi = ifft2(A)."""
+    """The fused kernel's terms for one batch against the oracle's forward: a
+    zero pattern gives chi = -psi.  A position whose whole window lies in a
+    zero region of the object has F = 0 exactly; the GPU takes angle 0 there
+    (chi = ifft2(A)), and the oracle's NumPy value is ifft2(A u) with u = +1
+    or -1 by the sign bit pocketfft leaves on each zero."""
     from py4dstem_amd import kernels as K
     from py4dstem_amd.synthetic import small_workload
     w = small_workload(seed=5, scan=(4, 4), object_px=128)
@@ -88,23 +97,29 @@ def test_zero_pattern_and_zero_patch_single_batch_terms(dev):
     o.preprocess(**w["preprocess"])
     amps = o._amplitudes.copy()
     amps[2] = 0.0
+    pos = o._positions_px
+    phat = K.probe_spectrum(torch.from_numpy(o._probe).to(dev))
+    ampk = K.to_kernel_layout(torch.from_numpy(amps).to(dev))
+    posd = torch.from_numpy(pos.copy()).to(dev)
+    # (a) zero pattern, ordinary object: every slot against the oracle
     obj = _start((128, 128))
-    x0, y0 = np.round(o._positions_px[7]).astype(int)
-    obj[x0 - 40:x0 + 40, y0 - 40:y0 + 40] = 0.0
-    sp, op, chi, _ = o.forward(obj, o._probe, o._positions_px, amps)
-    assert np.all(op[7] == 0)
+    sp, op, chi, _ = o.forward(obj, o._probe, pos, amps)
     np.testing.assert_allclose(chi[2], -(sp[2] * op[2]), atol=1e-5 * np.abs(sp[2]).max())
-    np.testing.assert_allclose(chi[7], np.fft.ifft2(amps[7]), atol=1e-6 * np.abs(amps[7]).max())
-    out = K.forward_adjoint(
-        K.probe_spectrum(torch.from_numpy(o._probe).to(dev)), torch.from_numpy(obj).to(dev),
-        torch.from_numpy(o._positions_px).to(dev),
-        K.to_kernel_layout(torch.from_numpy(amps).to(dev)),
-        torch.arange(16, dtype=torch.int32, device=dev))
-    gchi = out[1].cpu().numpy()
-    say(f"chi {rel(gchi, chi):.2e}")
+    gchi = K.forward_adjoint(phat, torch.from_numpy(obj).to(dev), posd, ampk,
+                             torch.arange(16, dtype=torch.int32, device=dev))[1].cpu().numpy()
+    say(f"zero pattern: chi {rel(gchi, chi):.2e}, slot 2 {rel(gchi[2], chi[2]):.2e}")
     assert rel(gchi[2], chi[2]) <= 1e-5
-    assert rel(gchi[7], chi[7]) <= 1e-5
     assert rel(gchi, chi) <= 2e-5
+    # (b) slot 7 alone over an object that is zero around it: F = 0 exactly
+    x0, y0 = np.round(pos[7]).astype(int)
+    obj[x0 - 40:x0 + 40, y0 - 40:y0 + 40] = 0.0
+    sp, op, chi, _ = o.forward(obj, o._probe, pos[7:8], amps[7:8])
+    assert np.all(op[0] == 0)
+    u = np.exp(1j * np.angle(np.fft.fft2((sp[0] * op[0]).astype(np.complex64))))
+    np.testing.assert_allclose(chi[0], np.fft.ifft2(amps[7] * u), atol=1e-6 * np.abs(amps[7]).max())
+    g7 = K.forward_adjoint(phat, torch.from_numpy(obj).to(dev), posd, ampk,
+                           torch.tensor([7], dtype=torch.int32, device=dev))[1].cpu().numpy()
+    assert rel(g7[0], np.fft.ifft2(amps[7].astype(np.float64))) <= 1e-6
 
 
 def test_nan_pattern_propagates_as_numpy(dev):

@@ -157,8 +157,8 @@ def test_sixty_four_problem_mix_batched_equals_sequential_and_oracle(dev):
     400 A, full-batch updates), 3 iterations.  Batched (one graph replay per
     iteration) equals the 64 sequential reconstruct() calls bit for bit, and
     the smallest 64 px and the smallest 128 px problem match the oracle:
-    probe and every error <= 1e-4, object <= max(1e-4, 3 x the oracle's own
-    complex64/complex128 gap) as for every flat start."""
+    object, probe and every error <= max(1e-4, 3 x the oracle's own
+    complex64/complex128 gap), as for every flat start."""
     from oracle import OracleSingleslicePtychography
     from py4dstem_amd.synthetic import many_workloads
     say("simulate the 64-problem mix")
@@ -191,10 +191,43 @@ def test_sixty_four_problem_mix_batched_equals_sequential_and_oracle(dev):
             runs[mode] = o
         o, o128 = runs["c64"], runs["c128"]
         obj, probe, errs = seq[i]
-        gap = rel(o.object, o128.object)
+        gaps = (rel(o.object, o128.object), rel(o.probe, o128.probe),
+                float(np.max(np.abs(np.array(o.error_iterations)
+                                    / np.array(o128.error_iterations) - 1))))
         err = float(np.max(np.abs(np.array(errs) / np.array(o.error_iterations) - 1)))
-        say(f"  object {rel(obj, o.object):.2e} (gap {gap:.2e}) probe {rel(probe, o.probe):.2e} "
-            f"error {err:.2e}")
-        assert rel(obj, o.object) <= max(1e-4, 3 * gap)
-        assert rel(probe, o.probe) <= 1e-4
-        assert err <= 1e-4
+        say(f"  object {rel(obj, o.object):.2e} probe {rel(probe, o.probe):.2e} error {err:.2e} "
+            f"(oracle gaps {gaps[0]:.2e} {gaps[1]:.2e} {gaps[2]:.2e})")
+        assert rel(obj, o.object) <= max(1e-4, 3 * gaps[0])
+        assert rel(probe, o.probe) <= max(1e-4, 3 * gaps[1])
+        assert err <= max(1e-4, 3 * gaps[2])
+
+
+def test_autotuner_picks_per_problem_batches(workloads, dev, tmp_path, monkeypatch):
+    """The autotuner times one-size and equal-split mini-batch candidates on
+    copies of the state and returns one mini-batch size per problem; the
+    problems' state is restored, the choice is cached, and a batched run with
+    the chosen sizes equals the sequential runs with the same sizes."""
+    from py4dstem_amd import autotune
+    from py4dstem_amd.multi import MultiPtychography
+    monkeypatch.setattr(autotune, "CACHE_PATH", str(tmp_path / "tune.json"))
+    items = [_make(w) for w in workloads]
+    m = MultiPtychography(items)
+    batches, stage, timings = m.tune(iters=1, use_cache=True)
+    sizes = [p._num_diffraction_patterns for p in items]
+    say(f"tuned mini-batches {batches} stage {stage}; candidates {len(timings)}")
+    assert len(batches) == len(items) and all(1 <= b <= n for b, n in zip(batches, sizes))
+    assert any(k.startswith("k") for k in timings) and any(k.startswith("b") for k in timings)
+    assert all(not getattr(p, "error_iterations", []) for p in items)
+    assert m.tune(use_cache=True)[0] == batches          # served from the cache
+    kws = [dict({k: v for k, v in w["reconstruct"].items() if k not in ("num_iter", "progress_bar")},
+                max_batch_size=b) for w, b in zip(workloads, batches)]
+    np.random.seed(5)
+    ref = []
+    for w, kw in zip(workloads, kws):
+        p = _make(w)
+        p.set_stage(stage)
+        p.reconstruct(num_iter=2, progress_bar=False, **kw)
+        ref.append((p.object.copy(), p.probe.copy(), list(p.error_iterations)))
+    np.random.seed(5)
+    m.reconstruct(2, kws)
+    _assert_same([(p.object, p.probe, p.error_iterations) for p in items], ref)

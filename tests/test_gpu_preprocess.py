@@ -240,13 +240,17 @@ def test_centred_positions_and_default_object_match_oracle(dev):
     g.reconstruct(progress_bar=False, **recon)
     o.reconstruct(**recon)
     err = float(np.max(np.abs(np.array(g.error_iterations) / np.array(o.error_iterations) - 1)))
-    say(f"object {rel(g.object, o.object):.2e} probe {rel(g.probe, o.probe):.2e} error {err:.2e}")
-    assert err <= 1e-4
-    assert rel(g.probe, o.probe) <= 1e-4
-    # flat start: the object is held to the oracle's own complex64/complex128
-    # gap as in the other flat-start tests
+    # flat start: exp(i angle F) of far-field pixels that are zero in the
+    # model amplifies rounding, so object, probe and errors are held to
+    # max(1e-4, 3 x the oracle's own complex64/complex128 gap), as in the
+    # other flat-start tests
     o128 = po.OracleSingleslicePtychography(datacube=w["datacube"], mode="c128", **init)
     o128.preprocess(**pre)
     o128.reconstruct(**recon)
-    gap = rel(o.object, o128.object)
-    assert rel(g.object, o.object) <= max(1e-4, 3 * gap)
+    gaps = (rel(o.object, o128.object), rel(o.probe, o128.probe),
+            float(np.max(np.abs(np.array(o.error_iterations) / np.array(o128.error_iterations) - 1))))
+    say(f"object {rel(g.object, o.object):.2e} probe {rel(g.probe, o.probe):.2e} error {err:.2e} "
+        f"(oracle gaps {gaps[0]:.2e} {gaps[1]:.2e} {gaps[2]:.2e})")
+    assert rel(g.object, o.object) <= max(1e-4, 3 * gaps[0])
+    assert rel(g.probe, o.probe) <= max(1e-4, 3 * gaps[1])
+    assert err <= max(1e-4, 3 * gaps[2])

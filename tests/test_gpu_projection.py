@@ -159,9 +159,9 @@ def test_projection_argument_errors(dev):
                       reconstruction_parameter_a=0.1)
     with pytest.raises(ValueError):
         p.reconstruct(num_iter=1, reconstruction_method="ePIE")
-    with pytest.raises(NotImplementedError):
-        p.reconstruct(num_iter=1, reconstruction_method="DM_AP", fix_positions=False,
-                      global_affine_transformation=True)
+    with pytest.raises(NotImplementedError):  # ptychographic_methods.py:1686-1687
+        p.reconstruct(num_iter=1, reconstruction_method="DM_AP",
+                      detector_fourier_mask=np.ones((64, 64), np.float32))
     # a gradient-descent run afterwards drops the projection state
     p.reconstruct(num_iter=1, reconstruction_method="SUPERFLIP", progress_bar=False, reset=True)
     assert p._exit_waves is not None
