@@ -11,10 +11,11 @@
  * Conventions
  *  - Complex arrays are complex64: interleaved float32 (re, im), 8-byte aligned.
  *  - All buffers are device memory owned by the caller; the library keeps no
- *    pointer after a call returns and allocates no device memory.
- *    pty_multi_run is the one entry that creates runtime objects (a stream,
- *    two events, a graph) and waits for its work before returning; every
- *    other entry is asynchronous and creates none.
+ *    pointer after a call returns and allocates no device memory (the one
+ *    static array of pty_gd_run_noalloc aside).  pty_multi_run is the one
+ *    entry that creates runtime objects (a stream, two events, a graph) and
+ *    waits for its work before returning; every other entry is
+ *    asynchronous and creates none.
  *  - Every call is asynchronous on `stream` (a hipStream_t, NULL = default
  *    stream) and returns 0 on success or a negative PTY_E* code; the message
  *    is available from pty_last_error() on the calling thread.
@@ -262,6 +263,26 @@ size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* p
  * per-iteration error (float64, device, num_iter entries). */
 int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const int* orders,
                double* errors, void* workspace, size_t workspace_bytes, pty_stream_t stream);
+
+/* Bytes of the static per-GPU workspace of pty_gd_run_noalloc. */
+#define PTY_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* pty_gd_run without a caller workspace and without any device allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a static device array
+ * of PTY_NOALLOC_SCRATCH_BYTES (64 MiB) on each GPU the library runs on, kept
+ * until the process ends; its address on the current device is looked up
+ * once, on the first call there.  Positions per fused launch
+ * (params->stage) are lowered in multiples of 64 until the workspace fits,
+ * which leaves the results bitwise equal to pty_gd_run's; a problem that
+ * does not fit even at 64 positions per launch (the object's running sums
+ * take 24 bytes per pixel) returns PTY_E_WORKSPACE.  Nothing in the array
+ * needs to start at zero.  Every call on one GPU uses the same array, so
+ * calls that overlap on one GPU (several streams or host threads) race; the
+ * caller must order them. */
+int pty_gd_run_noalloc(const pty_problem* problem, const pty_gd_params* params, const int* orders,
+                       double* errors, pty_stream_t stream);
 
 /* ---- position correction ---------------------------------------------------
  * reconstruct(fix_positions=False): after each mini-batch's object/probe

@@ -128,6 +128,8 @@ _SIGNATURES = {
     "pty_gd_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams)]),
     "pty_gd_run": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams), c_void_p,
                            c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pty_gd_run_noalloc": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams),
+                                   c_void_p, c_void_p, c_void_p]),
     "pty_gd_positions_workspace_bytes": (c_size_t, [ctypes.POINTER(PtyProblem),
                                                     ctypes.POINTER(PtyGdParams)]),
     "pty_gd_run_positions": (c_int, [ctypes.POINTER(PtyProblem), ctypes.POINTER(PtyGdParams),

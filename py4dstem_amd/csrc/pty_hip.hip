@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1919,6 +1920,64 @@ int validate_problem(const pty_problem* p) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Static workspace of the allocation-free gradient-descent entry point: one
+// copy per GPU (each device's code object holds its own), kept until the
+// process ends.  Host code must use the per-device address from
+// hipGetSymbolAddress, never the array's name.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PTY_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+// device address of the current GPU's scratch array, looked up once per device
+int scratch_pointer(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(PTY_E_LAUNCH, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+    return fail(PTY_E_ARG, "device %d beyond the %d the scratch table covers", dev, SCRATCH_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(PTY_E_LAUNCH, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PTY_OK;
+}
+
+// the largest positions-per-launch setting (the whole mini-batch, else a
+// multiple of 64 not above the caller's) whose workspace fits the scratch
+int fit_stage_to_scratch(const pty_problem& p, pty_gd_params& q) {
+  pty_gd_params t = q;
+  int s = (q.stage > 0 && q.stage < q.max_batch) ? stage_size(q) : q.max_batch;
+  size_t need = 0;
+  while (true) {
+    t.stage = s >= q.max_batch ? 0 : s;
+    need = layout(p, t).total;
+    if (need <= (size_t)PTY_NOALLOC_SCRATCH_BYTES) {
+      q.stage = t.stage;
+      return PTY_OK;
+    }
+    const int lower = ((s - 1) / 64) * 64;
+    if (lower < 64) break;
+    s = lower;
+  }
+  return fail(PTY_E_WORKSPACE,
+              "workspace of %zu bytes at 64 positions per launch exceeds the %zu-byte static scratch "
+              "(S=%d, object %dx%d); use pty_gd_run with a caller workspace",
+              need, (size_t)PTY_NOALLOC_SCRATCH_BYTES, p.S, p.Px, p.Py);
+}
+
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -2618,6 +2677,20 @@ size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* p
 int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const int* orders, double* errors,
                void* workspace, size_t workspace_bytes, pty_stream_t stream) {
   return gd_run(problem, params, nullptr, orders, errors, workspace, workspace_bytes, st(stream));
+}
+
+int pty_gd_run_noalloc(const pty_problem* problem, const pty_gd_params* params, const int* orders, double* errors,
+                       pty_stream_t stream) {
+  int rc = validate_problem(problem);
+  if (rc) return rc;
+  if (!params) return fail(PTY_E_ARG, "params is NULL");
+  if (params->max_batch <= 0 || params->max_batch > problem->N)
+    return fail(PTY_E_ARG, "bad max_batch=%d for N=%d", params->max_batch, problem->N);
+  pty_gd_params q = *params;
+  if ((rc = fit_stage_to_scratch(*problem, q))) return rc;
+  void* ws = nullptr;
+  if ((rc = scratch_pointer(&ws))) return rc;
+  return gd_run(problem, &q, nullptr, orders, errors, ws, (size_t)PTY_NOALLOC_SCRATCH_BYTES, st(stream));
 }
 
 size_t pty_gd_positions_workspace_bytes(const pty_problem* problem, const pty_gd_params* params) {
