@@ -12,17 +12,22 @@
  *  - Complex arrays are complex64: interleaved float32 (re, im), 8-byte aligned.
  *  - All buffers are device memory owned by the caller; the library keeps no
  *    pointer after a call returns and allocates no device memory (the one
- *    static array of pty_gd_run_noalloc aside).  pty_multi_run is the one
- *    entry that creates runtime objects (a stream, two events, a graph) and
- *    waits for its work before returning; every other entry is
+ *    static array of pty_gd_run_noalloc aside).  pty_multi_run creates
+ *    runtime objects (a stream, two events, a graph) and waits for its work
+ *    before returning; a pty_gd_run with pty_gd_params.pipeline set creates
+ *    a second stream and four events once per device (kept until the
+ *    process ends) and stays asynchronous; every other entry is
  *    asynchronous and creates none.
  *  - Every call is asynchronous on `stream` (a hipStream_t, NULL = default
  *    stream) and returns 0 on success or a negative PTY_E* code; the message
  *    is available from pty_last_error() on the calling thread.
- *  - Frame edge S must be 32, 64 or 128 (square frames; the batched runner
- *    pty_multi_run and the network entries take 64 or 128).  Objects need Px >= S
- *    and Py >= S.  "Kernel layout" amplitudes / probe spectra are stored in
- *    the FFT's permuted frequency order (see pty_permute_frames).
+ *  - Frame edge S must be 32, 64, 128 or 256 (square frames).  256 px frames
+ *    run through gradient descent (pty_gd_run and the single-batch entries);
+ *    projection sets, probe modes, the batched runner pty_multi_run and the
+ *    network entries take 32 to 128 (pty_multi_run and the network: 64 or
+ *    128).  Objects need Px >= S and Py >= S.  "Kernel layout" amplitudes /
+ *    probe spectra are stored in the FFT's permuted frequency order (see
+ *    pty_permute_frames; natural order at 256 px).
  */
 #ifndef PTY_HIP_H
 #define PTY_HIP_H
@@ -38,7 +43,8 @@ typedef void* pty_stream_t; /* hipStream_t */
 enum {
   PTY_OK = 0,
   PTY_E_ARG = -1,      /* bad size, null pointer or misalignment */
-  PTY_E_UNSUPPORTED = -2, /* frame size not compiled */
+  PTY_E_UNSUPPORTED = -2, /* frame size not compiled, option not on this path, or an
+                             object / staged batch beyond 2^31 elements */
   PTY_E_LAUNCH = -3,   /* HIP launch / runtime error */
   PTY_E_WORKSPACE = -4 /* workspace too small */
 };
@@ -253,6 +259,14 @@ typedef struct pty_gd_params {
   /* potential objects: clip V at zero after the shrinkage
    * (ptychographic_constraints.py:99-118, 458-462) */
   int object_positivity;
+  /* with stage < max_batch and no position correction: 1 runs the scatter
+   * and probe sums of stage k on a second stream of the library's, beside
+   * the fused pass of stage k + 1 (P_shift and chi staged double-buffered,
+   * twice the staging workspace).  A performance knob only: results are
+   * bitwise identical.  The library creates that stream and four events
+   * once per device, and serialises pipelined calls' launch sequences on a
+   * device with a lock. */
+  int pipeline;
 } pty_gd_params;
 
 size_t pty_gd_workspace_bytes(const pty_problem* problem, const pty_gd_params* params);
@@ -275,7 +289,8 @@ int pty_gd_run(const pty_problem* problem, const pty_gd_params* params, const in
  * until the process ends; its address on the current device is looked up
  * once, on the first call there.  Positions per fused launch
  * (params->stage) are lowered in multiples of 64 until the workspace fits,
- * which leaves the results bitwise equal to pty_gd_run's; a problem that
+ * which leaves the results bitwise equal to pty_gd_run's, and
+ * params->pipeline is ignored (one stream, nothing created); a problem that
  * does not fit even at 64 positions per launch (the object's running sums
  * take 24 bytes per pixel) returns PTY_E_WORKSPACE.  Nothing in the array
  * needs to start at zero.  Every call on one GPU uses the same array, so

@@ -39,7 +39,8 @@ class PtyGdParams(ctypes.Structure):
                 ("gaussian_weights", c_void_p), ("gaussian_radius", c_int),
                 ("butterworth", c_int), ("q_lowpass", ctypes.c_double),
                 ("q_highpass", ctypes.c_double), ("butterworth_order", ctypes.c_double),
-                ("sampling", ctypes.c_double * 2), ("object_positivity", c_int)]
+                ("sampling", ctypes.c_double * 2), ("object_positivity", c_int),
+                ("pipeline", c_int)]
 
 
 class PtyPositionParams(ctypes.Structure):

@@ -26,6 +26,12 @@ def default_stage(S, batch):
     return max(64, (2 << 30) // per_pos // 64 * 64)
 
 
+def default_execution(S, batch):
+    """(stage, pipeline) used when nothing was tuned: default_stage's
+    positions per fused launch, not pipelined."""
+    return default_stage(S, batch), False
+
+
 def _load():
     try:
         with open(CACHE_PATH) as f:

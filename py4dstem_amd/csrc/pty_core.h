@@ -247,6 +247,17 @@ PTY_HD int wrapi(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
+// wrapi for a value known to lie in [-n, 2n): one conditional correction
+// per side instead of an integer division
+PTY_HD int wrap1(int a, int n) {
+  a += a < 0 ? n : 0;
+  return a - (a >= n ? n : 0);
+}
+
+// wrapi with the division only off the common path (an index already in
+// [0, n) is returned as is)
+PTY_HD int wrapi_fast(int a, int n) { return (unsigned)a < (unsigned)n ? a : wrapi(a, n); }
+
 // Passes on one line (a row or a column) of a frame.  v[] is the thread's
 // register block; for A passes v[m] sits at line position n2 + N2*m, for B
 // passes v[j] sits at position N1*t + j.
@@ -255,13 +266,17 @@ template <int S> struct Pass {
   static constexpr int N1 = F::N1, N2 = F::N2, R = F::R;
 
   // tw: the twiddle table of Fused<S>::twiddles, W^(n2 k1) = exp(-2 pi i n2 k1 / S)
-  // stored twice: at n2 * N1 + k1 (the A pass reads a thread's N1 values at
-  // one base address) and at S + k1 * N2 + n2 (the B' pass reads N2 values
-  // at one base), so every twiddle load is base + immediate offset
+  // stored twice: at n2 (N1 + 1) + k1 (the A pass reads a thread's N1 values
+  // at one base address) and at TW_B + k1 (N2 + 1) + n2 (the B' pass reads N2
+  // values at one base), so every twiddle load is base + immediate offset.
+  // The one-word pad per table row keeps the lanes of a row pass (which
+  // differ in n2, or k1) on distinct LDS banks.
+  static constexpr int TW_A = N2 * (N1 + 1);
+  static constexpr int TW_WORDS = TW_A + N1 * (N2 + 1);
   template <bool INV> static PTY_HD void A(int n2, cf* v, const cf* tw) {
     if (!INV) {
       dft<N1, false>(v);
-      const cf* w = tw + n2 * N1;
+      const cf* w = tw + n2 * (N1 + 1);
 #pragma unroll
       for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], w[k1]);
     } else {
@@ -274,7 +289,7 @@ template <int S> struct Pass {
       cf* u = v + g * N2;
       dft<N2, INV>(u);
       if (INV) {
-        const cf* w = tw + S + (R * t + g) * N2;  // k1 = R t + g
+        const cf* w = tw + TW_A + (R * t + g) * (N2 + 1);  // k1 = R t + g
 #pragma unroll
         for (int n2 = 1; n2 < N2; ++n2) u[n2] = cmul(u[n2], cconj(w[n2]));
       }
@@ -360,21 +375,23 @@ template <int S> struct Fused {
   static constexpr int ROWS = F::LDS_WORDS + 2 * S;
   static constexpr int LDS_TOTAL = F::LDS_WORDS + 3 * S;
 
-  // twiddle table into LDS (tid < 2 S): W^(n2 k1) at n2 * N1 + k1 for the A
-  // passes and at S + k1 * N2 + n2 for the B' passes (n2 < N2, k1 < N1;
-  // n2 k1 < S), each the float32 rounding of the float64 value
-  static constexpr int TW_WORDS = 2 * S;
-  template <class L> static PTY_HD void twiddles(int tid, L tw) {
-    if ((unsigned)tid < (unsigned)(2 * S)) {
-      const int u = tid < S ? tid : tid - S;
-      const int m = tid < S ? (u / N1) * (u % N1) : (u / N2) * (u % N2);
+  // twiddle table into LDS: words tid, tid + nthreads, ... < TW_WORDS of the
+  // two layouts of Pass<S> (pad words included), each W^(n2 k1) the float32
+  // rounding of the float64 value (n2 < N2, k1 < N1, n2 k1 < S)
+  static constexpr int TW_WORDS = Pass<S>::TW_WORDS;
+  template <class L> static PTY_HD void twiddles(int tid, L tw, int nthreads) {
+    if (tid < 0) return;
+    for (int i = tid; i < TW_WORDS; i += nthreads) {
+      const bool a = i < Pass<S>::TW_A;
+      const int u = a ? i : i - Pass<S>::TW_A;
+      const int m = a ? (u / (N1 + 1)) * (u % (N1 + 1)) : (u / (N2 + 1)) * (u % (N2 + 1));
 #if defined(__HIP_DEVICE_COMPILE__)
       double sn, cs;
       sincospi(-2.0 * m / S, &sn, &cs);
 #else
       double cs = std::cos(-2.0 * M_PI * m / S), sn = std::sin(-2.0 * M_PI * m / S);
 #endif
-      tw.st(tid, mkcf((float)cs, (float)sn));
+      tw.st(i, mkcf((float)cs, (float)sn));
     }
   }
 
@@ -1204,8 +1221,11 @@ struct GatherArgs {
   int run_last;            // this stage writes grad/spsum/tilemax from the sums
 };
 
+// a window that meets the tile: its first staged element, and the window
+// coordinates (row, column) of the tile origin, in [0, Px) x [0, Py)
 struct GatherHit {
-  int b, x0, y0;
+  unsigned base;
+  int dr, dc;
 };
 
 // LDS of one gather work group, and the view the GPU runner hands to phases
@@ -1236,6 +1256,8 @@ struct Gather {
     int cnt;
     int total;
     double re, im, s;
+    int hdr[GATHER_CAND], hdc[GATHER_CAND];  // tile origin in this thread's candidate windows
+    unsigned hits;                           // bit k: candidate k hits the tile
   };
 
   static PTY_HD void origin(const GatherArgs& a, int tile, int& R0, int& C0) {
@@ -1243,17 +1265,19 @@ struct Gather {
     C0 = (tile % a.tiles_y) * GT_C;
   }
 
-  // anchor of slot b, wrapped into the object; hit test against the tile
-  static PTY_HD bool test(const GatherArgs& a, int R0, int C0, int b, int& x0, int& y0) {
+  // hit test of slot b's window against the tile; (dr, dc): the tile origin
+  // in window coordinates, wrapped into the object (R0, C0 in the object, the
+  // anchor wrapped into it and S <= Px, Py keep them within one wrap)
+  static PTY_HD bool test(const GatherArgs& a, int R0, int C0, int b, int& dr, int& dc) {
     int idx = a.order[b];
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
-    x0 = wrapi((int)rintf(a.positions[2 * idx]), a.Px);
-    y0 = wrapi((int)rintf(a.positions[2 * idx + 1]), a.Py);
+    const int x0 = wrapi_fast((int)rintf(a.positions[2 * idx]), a.Px);
+    const int y0 = wrapi_fast((int)rintf(a.positions[2 * idx + 1]), a.Py);
     const int S = a.S;
     const int tr = (a.Px - R0) < GT_R ? (a.Px - R0) : GT_R;
     const int tc = (a.Py - C0) < GT_C ? (a.Py - C0) : GT_C;
-    const int dr = wrapi(R0 - x0 + S / 2, a.Px);
-    const int dc = wrapi(C0 - y0 + S / 2, a.Py);
+    dr = wrap1(R0 - x0 + S / 2, a.Px);
+    dc = wrap1(C0 - y0 + S / 2, a.Py);
     return (dr < S || dr + tr > a.Px) && (dc < S || dc + tc > a.Py);
   }
 
@@ -1272,12 +1296,19 @@ struct Gather {
   // phase 1: count hits among this thread's candidates of the segment
   template <class L> static PTY_HD void p_count(int tid, const GatherArgs& a, int tile, int seg0,
                                                 Regs& r, L scan0) {
-    int R0, C0, x0, y0;
+    int R0, C0, dr, dc;
     origin(a, tile, R0, C0);
     int c = 0;
+    r.hits = 0u;
+#pragma unroll
     for (int k = 0; k < GATHER_CAND; ++k) {
       const int b = seg0 + tid * GATHER_CAND + k;
-      if (b < a.B && test(a, R0, C0, b, x0, y0)) ++c;
+      if (b < a.B && test(a, R0, C0, b, dr, dc)) {
+        ++c;
+        r.hits |= 1u << k;
+        r.hdr[k] = dr;
+        r.hdc[k] = dc;
+      }
     }
     r.cnt = c;
     scan0.st(tid, c);
@@ -1291,29 +1322,31 @@ struct Gather {
   }
 
   // phase 3: write this thread's hits at its offset of the in-order list
+  // (the hit tests of p_count, kept in registers)
   template <class L, class H> static PTY_HD void p_fill(int tid, const GatherArgs& a, int tile,
                                                        int seg0, Regs& r, L incl, H list) {
-    int R0, C0, x0, y0;
-    origin(a, tile, R0, C0);
+    (void)tile;
+    const unsigned SS = (unsigned)(a.S * a.S);
     int o = incl.ld(tid) - r.cnt;
     r.total = incl.ld(GATHER_THREADS - 1);
+#pragma unroll
     for (int k = 0; k < GATHER_CAND; ++k) {
-      const int b = seg0 + tid * GATHER_CAND + k;
-      if (b < a.B && test(a, R0, C0, b, x0, y0)) {
+      if (r.hits & (1u << k)) {
         GatherHit h;
-        h.b = b; h.x0 = x0; h.y0 = y0;
+        h.base = (unsigned)(seg0 + tid * GATHER_CAND + k) * SS;
+        h.dr = r.hdr[k];
+        h.dc = r.hdc[k];
         list.st(o++, h);
       }
     }
   }
 
-  // window element of pixel (row, col) for anchor (x0, y0), or -1 outside
-  static PTY_HD int welem(int row, int col, const GatherHit& h, int S, int Px, int Py) {
-    int dr = row - h.x0 + S / 2;
-    dr += dr < 0 ? Px : 0;
+  // window element of the pixel at (lr, lc) from the tile origin, or -1
+  // outside the window
+  static PTY_HD int welem(int lr, int lc, const GatherHit& h, int S, int Px, int Py) {
+    int dr = h.dr + lr;
     dr -= dr >= Px ? Px : 0;
-    int dc = col - h.y0 + S / 2;
-    dc += dc < 0 ? Py : 0;
+    int dc = h.dc + lc;
     dc -= dc >= Py ? Py : 0;
     if (dr >= S || dc >= S) return -1;
     return ((dr + S / 2) & (S - 1)) * S + ((dc + S / 2) & (S - 1));
@@ -1324,10 +1357,9 @@ struct Gather {
                                                      Regs& r, H list) {
     int R0, C0;
     origin(a, tile, R0, C0);
-    const int row = R0 + tid / GT_C, col = C0 + tid % GT_C;
-    if (row >= a.Px || col >= a.Py) return;
+    const int lr = tid / GT_C, lc = tid % GT_C;
+    if (R0 + lr >= a.Px || C0 + lc >= a.Py) return;
     const int S = a.S;
-    const size_t SS = (size_t)S * S;
     int h = 0;
     // PTY_GATHER_UNROLL hits per round: loads first, then the float64 adds
     // in order
@@ -1338,11 +1370,11 @@ struct Gather {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const GatherHit e = list.ld(h + u);
-        const int w = welem(row, col, e, S, a.Px, a.Py);
+        const int w = welem(lr, lc, e, S, a.Px, a.Py);
         in[u] = w >= 0;
-        const size_t o = (size_t)e.b * SS + (in[u] ? w : 0);
-        p[u] = a.ps[o];
-        c[u] = a.chi[o];
+        const unsigned o = e.base + (unsigned)(in[u] ? w : 0);
+        p[u] = elem(a.ps, o);
+        c[u] = elem(a.chi, o);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -1356,11 +1388,11 @@ struct Gather {
     }
     for (; h < r.total; ++h) {
       const GatherHit e = list.ld(h);
-      const int w = welem(row, col, e, S, a.Px, a.Py);
+      const int w = welem(lr, lc, e, S, a.Px, a.Py);
       if (w < 0) continue;
-      const size_t o = (size_t)e.b * SS + w;
-      const cf p = a.ps[o];
-      const cf g = cconjmul(p, a.chi[o]);
+      const unsigned o = e.base + (unsigned)w;
+      const cf p = elem(a.ps, o);
+      const cf g = cconjmul(p, elem(a.chi, o));
       r.re += (double)g.x;
       r.im += (double)g.y;
       r.s += (double)cabs2(p);

@@ -45,6 +45,10 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+int hip_check(hipError_t e, const char* what) {
+  return e == hipSuccess ? PTY_OK : fail(PTY_E_LAUNCH, "%s: %s", what, hipGetErrorString(e));
+}
+
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PTY_E_LAUNCH, "%s: %s", what, hipGetErrorString(e));
@@ -52,6 +56,16 @@ int check_launch(const char* what) {
 }
 
 __host__ __device__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// the kernels index the object and the staged frames with 32-bit element
+// offsets: both must stay below 2^31 elements
+int check_index_range(long long Px, long long Py, long long frames, long long S) {
+  const long long lim = 0x7fffffffll;
+  if (Px * Py > lim) return fail(PTY_E_UNSUPPORTED, "object %lldx%lld beyond 2^31 elements", Px, Py);
+  if (frames * S * S > lim)
+    return fail(PTY_E_UNSUPPORTED, "%lld staged %lldx%lld frames beyond 2^31 elements", frames, S, S);
+  return PTY_OK;
+}
 
 bool aligned8(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 7u) == 0; }
 
@@ -111,7 +125,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int b) {
   const int tid = threadIdx.x;
   // threads [2S, 3S) are idle in the setup phase; its closing barrier
   // publishes the table
-  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw}, Frame<S>::T - 2 * S);
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
   fused_position<S, GpuRunner<S>, MASK, POT>(run, a, b);
@@ -129,7 +143,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_mixed(FusedArgs a) {
   __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[Frame<S>::T / 64];
   const int tid = threadIdx.x;
-  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the first barrier
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw}, Frame<S>::T - 2 * S);  // published by the first barrier
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
   mixed_position<S>(run, a, a.b0 + (int)blockIdx.x);
@@ -142,7 +156,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_projection(FusedArgs a) {
   __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[Frame<S>::T / 64];
   const int tid = threadIdx.x;
-  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw}, Frame<S>::T - 2 * S);  // published by the setup barrier
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
   projection_position<S>(run, a, a.b0 + (int)blockIdx.x);
@@ -154,7 +168,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_grad(FusedArgs a) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
   __shared__ cf tw[Fused<S>::TW_WORDS];
   const int tid = threadIdx.x;
-  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw}, Frame<S>::T - 2 * S);  // published by the setup barrier
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, nullptr, a};
   probe_grad_position<S>(run, a, a.b0 + (int)blockIdx.x);
@@ -166,7 +180,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_simulate(FusedArgs a) {
   __shared__ cf lds_mem[Fused<S>::LDS_TOTAL];
   __shared__ cf tw[Fused<S>::TW_WORDS];
   const int tid = threadIdx.x;
-  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw});  // published by the setup barrier
+  Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw}, Frame<S>::T - 2 * S);  // published by the setup barrier
   typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, nullptr, a};
   simulate_position<S>(run, a, a.b0 + (int)blockIdx.x);
@@ -218,7 +232,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_position_correction(PosArgs a) 
   __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[Frame<S>::T / 64 * POS_SUMS];
   const int tid = threadIdx.x;
-  Fused<S>::twiddles(tid, LdsPtr{tw});  // published by the setup barrier
+  Fused<S>::twiddles(tid, LdsPtr{tw}, Frame<S>::T);  // published by the setup barrier
   typename PosCorr<S>::Regs r;
   GpuPosRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red};
   position_correction_position<S, GpuPosRunner<S>, ACC>(run, a, (int)blockIdx.x);
@@ -365,7 +379,7 @@ __device__ __forceinline__ void fft2_body(const cf* __restrict__ in, cf* __restr
   __shared__ cf lds_mem[F::LDS_WORDS];
   __shared__ cf tw[Fused<S>::TW_WORDS];
   const int tid = threadIdx.x;
-  K::twiddles(tid, LdsPtr{tw});
+  K::twiddles(tid, LdsPtr{tw}, Frame<S>::T);
   __syncthreads();
   const cf* src = in + (size_t)blk * S * S;
   cf* dst = out + (size_t)blk * S * S;
@@ -656,8 +670,8 @@ __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, in
   for (int t = threadIdx.x; t < nb; t += blockDim.x) {
     int idx = a.order[b0 + t];
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;
-    anchor[2 * t] = (int)rintf(a.positions[2 * idx]);
-    anchor[2 * t + 1] = (int)rintf(a.positions[2 * idx + 1]);
+    anchor[2 * t] = wrapi_fast((int)rintf(a.positions[2 * idx]), a.Px);
+    anchor[2 * t + 1] = wrapi_fast((int)rintf(a.positions[2 * idx + 1]), a.Py);
   }
   __syncthreads();
   const int pix = blk * blockDim.x + threadIdx.x;
@@ -667,14 +681,16 @@ __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, in
   const int fj = j < S / 2 ? j : j - S;
   const cf* chi = a.chi + (size_t)b0 * S * S + pix;
   double re = 0.0, im = 0.0, so = 0.0;
+  // anchors in the object and |fi|, |fj| <= S/2 <= Px/2, Py/2: one
+  // conditional wrap per side; offsets fit 32 bits (object < 2^31 elements)
 #pragma unroll 8
   for (int k = 0; k < nb; ++k) {  // batch order
     const int x0 = anchor[2 * k], y0 = anchor[2 * k + 1];
-    const size_t off = (size_t)wrapi(x0 + fi, a.Px) * a.Py + wrapi(y0 + fj, a.Py);
-    const cf o = a.object_pot ? expi(a.object_pot[off]) : a.object[off];
+    const unsigned off = (unsigned)(wrap1(x0 + fi, a.Px) * a.Py + wrap1(y0 + fj, a.Py));
+    const cf o = a.object_pot ? expi(elem(a.object_pot, off)) : elem(a.object, off);
     const float ao = hypotf(o.x, o.y);
     so += (double)(ao * ao);
-    const cf g = cconjmul(o, chi[(size_t)k * S * S]);
+    const cf g = cconjmul(o, elem(chi, (unsigned)(k * S * S)));
     re += (double)g.x;
     im += (double)g.y;
   }
@@ -909,7 +925,7 @@ template <int S> __device__ __forceinline__ void probe_com_body(cf* __restrict__
   __shared__ double red[3][T / 64];
   __shared__ float shift[2];
   const int tid = threadIdx.x;
-  K::twiddles(tid, LdsPtr{tw});
+  K::twiddles(tid, LdsPtr{tw}, Frame<S>::T);
   LdsPtr lds{lds_mem};
   typename K::Regs r;
   const int cc = F::col_line(tid), n2 = F::col_q(tid);
@@ -1030,7 +1046,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_shift_modes(cf* __restric
   __shared__ cf lds_mem[F::LDS_WORDS];
   __shared__ cf tw[Fused<S>::TW_WORDS];
   const int tid = threadIdx.x;
-  K::twiddles(tid, LdsPtr{tw});
+  K::twiddles(tid, LdsPtr{tw}, Frame<S>::T);
   LdsPtr lds{lds_mem};
   typename K::Regs r;
   cf* pk = probe + (size_t)blockIdx.x * S * S;
@@ -1121,7 +1137,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_aperture(cf* __restrict__
   __shared__ double red[T / 64];
   __shared__ float norm;
   const int tid = threadIdx.x;
-  K::twiddles(tid, LdsPtr{tw});
+  K::twiddles(tid, LdsPtr{tw}, Frame<S>::T);
   LdsPtr lds{lds_mem};
   typename K::Regs r;
   const int cc = F::col_line(tid), n2 = F::col_q(tid);
@@ -1877,17 +1893,59 @@ int stage_size(const pty_gd_params& q) {
 // correction reads the batch's shifted probes and far fields after the update.
 int stage_for(const pty_gd_params& q, bool positions) { return positions ? q.max_batch : stage_size(q); }
 
+// stages pipelined over two streams: P_shift and chi double-buffered
+bool pipelined(const pty_gd_params& q, bool positions) {
+  return q.pipeline != 0 && !positions && stage_size(q) < q.max_batch;
+}
+
+// the second stream and the stage events of pipelined runs, per device;
+// `mu` is held while a pipelined run enqueues its launches
+struct SidePipe {
+  std::mutex mu;
+  hipStream_t s = nullptr;
+  hipEvent_t fused_done[2] = {nullptr, nullptr};
+  hipEvent_t scatter_done[2] = {nullptr, nullptr};
+};
+constexpr int PIPE_MAX_DEVICES = 64;
+std::mutex g_pipe_mu;
+SidePipe g_pipe[PIPE_MAX_DEVICES];
+
+int side_pipe(SidePipe*& out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(PTY_E_LAUNCH, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= PIPE_MAX_DEVICES)
+    return fail(PTY_E_ARG, "device %d beyond the %d the pipeline table covers", dev, PIPE_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_pipe_mu);
+  SidePipe& sp = g_pipe[dev];
+  if (!sp.s) {
+    // default priority, not blocking on the null stream
+    if ((e = hipStreamCreateWithFlags(&sp.s, hipStreamNonBlocking)) != hipSuccess) {
+      sp.s = nullptr;
+      return fail(PTY_E_LAUNCH, "hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; i < 2; ++i) {
+      if ((e = hipEventCreateWithFlags(&sp.fused_done[i], hipEventDisableTiming)) != hipSuccess ||
+          (e = hipEventCreateWithFlags(&sp.scatter_done[i], hipEventDisableTiming)) != hipSuccess)
+        return fail(PTY_E_LAUNCH, "hipEventCreateWithFlags: %s", hipGetErrorString(e));
+    }
+  }
+  out = &sp;
+  return PTY_OK;
+}
+
 // projection sets stage no exit-wave differences (chi): the exit waves
 // themselves are the caller's persistent buffer
 WsLayout layout(const pty_problem& p, const pty_gd_params& q, bool positions = false, bool no_chi = false) {
   WsLayout L{};
   const size_t S2 = (size_t)p.S * p.S;
   const size_t B = (size_t)stage_for(q, positions);
+  const size_t nbuf = pipelined(q, positions) && !no_chi ? 2 : 1;
   const size_t npix = (size_t)p.Px * p.Py;
   size_t o = 0;
   L.phat = o; o = align256(o + S2 * 8);
-  L.ps = o; o = align256(o + B * S2 * 8);
-  L.chi = o; o = align256(o + (no_chi ? 0 : B * S2 * 8));
+  L.ps = o; o = align256(o + nbuf * B * S2 * 8);
+  L.chi = o; o = align256(o + (no_chi ? 0 : nbuf * B * S2 * 8));
   L.err = o; o = align256(o + (size_t)p.N * 8);
   L.grad = o; o = align256(o + npix * 8);
   L.spsum = o; o = align256(o + npix * 4);
@@ -1913,7 +1971,7 @@ int validate_problem(const pty_problem* p) {
     return fail(PTY_E_ARG, "object_type %d is neither PTY_OBJECT_COMPLEX nor PTY_OBJECT_POTENTIAL", p->object_type);
   if (!aligned8(p->probe) || (p->object_type == PTY_OBJECT_COMPLEX && !aligned8(p->object)))
     return fail(PTY_E_ARG, "complex buffers must be 8-byte aligned");
-  return PTY_OK;
+  return check_index_range(p->Px, p->Py, p->N, p->S);
 }
 
 #include "pty_multi.inc"
@@ -2022,6 +2080,17 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
   float* V = pot ? (float*)p.object : nullptr;
   cf* probe = (cf*)p.probe;
   const std::vector<int> batches = subdivide(p.N, q.max_batch);
+  // pipelined stages: stage k's scatter and probe sums on the side stream
+  // beside stage k + 1's fused pass; P_shift and chi alternate between two
+  // buffers, so the fused pass of stage k + 2 waits for the scatter of
+  // stage k, and the mini-batch's updates wait for its last scatter
+  SidePipe* pipe = nullptr;
+  std::unique_lock<std::mutex> pipe_lock;
+  if (pipelined(q, correct)) {
+    if ((rc = side_pipe(pipe))) return rc;
+    pipe_lock = std::unique_lock<std::mutex>(pipe->mu);
+  }
+  const size_t stage_elems = (size_t)stage * p.S * p.S;
   const int ntiles = num_tiles(p.Px, p.Py);
   const double norm = (double)(float)(p.mean_intensity * (float)p.N);
   bool phat_valid = false;
@@ -2038,16 +2107,26 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
         if ((rc = launch_fft(probe, phat, 1, p.S, false, 0, stream))) return rc;
         phat_valid = q.fix_probe != 0 || fft_with_update;
       }
-      for (int s0 = 0; s0 < nb; s0 += stage) {
+      int k = 0;
+      for (int s0 = 0; s0 < nb; s0 += stage, ++k) {
         const int ns = (nb - s0) < stage ? (nb - s0) : stage;
+        cf* ps_k = pipe ? ps + (size_t)(k & 1) * stage_elems : ps;
+        cf* chi_k = pipe ? chi + (size_t)(k & 1) * stage_elems : chi;
+        hipStream_t gs = pipe ? pipe->s : stream;
+        if (pipe && k >= 2 && (rc = hip_check(hipStreamWaitEvent(stream, pipe->scatter_done[k & 1], 0),
+                                              "hipStreamWaitEvent")))
+          return rc;
         FusedArgs a{};
         a.probe_hat = phat; a.object = obj; a.object_pot = V; a.positions = p.positions; a.amps = p.amplitudes;
         a.order = ob + s0; a.N = p.N; a.B = ns; a.Px = p.Px; a.Py = p.Py; a.b0 = 0;
-        a.ps_out = ps; a.chi_out = chi; a.err = err + start + s0; a.fwd_out = far;
+        a.ps_out = ps_k; a.chi_out = chi_k; a.err = err + start + s0; a.fwd_out = far;
         a.fmask = p.fourier_mask;
         if ((rc = launch_fused(a, p.S, stream))) return rc;
+        if (pipe && ((rc = hip_check(hipEventRecord(pipe->fused_done[k & 1], stream), "hipEventRecord")) ||
+                     (rc = hip_check(hipStreamWaitEvent(gs, pipe->fused_done[k & 1], 0), "hipStreamWaitEvent"))))
+          return rc;
         GatherArgs g{};
-        g.ps = ps; g.chi = chi; g.positions = p.positions; g.order = ob + s0; g.N = p.N; g.B = ns; g.S = p.S;
+        g.ps = ps_k; g.chi = chi_k; g.positions = p.positions; g.order = ob + s0; g.N = p.N; g.B = ns; g.S = p.S;
         g.Px = p.Px; g.Py = p.Py; g.tiles_y = gather_tiles_y(p.Py); g.grad = grad; g.spsum = spsum;
         g.tilemax = tilemax;
         if (ns < nb) {
@@ -2056,17 +2135,21 @@ int gd_run(const pty_problem* problem, const pty_gd_params* params, const pty_po
           g.run_last = s0 + ns == nb;
         }
         if (q.fix_probe) {
-          if ((rc = launch_gather(g, p.Px, p.Py, stream))) return rc;
+          if ((rc = launch_gather(g, p.Px, p.Py, gs))) return rc;
         } else {
-          ProbePartialArgs pa{chi, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
+          ProbePartialArgs pa{chi_k, obj, p.positions, ob + s0, p.N, ns, p.S, p.Px, p.Py,
                               ppart + (size_t)(s0 / PROBE_CHUNK) * p.S * p.S * 3};
           pa.object_pot = V;
           const int pix_blocks = (p.S * p.S + 255) / 256;
           hipLaunchKernelGGL(k_gather_probe, dim3(ntiles + pix_blocks * probe_nchunk(ns)), dim3(GATHER_THREADS), 0,
-                             stream, g, pa, ntiles, pix_blocks);
+                             gs, g, pa, ntiles, pix_blocks);
           if ((rc = check_launch("k_gather_probe"))) return rc;
         }
+        if (pipe && (rc = hip_check(hipEventRecord(pipe->scatter_done[k & 1], gs), "hipEventRecord"))) return rc;
       }
+      if (pipe && (rc = hip_check(hipStreamWaitEvent(stream, pipe->scatter_done[(k - 1) & 1], 0),
+                                  "hipStreamWaitEvent")))
+        return rc;
       // the probe step first: the object step may carry the probe's fft2
       if (!q.fix_probe) {
         if ((rc = launch_probe_apply(probe, ppart, p.S, nb, q.step_size, q.normalization_min, stream))) return rc;
@@ -2289,6 +2372,7 @@ int mixed_run(const pty_problem* problem, const pty_gd_params* params, int K, co
   if (p.object_type != PTY_OBJECT_COMPLEX)
     return fail(PTY_E_UNSUPPORTED, "potential objects run through pty_gd_run (gradient descent)");
   if (K < 1 || K > 64) return fail(PTY_E_ARG, "num_probes=%d out of range 1..64", K);
+  if ((rc = check_index_range(p.Px, p.Py, (long long)q.max_batch * K, p.S))) return rc;
   if (q.num_iter < 0 || q.max_batch <= 0 || q.max_batch > p.N)
     return fail(PTY_E_ARG, "bad num_iter=%d / max_batch=%d for N=%d", q.num_iter, q.max_batch, p.N);
   if (!orders || !errors || !workspace) return fail(PTY_E_ARG, "null orders/errors/workspace");
@@ -2445,6 +2529,7 @@ int pty_forward_adjoint_batch(const void* probe_hat, const void* object, int Px,
   if (N <= 0 || B < 0) return fail(PTY_E_ARG, "bad N=%d or B=%d", N, B);
   if (!probe_hat || !object || !positions || !amplitudes || !order || !ps || !chi || !err)
     return fail(PTY_E_ARG, "null buffer in forward_adjoint_batch");
+  if (int rc = check_index_range(Px, Py, B, S)) return rc;
   FusedArgs a{};
   a.probe_hat = (const cf*)probe_hat;
   a.object = (const cf*)object;
@@ -2462,6 +2547,7 @@ int pty_scatter_patches(const void* ps, const void* chi, const float* positions,
   if (!frame_supported_gd(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (Px < S || Py < S || N <= 0 || B < 0) return fail(PTY_E_ARG, "bad scatter sizes");
   if (!ps || !chi || !positions || !order || !grad || !spsum || !tilemax) return fail(PTY_E_ARG, "null buffer in scatter");
+  if (int rc = check_index_range(Px, Py, B, S)) return rc;
   GatherArgs a{};
   a.ps = (const cf*)ps; a.chi = (const cf*)chi; a.positions = positions; a.order = order;
   a.N = N; a.B = B; a.S = S; a.Px = Px; a.Py = Py; a.tiles_y = gather_tiles_y(Py);
@@ -2487,6 +2573,7 @@ int pty_probe_update(void* probe, const void* chi, const void* object, int Px, i
   if (!frame_supported_gd(S)) return fail(PTY_E_UNSUPPORTED, "frame size S=%d not supported", S);
   if (!probe || !chi || !object || !positions || !order || !partials || B <= 0 || N <= 0)
     return fail(PTY_E_ARG, "bad probe update arguments");
+  if (int rc = check_index_range(Px, Py, B, S)) return rc;
   return launch_probe_update((cf*)probe, (const cf*)chi, (const cf*)object, Px, Py, positions, N, order, B, S,
                              step_size, normalization_min, (double*)partials, st(stream));
 }
@@ -2687,6 +2774,7 @@ int pty_gd_run_noalloc(const pty_problem* problem, const pty_gd_params* params, 
   if (params->max_batch <= 0 || params->max_batch > problem->N)
     return fail(PTY_E_ARG, "bad max_batch=%d for N=%d", params->max_batch, problem->N);
   pty_gd_params q = *params;
+  q.pipeline = 0;  // the side stream would be created on first use
   if ((rc = fit_stage_to_scratch(*problem, q))) return rc;
   void* ws = nullptr;
   if ((rc = scratch_pointer(&ws))) return rc;
@@ -2753,6 +2841,7 @@ int pty_projection_batch(const void* probe_hat, const void* object, int Px, int 
   if (!probe_hat || !object || !positions || !amplitudes || !order || !projection || !projection->exit_waves ||
       !err)
     return fail(PTY_E_ARG, "null buffer in projection_batch");
+  if (int rc = check_index_range(Px, Py, B, S)) return rc;
   const double x = 1.0 - projection->a - projection->b, y = 1.0 - projection->c;
   FusedArgs a{};
   a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions;
@@ -2796,6 +2885,7 @@ int pty_mixed_forward_adjoint_batch(const void* probe_hat, int num_probes, const
   if (N <= 0 || B < 0 || num_probes < 1 || num_probes > 64) return fail(PTY_E_ARG, "bad N, B or num_probes");
   if (!probe_hat || !object || !positions || !amplitudes || !order || !ps || !chi || !err || !far)
     return fail(PTY_E_ARG, "null buffer in mixed_forward_adjoint_batch");
+  if (int rc = check_index_range(Px, Py, (long long)B * num_probes, S)) return rc;
   FusedArgs a{};
   a.probe_hat = (const cf*)probe_hat; a.object = (const cf*)object; a.positions = positions;
   a.amps = amplitudes; a.order = order; a.N = N; a.B = B; a.Px = Px; a.Py = Py; a.b0 = 0;

@@ -532,18 +532,22 @@ class SingleslicePtychography:
         return self
 
     # -- execution tuning (results do not depend on it) --------------------------
-    def set_stage(self, stage):
-        """Positions per fused launch inside a mini-batch (0 = whole batch).
-        A performance setting only: the update is bitwise identical."""
+    def set_stage(self, stage, pipeline=False):
+        """Positions per fused launch inside a mini-batch (0 = whole batch),
+        and whether gradient descent pipelines those launches: the scatter of
+        one stage runs on a second stream beside the fused pass of the next.
+        Performance settings only: the update is bitwise identical."""
         self._stage = int(stage)
+        self._pipeline = bool(pipeline)
         return self
 
     def _stage_for(self, batch):
+        """(positions per fused launch, pipelined) for mini-batches of `batch`."""
         stage = getattr(self, "_stage", None)
         if stage is None:
-            from .autotune import default_stage
-            stage = default_stage(int(self._region_of_interest_shape[0]), batch)
-        return stage
+            from .autotune import default_execution
+            return default_execution(int(self._region_of_interest_shape[0]), batch)
+        return stage, getattr(self, "_pipeline", False)
 
     # -- reconstruct ------------------------------------------------------------
     def _reset_reconstruction(self, store_iterations, reset):
@@ -760,8 +764,10 @@ class SingleslicePtychography:
         params = PtyGdParams(per_call, B, int(bool(fix_probe)), int(bool(fix_probe_com)),
                              # singleslice_ptychography.py:976
                              int(bool(pure_phase_object) and not potential), float(step_size),
-                             float(normalization_min), float(shrinkage_rad),
-                             int(self._stage_for(B)))
+                             float(normalization_min), float(shrinkage_rad))
+        stage, pipeline = self._stage_for(B)
+        params.stage = int(stage)
+        params.pipeline = int(bool(pipeline))
         params.object_positivity = int(bool(object_positivity))
         aperture_dev = None
         if fix_probe_aperture and not fix_probe:

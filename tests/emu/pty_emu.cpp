@@ -65,10 +65,8 @@ template <int S, class RegsT = typename Fused<S>::Regs> struct EmuRunner {
   EmuRunner(const FusedArgs* args)
       : mem(K::LDS_TOTAL, mkcf(0, 0)), wt(K::LDS_TOTAL, -1), rd(K::LDS_TOTAL, -1),
         ww(K::LDS_TOTAL, -1), rw(K::LDS_TOTAL, -1), regs(T), tw(K::TW_WORDS), a(args) {
-    for (int t = 0; t < K::TW_WORDS; ++t) {
-      struct TwSt { cf* p; void st(int i, cf v) const { p[i] = v; } } view{tw.data()};
-      K::twiddles(t, view);
-    }
+    struct TwSt { cf* p; void st(int i, cf v) const { p[i] = v; } } view{tw.data()};
+    K::twiddles(0, view, 1);
   }
   template <class Fn> void phase(Fn fn) {
     LdsCheck view{mem.data(), (int)mem.size(), wt.data(), rd.data(), ww.data(), rw.data(), &cur};

@@ -103,3 +103,32 @@ def test_error_message_is_per_call(lib):
     first = lib.last_error()
     _gd(lib, _problem(lib, N=0), _params(lib))
     assert lib.last_error() != first
+
+
+def test_sizes_beyond_32_bit_element_offsets_are_rejected(lib):
+    # the kernels index the object and the staged frames with 32-bit offsets
+    rc = _gd(lib, _problem(lib, Px=50000, Py=50000), _params(lib))
+    assert rc == PTY_E_UNSUPPORTED and "2^31" in lib.last_error()
+    rc = _gd(lib, _problem(lib, S=128, N=140000), _params(lib, max_batch=1000))
+    assert rc == PTY_E_UNSUPPORTED and "frames" in lib.last_error()
+    with pytest.raises(RuntimeError, match="2\\^31"):
+        lib.call("pty_scatter_patches", ctypes.c_void_p(FAKE), ctypes.c_void_p(FAKE),
+                 ctypes.c_void_p(FAKE), 200000, ctypes.c_void_p(FAKE), 140000, 128, 720, 720,
+                 ctypes.c_void_p(FAKE), ctypes.c_void_p(FAKE), ctypes.c_void_p(FAKE), None)
+
+
+def test_pipelined_workspace_doubles_the_staging(lib):
+    # P_shift and chi (8 bytes per pixel each) are double-buffered when the
+    # stages of a mini-batch are pipelined; without stages nothing changes
+    L = lib.load()
+    p = _problem(lib, S=128, Px=720, Py=720, N=4096)
+    q = _params(lib, max_batch=1024)
+    q.stage = 256
+    plain = L.pty_gd_workspace_bytes(ctypes.byref(p), ctypes.byref(q))
+    q.pipeline = 1
+    piped = L.pty_gd_workspace_bytes(ctypes.byref(p), ctypes.byref(q))
+    assert piped - plain == 2 * 256 * 128 * 128 * 8
+    q.stage = 0
+    whole = L.pty_gd_workspace_bytes(ctypes.byref(p), ctypes.byref(q))
+    q.pipeline = 0
+    assert whole == L.pty_gd_workspace_bytes(ctypes.byref(p), ctypes.byref(q))
