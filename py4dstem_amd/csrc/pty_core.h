@@ -50,6 +50,20 @@ PTY_HD cf cconjmul(cf a, cf b) { return mkcf(a.x * b.x + a.y * b.y, a.x * b.y - 
 PTY_HD cf cscale(cf a, float s) { return mkcf(a.x * s, a.y * s); }
 PTY_HD cf cconj(cf a) { return mkcf(a.x, -a.y); }
 PTY_HD float cabs2(cf a) { return a.x * a.x + a.y * a.y; }
+// conj(a) * b and |a|^2 with every product rounded to float32 before the
+// sum (no fused multiply-add), as NumPy's complex64 arithmetic forms them
+PTY_HD cf cconjmul_rn(cf a, cf b) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  return mkcf(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+PTY_HD float cabs2_rn(cf a) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  return a.x * a.x + a.y * a.y;
+}
 
 // reciprocal square root (hardware v_rsq_f32 on the GPU)
 PTY_HD float frsq(float x) {
@@ -246,6 +260,8 @@ PTY_HD int wrapi(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
 }
+
+PTY_HD unsigned umin(unsigned a, unsigned b) { return a < b ? a : b; }
 
 // wrapi for a value known to lie in [-n, 2n): one conditional correction
 // per side instead of an integer division
@@ -1184,7 +1200,9 @@ PTY_HD void position_correction_position(Run& run, const PosArgs& a, int b) {
 
 // ---------------------------------------------------------------------------
 // Deterministic scatter-add by object pixel ("sort by pixel").  One work group
-// owns a TR x TC object tile, one pixel per thread.  For each segment of the
+// owns a TR x TC object tile, GT_RPT pixels of one column per thread (the
+// column test of a hit is shared by them; the 64 lanes of a wave read 512
+// contiguous bytes of a patch row).  For each segment of the
 // batch it (1) tests which positions' windows touch the tile, (2) compacts
 // the hits into an in-order list in LDS with a parallel prefix scan, and
 // (3) every thread sums, in float64 and in list (= batch) order,
@@ -1193,15 +1211,21 @@ PTY_HD void position_correction_position(Run& run, const PosArgs& a, int b) {
 // np.bincount; the float32 products are formed exactly as the reference does.
 // ---------------------------------------------------------------------------
 #ifndef PTY_GT_R
-#define PTY_GT_R 4
+#define PTY_GT_R 8
 #define PTY_GT_C 64
+#endif
+#ifndef PTY_GATHER_ROWS
+#define PTY_GATHER_ROWS 2
 #endif
 #ifndef PTY_GATHER_UNROLL
 #define PTY_GATHER_UNROLL 4
 #endif
 constexpr int GT_R = PTY_GT_R;           // tile rows
 constexpr int GT_C = PTY_GT_C;           // tile columns (512-byte patch-row reads)
-constexpr int GATHER_THREADS = GT_R * GT_C;
+constexpr int GT_RPT = PTY_GATHER_ROWS;  // tile rows per thread, GT_R / GT_RPT apart
+constexpr int GT_RSTEP = GT_R / GT_RPT;
+static_assert(GT_R % GT_RPT == 0, "tile rows must split evenly over a thread's rows");
+constexpr int GATHER_THREADS = GT_RSTEP * GT_C;
 constexpr int GATHER_CAND = 4;           // candidates tested per thread per segment
 constexpr int GATHER_SEG = GATHER_THREADS * GATHER_CAND;
 
@@ -1255,7 +1279,7 @@ struct Gather {
   struct Regs {
     int cnt;
     int total;
-    double re, im, s;
+    double re[GT_RPT], im[GT_RPT], s[GT_RPT];  // rows lr + g * GT_RSTEP of the tile
     int hdr[GATHER_CAND], hdc[GATHER_CAND];  // tile origin in this thread's candidate windows
     unsigned hits;                           // bit k: candidate k hits the tile
   };
@@ -1283,14 +1307,17 @@ struct Gather {
 
   // running sums: zero, or carried over from the previous stage of the batch
   static PTY_HD void p_init(int tid, const GatherArgs& a, int tile, Regs& r) {
-    r.re = 0.0; r.im = 0.0; r.s = 0.0;
-    if (!a.run || a.run_first) return;
     int R0, C0;
     origin(a, tile, R0, C0);
-    const int row = R0 + tid / GT_C, col = C0 + tid % GT_C;
-    if (row >= a.Px || col >= a.Py) return;
-    const double* q = a.run + ((size_t)row * a.Py + col) * 3;
-    r.re = q[0]; r.im = q[1]; r.s = q[2];
+    const int col = C0 + tid % GT_C;
+#pragma unroll
+    for (int g = 0; g < GT_RPT; ++g) {
+      r.re[g] = 0.0; r.im[g] = 0.0; r.s[g] = 0.0;
+      const int row = R0 + tid / GT_C + g * GT_RSTEP;
+      if (!a.run || a.run_first || row >= a.Px || col >= a.Py) continue;
+      const double* q = a.run + ((size_t)row * a.Py + col) * 3;
+      r.re[g] = q[0]; r.im[g] = q[1]; r.s[g] = q[2];
+    }
   }
 
   // phase 1: count hits among this thread's candidates of the segment
@@ -1341,61 +1368,86 @@ struct Gather {
     }
   }
 
-  // window element of the pixel at (lr, lc) from the tile origin, or -1
-  // outside the window
-  static PTY_HD int welem(int lr, int lc, const GatherHit& h, int S, int Px, int Py) {
-    int dr = h.dr + lr;
-    dr -= dr >= Px ? Px : 0;
-    int dc = h.dc + lc;
-    dc -= dc >= Py ? Py : 0;
-    if (dr >= S || dc >= S) return -1;
-    return ((dr + S / 2) & (S - 1)) * S + ((dc + S / 2) & (S - 1));
+  // one pixel of the window: conj(P_shift) * chi and |P_shift|^2 added in
+  // float64
+  static PTY_HD void add_term(const GatherArgs& a, unsigned o, double& re, double& im, double& s) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    const cf p = elem(a.ps, o);
+    const cf g = cconjmul_rn(p, elem(a.chi, o));
+    re += (double)g.x;
+    im += (double)g.y;
+    s += (double)cabs2_rn(p);
   }
 
-  // phase 4: accumulate this thread's pixel over the segment's hit list
+  // phase 4: accumulate this thread's pixels over the segment's hit list,
+  // PTY_GATHER_UNROLL hits per round (the loads first, then the float64 adds
+  // in list order); the column test is shared by the thread's rows
   template <class H> static PTY_HD void p_accumulate(int tid, const GatherArgs& a, int tile,
                                                      Regs& r, H list) {
+    // separately rounded float32 products (no fused multiply-add), in every
+    // loop alike: the sums do not depend on how the hits fall into rounds
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
     int R0, C0;
     origin(a, tile, R0, C0);
     const int lr = tid / GT_C, lc = tid % GT_C;
     if (R0 + lr >= a.Px || C0 + lc >= a.Py) return;
     const int S = a.S;
-    int h = 0;
-    // PTY_GATHER_UNROLL hits per round: loads first, then the float64 adds
-    // in order
+    const unsigned uS = (unsigned)S, half = uS / 2;
+    // rows of this thread inside the object (the first one is)
+    bool row_ok[GT_RPT];
+#pragma unroll
+    for (int g = 0; g < GT_RPT; ++g) row_ok[g] = R0 + lr + g * GT_RSTEP < a.Px;
     constexpr int U = PTY_GATHER_UNROLL;
+    int h = 0;
     for (; h + U <= r.total; h += U) {
-      cf p[U], c[U];
-      bool in[U];
+      cf p[U][GT_RPT], c[U][GT_RPT];
+      bool in[U][GT_RPT];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const GatherHit e = list.ld(h + u);
-        const int w = welem(lr, lc, e, S, a.Px, a.Py);
-        in[u] = w >= 0;
-        const unsigned o = e.base + (unsigned)(in[u] ? w : 0);
-        p[u] = elem(a.ps, o);
-        c[u] = elem(a.chi, o);
+        unsigned dc = (unsigned)(e.dc + lc);
+        dc = umin(dc, dc - (unsigned)a.Py);
+        const bool col_in = dc < uS;
+        const unsigned wc = dc ^ half;
+#pragma unroll
+        for (int g = 0; g < GT_RPT; ++g) {
+          unsigned dr = (unsigned)(e.dr + lr + g * GT_RSTEP);
+          dr = umin(dr, dr - (unsigned)a.Px);
+          in[u][g] = col_in && dr < uS && row_ok[g];
+          const unsigned o = e.base + (in[u][g] ? ((dr ^ half) * uS | wc) : 0u);
+          p[u][g] = elem(a.ps, o);
+          c[u][g] = elem(a.chi, o);
+        }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        if (in[u]) {
-          const cf g = cconjmul(p[u], c[u]);
-          r.re += (double)g.x;
-          r.im += (double)g.y;
-          r.s += (double)cabs2(p[u]);
+#pragma unroll
+        for (int g = 0; g < GT_RPT; ++g) {
+          if (in[u][g]) {
+            const cf gg = cconjmul_rn(p[u][g], c[u][g]);
+            r.re[g] += (double)gg.x;
+            r.im[g] += (double)gg.y;
+            r.s[g] += (double)cabs2_rn(p[u][g]);
+          }
         }
       }
     }
     for (; h < r.total; ++h) {
       const GatherHit e = list.ld(h);
-      const int w = welem(lr, lc, e, S, a.Px, a.Py);
-      if (w < 0) continue;
-      const unsigned o = e.base + (unsigned)w;
-      const cf p = elem(a.ps, o);
-      const cf g = cconjmul(p, elem(a.chi, o));
-      r.re += (double)g.x;
-      r.im += (double)g.y;
-      r.s += (double)cabs2(p);
+      unsigned dc = (unsigned)(e.dc + lc);
+      dc = umin(dc, dc - (unsigned)a.Py);
+      if (dc >= uS) continue;
+      const unsigned wc = dc ^ half;
+#pragma unroll
+      for (int g = 0; g < GT_RPT; ++g) {
+        unsigned dr = (unsigned)(e.dr + lr + g * GT_RSTEP);
+        dr = umin(dr, dr - (unsigned)a.Px);
+        if (dr < uS && row_ok[g]) add_term(a, e.base + ((dr ^ half) * uS | wc), r.re[g], r.im[g], r.s[g]);
+      }
     }
   }
 
@@ -1404,20 +1456,24 @@ struct Gather {
                                                 L red) {
     int R0, C0;
     origin(a, tile, R0, C0);
-    const int row = R0 + tid / GT_C, col = C0 + tid % GT_C;
+    const int col = C0 + tid % GT_C;
     float m = 0.f;
-    if (row < a.Px && col < a.Py) {
+#pragma unroll
+    for (int g = 0; g < GT_RPT; ++g) {
+      const int row = R0 + tid / GT_C + g * GT_RSTEP;
+      if (row >= a.Px || col >= a.Py) continue;
       const size_t o = (size_t)row * a.Py + col;
       if (a.acc) {
-        a.acc[2 * o] += r.re;
-        a.acc[2 * o + 1] += r.im;
+        a.acc[2 * o] += r.re[g];
+        a.acc[2 * o + 1] += r.im[g];
       } else if (a.run && !a.run_last) {
         double* q = a.run + o * 3;
-        q[0] = r.re; q[1] = r.im; q[2] = r.s;
+        q[0] = r.re[g]; q[1] = r.im[g]; q[2] = r.s[g];
       } else {
-        a.grad[o] = mkcf((float)r.re, (float)r.im);
-        m = (float)r.s;
-        a.spsum[o] = m;
+        a.grad[o] = mkcf((float)r.re[g], (float)r.im[g]);
+        const float v = (float)r.s[g];
+        a.spsum[o] = v;
+        m = v > m ? v : m;
       }
     }
     red.st(tid, m);

@@ -660,6 +660,9 @@ struct ProbePartialArgs {
 };
 
 __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, int blk, int chunk) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   __shared__ int anchor[2 * PROBE_CHUNK];
   const int S = a.S;
   const int b0 = chunk * PROBE_CHUNK;
@@ -688,9 +691,9 @@ __device__ __forceinline__ void probe_partial_body(const ProbePartialArgs& a, in
     const int x0 = anchor[2 * k], y0 = anchor[2 * k + 1];
     const unsigned off = (unsigned)(wrap1(x0 + fi, a.Px) * a.Py + wrap1(y0 + fj, a.Py));
     const cf o = a.object_pot ? expi(elem(a.object_pot, off)) : elem(a.object, off);
-    const float ao = hypotf(o.x, o.y);
+    const float ao = hypot_f(o.x, o.y);
     so += (double)(ao * ao);
-    const cf g = cconjmul(o, elem(chi, (unsigned)(k * S * S)));
+    const cf g = cconjmul_rn(o, elem(chi, (unsigned)(k * S * S)));
     re += (double)g.x;
     im += (double)g.y;
   }
@@ -863,13 +866,13 @@ __device__ __forceinline__ void constraint_apply_body(cf* __restrict__ obj, long
     cf o = obj[p];
     if (do_shrink) {
       const float ph = atan2f(o.y, o.x) - shrink;
-      const float am = hypotf(o.x, o.y);
+      const float am = hypot_f(o.x, o.y);
       float sn, cs;
       sincosf(ph, &sn, &cs);
       o = mkcf(am * cs, am * sn);
     }
     const float ph = atan2f(o.y, o.x);
-    const float am = pure_phase ? 1.0f : fminf(hypotf(o.x, o.y), 1.0f);
+    const float am = pure_phase ? 1.0f : fminf(hypot_f(o.x, o.y), 1.0f);
     float sn, cs;
     sincosf(ph, &sn, &cs);
     obj[p] = mkcf(am * cs, am * sn);
@@ -934,7 +937,7 @@ template <int S> __device__ __forceinline__ void probe_com_body(cf* __restrict__
   for (int m = 0; m < F::N1; ++m) {
     const int i = n2 + F::N2 * m;
     r.v[m] = probe[i * S + cc];
-    const float a = hypotf(r.v[m].x, r.v[m].y);
+    const float a = hypot_f(r.v[m].x, r.v[m].y);
     const double I = (double)(a * a);
     sI += I;
     sx += (double)F::fidx(i) * I;
@@ -1012,7 +1015,7 @@ __global__ __launch_bounds__(Frame<S>::T) void k_probe_com_measure(const cf* __r
   for (int m = 0; m < F::N1; ++m) {
     const int i = n2 + F::N2 * m;
     const cf v = probe[i * S + cc];
-    const float a = hypotf(v.x, v.y);
+    const float a = hypot_f(v.x, v.y);
     const double I = (double)(a * a);
     sI += I;
     sx += (double)F::fidx(i) * I;

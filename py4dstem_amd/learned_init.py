@@ -399,6 +399,30 @@ def _data_errors(p, objects, probe, order):
     return losses.cpu().numpy().tolist()
 
 
+_TABLES = {}
+
+
+def _grid_tables(nx, ny, stride, n_judge, seed, dev):
+    """Device tables of a raster scan, kept per (scan, stride, judge, device):
+    raster indices of the network's patch centres (every `stride`-th grid
+    point per axis plus the last), their 3x3 neighbour tables, and the
+    judge's seeded positions."""
+    key = (nx, ny, stride, n_judge, seed, str(dev))
+    hit = _TABLES.get(key)
+    if hit is None:
+        gi = np.unique(np.r_[np.arange(0, nx, stride), nx - 1])
+        gj = np.unique(np.r_[np.arange(0, ny, stride), ny - 1])
+        ii, jj = np.meshgrid(gi, gj, indexing="ij")
+        ij = np.stack([ii.ravel(), jj.ravel()], 1)
+        centres = torch.from_numpy((ij[:, 0] * ny + ij[:, 1]).astype(np.int64)).to(dev)
+        nbr = torch.from_numpy(neighbour_table((nx, ny), ij)).to(dev)
+        rng = np.random.default_rng(seed)
+        order = torch.from_numpy(rng.choice(nx * ny, n_judge, replace=False).astype(np.int32)).to(dev)
+        hit = (centres, nbr, order, int(len(ij)))
+        _TABLES[key] = hit
+    return hit
+
+
 def initialize_object(p, net=None, stride=4, judge_positions=512, seed=0):
     """Learned starting object for a preprocessed SingleslicePtychography
     with a raster scan: network on every `stride`-th position per scan axis,
@@ -415,27 +439,19 @@ def initialize_object(p, net=None, stride=4, judge_positions=512, seed=0):
     if nx * ny != p._num_diffraction_patterns:
         raise ValueError("learned initialiser needs a raster scan (one pattern per grid point)")
     t0 = time.perf_counter()
-    gi = np.unique(np.r_[np.arange(0, nx, stride), nx - 1])
-    gj = np.unique(np.r_[np.arange(0, ny, stride), ny - 1])
-    ii, jj = np.meshgrid(gi, gj, indexing="ij")
-    ij = np.stack([ii.ravel(), jj.ravel()], 1)
-    centres = torch.from_numpy((ij[:, 0] * ny + ij[:, 1]).astype(np.int64)).to(dev)
-    nbr = torch.from_numpy(neighbour_table((nx, ny), ij)).to(dev)
+    n = min(judge_positions, p._num_diffraction_patterns)
+    centres, nbr, order, npatch = _grid_tables(nx, ny, stride, n, seed, dev)
     x = net_inputs(p._amplitudes_dev, 1, nbr)
     y = net.predict(x, chunk=8192)
     learned = stitch(y, p._positions_dev[centres], tuple(p._object_shape))
     probe = torch.from_numpy(np.ascontiguousarray(p._probe_initial, dtype=np.complex64)).to(dev)
-    rng = np.random.default_rng(seed)
-    n = min(judge_positions, p._num_diffraction_patterns)
-    order = torch.from_numpy(rng.choice(p._num_diffraction_patterns, n, replace=False)
-                             .astype(np.int32)).to(dev)
     flat = getattr(p, "_object_initial_dev", None)
     if flat is None:
         flat = torch.from_numpy(np.ascontiguousarray(p._object_initial, dtype=np.complex64)).to(dev)
     e_learned, e_flat = _data_errors(p, (learned, flat), probe, order)
     use = e_learned < e_flat
     info = {"seconds": time.perf_counter() - t0, "error_learned": e_learned, "error_flat": e_flat,
-            "patches": int(len(ij)), "used_learned": bool(use)}
+            "patches": npatch, "used_learned": bool(use)}
     return (learned if use else flat), bool(use), info
 
 

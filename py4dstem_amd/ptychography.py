@@ -93,6 +93,37 @@ def subdivide_into_batches(num_items, max_batch):
     return [base + 1] * extra + [base] * (num_batches - extra)
 
 
+class _DeviceMirror:
+    """A host array not yet downloaded: the device tensor holding its values
+    (never written afterwards)."""
+
+    __slots__ = ("tensor",)
+
+    def __init__(self, tensor):
+        self.tensor = tensor
+
+    def download(self):
+        return self.tensor.cpu().numpy()
+
+
+def _host_attr(name):
+    """Property for a host array that may be held as a _DeviceMirror until
+    it is first read."""
+    key = name + "_host"
+
+    def get(self):
+        v = self.__dict__.get(key)
+        if isinstance(v, _DeviceMirror):
+            v = v.download()
+            self.__dict__[key] = v
+        return v
+
+    def put(self, value):
+        self.__dict__[key] = value
+
+    return property(get, put)
+
+
 class SingleslicePtychography:
     """Single-slice ptychographic reconstruction (GD) on MI355X.
 
@@ -103,6 +134,12 @@ class SingleslicePtychography:
     """
 
     _class_specific_metadata = ()
+
+    # the starting object after the learned initialiser, and the object after
+    # a reset from a device-resident start, stay on the device until the host
+    # array is read
+    _object = _host_attr("_object")
+    _object_initial = _host_attr("_object_initial")
 
     def __init__(self, energy, datacube=None, semiangle_cutoff=None, semiangle_cutoff_pixels=None,
                  rolloff=2.0, vacuum_probe_intensity=None, polar_parameters=None,
@@ -522,9 +559,10 @@ class SingleslicePtychography:
                                                          judge_positions=judge_positions)
         self.learned_init_info = info
         if used:
-            self._object = obj.cpu().numpy()
-            self._object_initial = self._object.copy()
-            # the next run (and any reset) starts from the device copy
+            # the next run (and any reset) starts from the device copy; the
+            # host arrays are downloaded when first read
+            self._object = _DeviceMirror(obj)
+            self._object_initial = _DeviceMirror(obj)
             self._object_initial_dev = obj
             self._object_dev = obj.clone()
             if hasattr(self, "_probe_dev"):
@@ -557,7 +595,10 @@ class SingleslicePtychography:
             self.probe_iterations = []
         if reset is True:
             self.error_iterations = []
-            self._object = self._object_initial.copy()
+            init_dev = getattr(self, "_object_initial_dev", None)
+            # (the device copy holds the values of _object_initial)
+            self._object = (_DeviceMirror(init_dev) if init_dev is not None
+                            else self._object_initial.copy())
             self._probe = self._probe_initial.copy()
             self._positions_px = self._positions_px_initial.copy()
             if hasattr(self, "_positions_initial_dev"):
@@ -567,9 +608,8 @@ class SingleslicePtychography:
             for attr in ("_object_dev", "_probe_dev"):
                 if hasattr(self, attr):
                     delattr(self, attr)
-            init_dev = getattr(self, "_object_initial_dev", None)
             if init_dev is not None:
-                self._object_dev = init_dev.clone()  # same values as _object_initial
+                self._object_dev = init_dev.clone()
         elif reset is None:
             if hasattr(self, "error"):
                 warnings.warn("Continuing reconstruction from previous result. "

@@ -401,3 +401,27 @@ def test_mixed_state_phases_match_oracle(emu, frame_case, K):
     assert rel(ps.transpose(1, 0, 2, 3), sp) < 1e-6
     assert rel(chi.transpose(1, 0, 2, 3), ref) < 2e-5
     assert abs(err.sum() / e - 1) < 1e-5
+
+
+@pytest.mark.parametrize("S,shape", [(32, (101, 83)), (64, (71, 130))])
+def test_scatter_ragged_object_and_wrapped_windows(emu, S, shape):
+    """Object edges that cut the scatter tiles (sizes no multiple of the tile
+    rows or columns) and windows that wrap around both object edges: the
+    float64 sums must still equal np.bincount in batch order bit for bit."""
+    rng = np.random.default_rng(S + shape[0])
+    B = 300
+    pos = np.stack([rng.uniform(-S, shape[0] + S, B),
+                    rng.uniform(-S, shape[1] + S, B)], 1).astype(np.float32)
+    order = rng.permutation(B).astype(np.int32)
+    ps = (rng.standard_normal((B, S, S)) + 1j * rng.standard_normal((B, S, S))).astype(np.complex64)
+    grad = np.zeros(shape, np.complex64)
+    spsum = np.zeros(shape, np.float32)
+    tmax = np.zeros(emu.emu_gather_num_tiles(*shape), np.float32)
+    emu.emu_reset()
+    emu.emu_gather(S, cptr(ps), cptr(np.ones_like(ps)), cptr(pos), B, cptr(order), B,
+                   shape[0], shape[1], cptr(grad), cptr(spsum), cptr(tmax))
+    assert emu.emu_violations() == 0, emu.emu_first_violation()
+    np.testing.assert_array_equal(grad, po.scatter_bincount(np.conj(ps), pos[order], (S, S), shape))
+    sq = (ps.real * ps.real + ps.imag * ps.imag).astype(np.float32)
+    np.testing.assert_array_equal(spsum, po.scatter_bincount(sq, pos[order], (S, S), shape))
+    assert tmax.max() == spsum.max()
