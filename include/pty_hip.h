@@ -43,8 +43,9 @@ typedef void* pty_stream_t; /* hipStream_t */
 enum {
   PTY_OK = 0,
   PTY_E_ARG = -1,      /* bad size, null pointer or misalignment */
-  PTY_E_UNSUPPORTED = -2, /* frame size not compiled, option not on this path, or an
-                             object / staged batch beyond 2^31 elements */
+  PTY_E_UNSUPPORTED = -2, /* frame size not compiled, option not on this path, an
+                             object of 2^28 elements or more, or a staged batch
+                             beyond 2^31 elements */
   PTY_E_LAUNCH = -3,   /* HIP launch / runtime error */
   PTY_E_WORKSPACE = -4 /* workspace too small */
 };

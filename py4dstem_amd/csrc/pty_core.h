@@ -236,6 +236,54 @@ template <class T> PTY_HD const T& elem(const T* base, unsigned i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
 }
 
+// A complex64 array in global memory addressed through a buffer resource on
+// the GPU: element offset = a per-thread part (vi) plus a part the same for
+// the whole work group (si, kept in a scalar register or the instruction's
+// immediate), so a run of loads or stores costs no vector address
+// arithmetic.  The resource covers `elems` elements from `p` (elems * 8 below
+// 2^31 bytes); the hardware drops stores and zero-fills loads beyond it.
+// The host version checks the range (an emulator run fails on an access the
+// GPU would drop).
+struct CfBuf {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __amdgpu_buffer_rsrc_t r;
+#else
+  cf* p;
+  unsigned n;
+#endif
+};
+
+PTY_HD CfBuf cf_buf(const cf* p, unsigned elems) {
+  CfBuf b;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // raw buffer, stride 0; 0x00020000 = the CDNA (gfx9 family) data format word
+  b.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<cf*>(p), (short)0, (int)(elems * 8u), 0x00020000);
+#else
+  b.p = const_cast<cf*>(p);
+  b.n = elems;
+#endif
+  return b;
+}
+
+PTY_HD void cf_st(const CfBuf& b, unsigned vi, unsigned si, cf v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, vi * 8u, si * 8u, 0);
+#else
+  PTY_CHECK(vi + si < b.n, "buffer store out of range");
+  b.p[vi + si] = v;
+#endif
+}
+
+PTY_HD cf cf_ld(const CfBuf& b, unsigned vi, unsigned si) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(cf, __builtin_amdgcn_raw_buffer_load_b64(b.r, vi * 8u, si * 8u, 0));
+#else
+  PTY_CHECK(vi + si < b.n, "buffer load out of range");
+  return b.p[vi + si];
+#endif
+}
+
 // bit patterns between float32 and uint32 (LDS tables of offsets)
 PTY_HD float bits_f32(unsigned u) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -435,8 +483,8 @@ template <int S> struct Fused {
     const float px = a.positions[2 * idx], py = a.positions[2 * idx + 1];
     const float rx = rintf(px), ry = rintf(py);  // round half to even, like np.round
     r.idx = idx;
-    r.x0 = wrapi((int)rx, a.Px);
-    r.y0 = wrapi((int)ry, a.Py);
+    r.x0 = wrapi_fast((int)rx, a.Px);
+    r.y0 = wrapi_fast((int)ry, a.Py);
     r.err = 0.0;
     if (load_amps) {
       const float* A = a.amps + (size_t)idx * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
@@ -545,20 +593,21 @@ template <int S> struct Fused {
     p_colAinv_to_regs<L, false>(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
     if (a.ps_out) {
-      cf* ps_b = a.ps_out + (size_t)b * S * S;
+      const CfBuf psb = cf_buf(a.ps_out + (size_t)b * S * S, S * S);
 #pragma unroll
-      for (int m = 0; m < N1; ++m) elem(ps_b, (n2 + N2 * m) * S + cc) = r.v[m];
+      for (int m = 0; m < N1; ++m) cf_st(psb, n2 * S + cc, N2 * m * S, r.v[m]);
     }
     int col = r.y0 + F::fidx(cc);
     col += col < 0 ? a.Py : 0;
     col -= col >= a.Py ? a.Py : 0;
+    const CfBuf objb = cf_buf(POT ? nullptr : a.object, POT ? 0u : (unsigned)a.Px * (unsigned)a.Py);
 #pragma unroll
     for (int m = 0; m < N1; ++m) {
       // row offset from the set-up table: window row n2 + N2 m
       const unsigned o = f32_bits(lds.ld(ROWS + n2 + N2 * m).x) + (unsigned)col;
       PTY_CHECK(o < (unsigned)a.Px * (unsigned)a.Py && o == object_offset(a, r, n2 + N2 * m, cc),
                 "object index out of range");
-      r.v[m] = cmul(r.v[m], POT ? expi(elem(a.object_pot, o)) : elem(a.object, o));
+      r.v[m] = cmul(r.v[m], POT ? expi(elem(a.object_pot, o)) : cf_ld(objb, o, 0u));
     }
     P::template A<false>(n2, r.v, tw);
 #pragma unroll
@@ -576,12 +625,13 @@ template <int S> struct Fused {
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
-    cf* far = a.fwd_out ? a.fwd_out + (size_t)b * S * S : nullptr;
+    const bool far = a.fwd_out != nullptr;
+    const CfBuf farb = cf_buf(far ? a.fwd_out + (size_t)b * S * S : nullptr, far ? S * S : 0u);
     const float* fm = MASK ? a.fmask + rr * S + N1 * t : nullptr;
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
-      if (far) elem(far, j * T + tid) = r.v[j];
+      if (far) cf_st(farb, tid, j * T, r.v[j]);
       const float mj = MASK ? fm[j] : 1.f;
       const cf f = MASK ? cscale(r.v[j], mj) : r.v[j];
       const float amp = r.amp[j];
@@ -775,9 +825,9 @@ template <int S> struct Fused {
                                                   L lds, const cf* tw) {
     p_colAinv_to_regs(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
-    cf* chi_b = a.chi_out + (size_t)b * S * S;
+    const CfBuf chib = cf_buf(a.chi_out + (size_t)b * S * S, S * S);
 #pragma unroll
-    for (int m = 0; m < N1; ++m) elem(chi_b, (n2 + N2 * m) * S + cc) = r.v[m];
+    for (int m = 0; m < N1; ++m) cf_st(chib, n2 * S + cc, N2 * m * S, r.v[m]);
   }
 };
 

@@ -58,10 +58,12 @@ int check_launch(const char* what) {
 __host__ __device__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // the kernels index the object and the staged frames with 32-bit element
-// offsets: both must stay below 2^31 elements
+// offsets, and read the object through a buffer resource of 32-bit byte
+// range: the object must stay below 2^28 elements (16384^2; 2 GiB of
+// complex64), the staged frames below 2^31 elements
 int check_index_range(long long Px, long long Py, long long frames, long long S) {
   const long long lim = 0x7fffffffll;
-  if (Px * Py > lim) return fail(PTY_E_UNSUPPORTED, "object %lldx%lld beyond 2^31 elements", Px, Py);
+  if (Px * Py >= (1ll << 28)) return fail(PTY_E_UNSUPPORTED, "object %lldx%lld beyond 2^28 elements", Px, Py);
   if (frames * S * S > lim)
     return fail(PTY_E_UNSUPPORTED, "%lld staged %lldx%lld frames beyond 2^31 elements", frames, S, S);
   return PTY_OK;

@@ -107,8 +107,8 @@ def test_error_message_is_per_call(lib):
 
 def test_sizes_beyond_32_bit_element_offsets_are_rejected(lib):
     # the kernels index the object and the staged frames with 32-bit offsets
-    rc = _gd(lib, _problem(lib, Px=50000, Py=50000), _params(lib))
-    assert rc == PTY_E_UNSUPPORTED and "2^31" in lib.last_error()
+    rc = _gd(lib, _problem(lib, Px=16384, Py=16385), _params(lib))
+    assert rc == PTY_E_UNSUPPORTED and "2^28" in lib.last_error()
     rc = _gd(lib, _problem(lib, S=128, N=140000), _params(lib, max_batch=1000))
     assert rc == PTY_E_UNSUPPORTED and "frames" in lib.last_error()
     with pytest.raises(RuntimeError, match="2\\^31"):
