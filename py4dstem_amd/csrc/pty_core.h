@@ -590,24 +590,43 @@ template <int S> struct Fused {
   // POT: the object is a potential V and the patch exp(i V)
   template <class L, bool POT = false>
   static PTY_HD void p_exitwave_colA(int tid, const FusedArgs& a, int b, Regs& r, L lds, const cf* tw) {
-    p_colAinv_to_regs<L, false>(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
-    if (a.ps_out) {
-      const CfBuf psb = cf_buf(a.ps_out + (size_t)b * S * S, S * S);
-#pragma unroll
-      for (int m = 0; m < N1; ++m) cf_st(psb, n2 * S + cc, N2 * m * S, r.v[m]);
-    }
     int col = r.y0 + F::fidx(cc);
     col += col < 0 ? a.Py : 0;
     col -= col >= a.Py ? a.Py : 0;
-    const CfBuf objb = cf_buf(POT ? nullptr : a.object, POT ? 0u : (unsigned)a.Px * (unsigned)a.Py);
-#pragma unroll
-    for (int m = 0; m < N1; ++m) {
-      // row offset from the set-up table: window row n2 + N2 m
+    // window row n2 + N2 m: row offset from the set-up table
+    auto offset = [&](int m) {
       const unsigned o = f32_bits(lds.ld(ROWS + n2 + N2 * m).x) + (unsigned)col;
       PTY_CHECK(o < (unsigned)a.Px * (unsigned)a.Py && o == object_offset(a, r, n2 + N2 * m, cc),
                 "object index out of range");
-      r.v[m] = cmul(r.v[m], POT ? expi(elem(a.object_pot, o)) : cf_ld(objb, o, 0u));
+      return o;
+    };
+    auto store_ps = [&]() {
+      if (a.ps_out) {
+        const CfBuf psb = cf_buf(a.ps_out + (size_t)b * S * S, S * S);
+#pragma unroll
+        for (int m = 0; m < N1; ++m) cf_st(psb, n2 * S + cc, N2 * m * S, r.v[m]);
+      }
+    };
+    if constexpr (!POT) {
+      // a complex patch is requested first, so its loads are in flight
+      // during the inverse column pass
+      cf op[N1];
+      const CfBuf objb = cf_buf(a.object, (unsigned)a.Px * (unsigned)a.Py);
+#pragma unroll
+      for (int m = 0; m < N1; ++m) op[m] = cf_ld(objb, offset(m), 0u);
+#if defined(__HIP_DEVICE_COMPILE__)
+      __builtin_amdgcn_sched_barrier(0);
+#endif
+      p_colAinv_to_regs<L, false>(tid, r, lds, tw);
+      store_ps();
+#pragma unroll
+      for (int m = 0; m < N1; ++m) r.v[m] = cmul(r.v[m], op[m]);
+    } else {
+      p_colAinv_to_regs<L, false>(tid, r, lds, tw);
+      store_ps();
+#pragma unroll
+      for (int m = 0; m < N1; ++m) r.v[m] = cmul(r.v[m], expi(elem(a.object_pot, offset(m))));
     }
     P::template A<false>(n2, r.v, tw);
 #pragma unroll
