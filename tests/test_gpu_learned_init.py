@@ -43,7 +43,7 @@ def bf16_to_f64(h):
     return u.view(torch.float32).to(torch.float64)
 
 
-LAYERS = [(0, 2), (1, 2), (2, 2), (4, 1), (5, 2)]  # (layer index, samples)
+LAYERS = [(0, 2), (1, 2), (2, 2), (3, 3), (4, 1), (5, 2)]  # (layer index, samples)
 
 
 @pytest.mark.parametrize("li,n", LAYERS)
