@@ -158,8 +158,12 @@ template <bool INV> PTY_HD void dft16(cf* x) {
 #pragma unroll
     for (int k1 = 1; k1 < 4; ++k1) {
       const int m = n2 * k1;
-      cf w = mkcf(wc[m], sg * ws[m]);
-      y[n2 * 4 + k1] = cmul(y[n2 * 4 + k1], w);
+      if (m == 4) {  // W16^4 is the quarter turn: a swap and a sign
+        y[n2 * 4 + k1] = rot90<INV>(y[n2 * 4 + k1]);
+      } else {
+        cf w = mkcf(wc[m], sg * ws[m]);
+        y[n2 * 4 + k1] = cmul(y[n2 * 4 + k1], w);
+      }
     }
   }
 #pragma unroll
@@ -334,9 +338,14 @@ template <int S> struct Pass {
   // at one base address) and at TW_B + k1 (N2 + 1) + n2 (the B' pass reads N2
   // values at one base), so every twiddle load is base + immediate offset.
   // The one-word pad per table row keeps the lanes of a row pass (which
-  // differ in n2, or k1) on distinct LDS banks.
+  // differ in n2, or k1) on distinct LDS banks.  A third copy at TW_BS holds
+  // the B' layout times 1/S^2, for the last inverse pass of a transform whose
+  // output carries the ifft2 normalisation (a power of two: scaling the
+  // twiddles is exact, and every later step is a sum or a product by a
+  // constant, so the result equals the scaled-after one bit for bit).
   static constexpr int TW_A = N2 * (N1 + 1);
-  static constexpr int TW_WORDS = TW_A + N1 * (N2 + 1);
+  static constexpr int TW_BS = TW_A + N1 * (N2 + 1);
+  static constexpr int TW_WORDS = TW_BS + N1 * (N2 + 1);
   template <bool INV> static PTY_HD void A(int n2, cf* v, const cf* tw) {
     if (!INV) {
       dft<N1, false>(v);
@@ -347,13 +356,16 @@ template <int S> struct Pass {
       dft<N1, true>(v);
     }
   }
-  template <bool INV> static PTY_HD void B(int t, cf* v, const cf* tw) {
+  // SCALED (inverse only): the outputs carry the factor 1/S^2
+  template <bool INV, bool SCALED = false> static PTY_HD void B(int t, cf* v, const cf* tw) {
+    static_assert(INV || !SCALED, "only the inverse pass is scaled");
 #pragma unroll
     for (int g = 0; g < R; ++g) {
       cf* u = v + g * N2;
       dft<N2, INV>(u);
       if (INV) {
-        const cf* w = tw + TW_A + (R * t + g) * (N2 + 1);  // k1 = R t + g
+        const cf* w = tw + (SCALED ? TW_BS : TW_A) + (R * t + g) * (N2 + 1);  // k1 = R t + g
+        if (SCALED) u[0] = cscale(u[0], 1.0f / (float)(S * S));
 #pragma unroll
         for (int n2 = 1; n2 < N2; ++n2) u[n2] = cmul(u[n2], cconj(w[n2]));
       }
@@ -446,8 +458,8 @@ template <int S> struct Fused {
   template <class L> static PTY_HD void twiddles(int tid, L tw, int nthreads) {
     if (tid < 0) return;
     for (int i = tid; i < TW_WORDS; i += nthreads) {
-      const bool a = i < Pass<S>::TW_A;
-      const int u = a ? i : i - Pass<S>::TW_A;
+      const bool a = i < Pass<S>::TW_A, scaled = i >= Pass<S>::TW_BS;
+      const int u = a ? i : i - (scaled ? Pass<S>::TW_BS : Pass<S>::TW_A);
       const int m = a ? (u / (N1 + 1)) * (u % (N1 + 1)) : (u / (N2 + 1)) * (u % (N2 + 1));
 #if defined(__HIP_DEVICE_COMPILE__)
       double sn, cs;
@@ -455,7 +467,8 @@ template <int S> struct Fused {
 #else
       double cs = std::cos(-2.0 * M_PI * m / S), sn = std::sin(-2.0 * M_PI * m / S);
 #endif
-      tw.st(i, mkcf((float)cs, (float)sn));
+      const float k = scaled ? 1.0f / (float)(S * S) : 1.0f;
+      tw.st(i, mkcf((float)cs * k, (float)sn * k));
     }
   }
 
@@ -548,11 +561,12 @@ template <int S> struct Fused {
 #pragma unroll
     for (int j = 0; j < N1; ++j) lds.st(F::at(rr, N1 * t + j), r.v[j]);
   }
-  template <bool INV, class L, class RG> static PTY_HD void p_colB(int tid, RG& r, L lds, const cf* tw) {
+  template <bool INV, class L, class RG, bool SCALED = false>
+  static PTY_HD void p_colB(int tid, RG& r, L lds, const cf* tw) {
     const int cc = F::col_line(tid), t = F::col_q(tid);
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(N1 * t + j, cc));
-    P::template B<INV>(t, r.v, tw);
+    P::template B<INV, SCALED>(t, r.v, tw);
 #pragma unroll
     for (int j = 0; j < N1; ++j) lds.st(F::at(N1 * t + j, cc), r.v[j]);
   }
@@ -644,29 +658,26 @@ template <int S> struct Fused {
 #pragma unroll
     for (int j = 0; j < N1; ++j) r.v[j] = lds.ld(F::at(rr, N1 * t + j));
     P::template B<false>(t, r.v, tw);
-    const bool far = a.fwd_out != nullptr;
-    const CfBuf farb = cf_buf(far ? a.fwd_out + (size_t)b * S * S : nullptr, far ? S * S : 0u);
+    if (a.fwd_out) {  // one uniform branch for the thread's whole row of F
+      const CfBuf farb = cf_buf(a.fwd_out + (size_t)b * S * S, S * S);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) cf_st(farb, tid, j * T, r.v[j]);
+    }
     const float* fm = MASK ? a.fmask + rr * S + N1 * t : nullptr;
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
-      if (far) cf_st(farb, tid, j * T, r.v[j]);
       const float mj = MASK ? fm[j] : 1.f;
       const cf f = MASK ? cscale(r.v[j], mj) : r.v[j];
       const float amp = r.amp[j];
       const float m2 = cabs2(f);
-      cf g;
-      float mag;
-      // |F| = 0 takes angle 0 (NumPy's angle(0)); a NaN far field stays NaN
-      if (m2 != 0.f) {
-        const float inv = frsq(m2);
-        mag = m2 * inv;
-        g = PROJ ? mkcf(amp * (f.x * inv), amp * (f.y * inv))
-                 : mkcf(amp * (f.x * inv) - f.x, amp * (f.y * inv) - f.y);
-      } else {
-        mag = 0.f;
-        g = PROJ ? mkcf(amp, 0.f) : mkcf(amp - f.x, -f.y);
-      }
+      // |F| = 0 takes angle 0 (NumPy's angle(0)): the unit phasor (1, 0) in
+      // place of F/|F|; a NaN far field stays NaN.  Selects, not a branch.
+      const bool nz = m2 != 0.f;
+      const float inv = nz ? frsq(m2) : 0.f;
+      const float mag = m2 * inv;
+      const float ux = nz ? f.x * inv : 1.f, uy = f.y * inv;
+      const cf g = PROJ ? mkcf(amp * ux, amp * uy) : mkcf(amp * ux - f.x, amp * uy - f.y);
       const float d = amp - mag;
       acc += (double)(d * d);
       r.v[j] = MASK ? cscale(g, mj) : g;
@@ -839,10 +850,11 @@ template <int S> struct Fused {
     for (int j = 0; j < N1; ++j) out[F::kidx(N1 * t + j)] = cabs2(r.v[j]);
   }
 
-  // P11: chi (natural order) out
-  template <class L> static PTY_HD void p_outputs(int tid, const FusedArgs& a, int b, Regs& r,
-                                                  L lds, const cf* tw) {
-    p_colAinv_to_regs(tid, r, lds, tw);
+  // P11: chi (natural order) out; PRESCALED: the column pass B' before it
+  // already applied the 1/S^2
+  template <class L, bool PRESCALED = false>
+  static PTY_HD void p_outputs(int tid, const FusedArgs& a, int b, Regs& r, L lds, const cf* tw) {
+    p_colAinv_to_regs<L, !PRESCALED>(tid, r, lds, tw);
     const int cc = F::col_line(tid), n2 = F::col_q(tid);
     const CfBuf chib = cf_buf(a.chi_out + (size_t)b * S * S, S * S);
 #pragma unroll
@@ -885,9 +897,13 @@ PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
   run.barrier();
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::template p_colB<true, decltype(lds), Regs, true>(tid, r, lds, tw);
+  });
   run.barrier();
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_outputs(tid, a, b, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::template p_outputs<decltype(lds), true>(tid, a, b, r, lds, tw);
+  });
   run.reduce_err(b);  // ends with a barrier
 }
 
