@@ -272,7 +272,10 @@ PTY_HD CfBuf cf_buf(const cf* p, unsigned elems) {
 PTY_HD void cf_st(const CfBuf& b, unsigned vi, unsigned si, cf v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, vi * 8u, si * 8u, 0);
+  // cache policy 2: non-temporal (the frames are written once per
+  // position and read by another launch, so they should not evict the
+  // probe spectrum and the object from L2)
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, vi * 8u, si * 8u, 2);
 #else
   PTY_CHECK(vi + si < b.n, "buffer store out of range");
   b.p[vi + si] = v;
@@ -314,6 +317,18 @@ PTY_HD int wrapi(int a, int n) {
 }
 
 PTY_HD unsigned umin(unsigned a, unsigned b) { return a < b ? a : b; }
+
+// p[i] where the address is the same for the whole work group and the data
+// do not change during the launch: on the GPU a scalar load (constant
+// address space), which does not queue behind the work group's vector loads
+template <class T> PTY_HD T uniform_load(const T* p, unsigned i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) T* constant_ptr;
+  return ((constant_ptr)p)[i];
+#else
+  return p[i];
+#endif
+}
 
 // wrapi for a value known to lie in [-n, 2n): one conditional correction
 // per side instead of an integer division
@@ -490,10 +505,10 @@ template <int S> struct Fused {
   template <class L> static PTY_HD void p_setup(int tid, const FusedArgs& a, int b, Regs& r, L lds,
                                                 bool conj_ramp = false, bool load_amps = false,
                                                 bool load_probe = false) {
-    int idx = a.order ? a.order[b] : b;  // no order: slot b is position b
+    int idx = a.order ? uniform_load(a.order, (unsigned)b) : b;  // no order: slot b is position b
     PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;  // memory-safe on bad input
-    const float px = a.positions[2 * idx], py = a.positions[2 * idx + 1];
+    const float px = uniform_load(a.positions, 2u * idx), py = uniform_load(a.positions, 2u * idx + 1u);
     const float rx = rintf(px), ry = rintf(py);  // round half to even, like np.round
     r.idx = idx;
     r.x0 = wrapi_fast((int)rx, a.Px);
@@ -530,14 +545,17 @@ template <int S> struct Fused {
   }
 
   // P1: row pass B' of ifft2(probe_hat * ramp), input from global memory
-  template <class L> static PTY_HD void p_probe_rowBinv(int tid, const FusedArgs& a, Regs& r,
-                                                        L lds, const cf* tw) {
+  // FROM_LDS: the probe spectrum was staged in the frame (p_stage_probe);
+  // else this thread's values were loaded into r.v at setup
+  template <class L, bool FROM_LDS = false>
+  static PTY_HD void p_probe_rowBinv(int tid, const FusedArgs& a, Regs& r, L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
     const cf ex = lds.ld(RAMP + F::kidx(rr));
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
       const cf e = cmul(ex, lds.ld(RAMP + S + F::kidx(N1 * t + j)));
-      r.v[j] = cmul(r.v[j], e);  // probe spectrum loaded at setup
+      const cf p = FROM_LDS ? lds.ld(F::at(rr, N1 * t + j)) : r.v[j];
+      r.v[j] = cmul(p, e);
     }
     P::template B<true>(t, r.v, tw);
 #pragma unroll
@@ -735,6 +753,39 @@ template <int S> struct Fused {
     }
   }
 
+  // The probe spectrum in coalesced pieces: piece k (k < N1/2) of thread tid
+  // is the element pair 2 (k T + tid), 2 (k T + tid) + 1 of the [S][S]
+  // kernel-layout array, into r.v[2k], r.v[2k+1].  Consecutive lanes read
+  // consecutive 16-byte words (a thread's own row segment, 128 bytes at
+  // S = 128, would make every load instruction touch 64 cache lines).
+  static PTY_HD void p_load_probe_coalesced(int tid, const FusedArgs& a, Regs& r) {
+#pragma unroll
+    for (int k = 0; k < N1 / 2; ++k) {
+      const int e = 2 * (k * T + tid);
+      r.v[2 * k] = a.probe_hat[e];
+      r.v[2 * k + 1] = a.probe_hat[e + 1];
+    }
+  }
+  // the pieces of p_load_probe_coalesced into the frame's LDS layout (the
+  // pair shares a row and a 16-word block: e is even)
+  template <class L> static PTY_HD void p_stage_probe(int tid, Regs& r, L lds) {
+#pragma unroll
+    for (int k = 0; k < N1 / 2; ++k) {
+      const int e = 2 * (k * T + tid);
+      const int row = e / S, col = e % S;
+      lds.st(F::at(row, col), r.v[2 * k]);
+      lds.st(F::at(row, col) + 1, r.v[2 * k + 1]);
+    }
+  }
+
+  // this thread's measured amplitudes (row pass B layout) into r.amp; r.idx
+  // set by p_setup
+  static PTY_HD void p_load_amps(int tid, const FusedArgs& a, Regs& r) {
+    const float* A = a.amps + (size_t)r.idx * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) r.amp[j] = A[j];
+  }
+
   // ---- mixed state (ptychographic_methods.py:2696-2805) -------------------
   // mode k > 0: this thread's probe-spectrum values for the first row pass
   static PTY_HD void p_load_probe(int tid, const FusedArgs& a, int k, Regs& r) {
@@ -868,16 +919,34 @@ template <int S> struct Fused {
 // (the ramp table, column passes, row <-> column); ``run.wave_barrier()``
 // only orders the LDS accesses of one wave, enough between two passes along
 // rows.  Nine work-group barriers per position.
-template <int S, class Run, bool MASK = false, bool POT = false>
+//
+// The measured amplitudes are requested in the second phase, not at set-up:
+// every work group of a launch starts at about the same time, and set-up
+// loads beside the probe spectrum were issue-blocked behind the whole chip's
+// simultaneous reads (they are first used by the projection, five phases
+// later).  The probe spectrum is read in coalesced pieces
+// (p_load_probe_coalesced) and staged through the frame's LDS by the set-up
+// phase.  PROBE_LOADED: the caller has already loaded the pieces into r.v
+// (issued before its twiddle-table work), else the set-up phase loads them.
+template <int S, class Run, bool MASK = false, bool POT = false, bool PROBE_LOADED = false>
 PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
   using K = Fused<S>;
   using Regs = typename K::Regs;
   // ifft2(probe_hat * ramp): B', A' along rows, then B', A' along columns
-  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, false, true, true); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) {
+    if (!PROBE_LOADED) K::p_load_probe_coalesced(tid, a, r);
+    K::p_setup(tid, a, b, r, lds, false, false, false);
+    K::p_stage_probe(tid, r, lds);
+  });
   run.barrier();
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::template p_probe_rowBinv<decltype(lds), true>(tid, a, r, lds, tw);
+  });
   run.wave_barrier();
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::p_load_amps(tid, a, r);
+    K::template p_rowA<true>(tid, r, lds, tw);
+  });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
   run.barrier();

@@ -125,12 +125,15 @@ __device__ __forceinline__ void fused_body(const FusedArgs& a, int b) {
   __shared__ cf tw[Fused<S>::TW_WORDS];
   __shared__ double red[Frame<S>::T / 64];
   const int tid = threadIdx.x;
+  // the probe-spectrum loads (independent of the slot) go out first, ahead
+  // of the twiddle work and the set-up's slot/position reads
+  typename Fused<S>::Regs r;
+  Fused<S>::p_load_probe_coalesced(tid, a, r);
   // threads [2S, 3S) are idle in the setup phase; its closing barrier
   // publishes the table
   Fused<S>::twiddles(tid - 2 * S, LdsPtr{tw}, Frame<S>::T - 2 * S);
-  typename Fused<S>::Regs r;
   GpuRunner<S> run{r, LdsPtr{lds_mem}, tw, tid, red, a};
-  fused_position<S, GpuRunner<S>, MASK, POT>(run, a, b);
+  fused_position<S, GpuRunner<S>, MASK, POT, true>(run, a, b);
 }
 
 template <int S, bool MASK, bool POT = false>
