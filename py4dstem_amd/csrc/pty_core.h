@@ -981,11 +981,20 @@ PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
 template <int S, class Run> PTY_HD void projection_position(Run& run, const FusedArgs& a, int b) {
   using K = Fused<S>;
   using Regs = typename K::Regs;
-  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, false, true, true); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) {
+    K::p_load_probe_coalesced(tid, a, r);
+    K::p_setup(tid, a, b, r, lds);
+    K::p_stage_probe(tid, r, lds);
+  });
   run.barrier();
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::template p_probe_rowBinv<decltype(lds), true>(tid, a, r, lds, tw);
+  });
   run.wave_barrier();
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::p_load_amps(tid, a, r);
+    K::template p_rowA<true>(tid, r, lds, tw);
+  });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_colB<true>(tid, r, lds, tw); });
   run.barrier();
@@ -1059,9 +1068,15 @@ template <int S, class Run> PTY_HD void mixed_position(Run& run, const FusedArgs
 template <int S, class Run> PTY_HD void simulate_position(Run& run, const FusedArgs& a, int b) {
   using K = Fused<S>;
   using Regs = typename K::Regs;
-  run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, a, b, r, lds, false, false, true); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf*) {
+    K::p_load_probe_coalesced(tid, a, r);
+    K::p_setup(tid, a, b, r, lds);
+    K::p_stage_probe(tid, r, lds);
+  });
   run.barrier();
-  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, a, r, lds, tw); });
+  run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+    K::template p_probe_rowBinv<decltype(lds), true>(tid, a, r, lds, tw);
+  });
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
   run.barrier();
