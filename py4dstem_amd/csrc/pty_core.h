@@ -452,7 +452,7 @@ template <int S> struct Fused {
   struct Regs {
     cf v[N1];
     float amp[N1]; // this thread's measured amplitudes (row pass B layout),
-                   // loaded at setup so their latency hides behind the FFTs
+                   // requested phases before the projection reads them
     double err;
     int x0, y0;   // patch anchor, wrapped into [0, Px) x [0, Py)
     int idx;      // position index of this batch slot
@@ -503,8 +503,7 @@ template <int S> struct Fused {
 
   // P0: anchor of slot b and the ramp table (threads tid < 2S)
   template <class L> static PTY_HD void p_setup(int tid, const FusedArgs& a, int b, Regs& r, L lds,
-                                                bool conj_ramp = false, bool load_amps = false,
-                                                bool load_probe = false) {
+                                                bool conj_ramp = false) {
     int idx = a.order ? uniform_load(a.order, (unsigned)b) : b;  // no order: slot b is position b
     PTY_CHECK(idx >= 0 && idx < a.N, "order index out of range");
     idx = ((unsigned)idx < (unsigned)a.N) ? idx : 0;  // memory-safe on bad input
@@ -514,18 +513,6 @@ template <int S> struct Fused {
     r.x0 = wrapi_fast((int)rx, a.Px);
     r.y0 = wrapi_fast((int)ry, a.Py);
     r.err = 0.0;
-    if (load_amps) {
-      const float* A = a.amps + (size_t)idx * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
-#pragma unroll
-      for (int j = 0; j < N1; ++j) r.amp[j] = A[j];
-    }
-    // this thread's probe spectrum values for the first row pass; the loads
-    // stay in flight across the barrier that follows (it waits for LDS only)
-    if (load_probe) {
-      const cf* ph = a.probe_hat + F::row_line(tid) * S + N1 * F::row_q(tid);
-#pragma unroll
-      for (int j = 0; j < N1; ++j) r.v[j] = ph[j];
-    }
     if (tid < 2 * S) {
       const int k = tid < S ? tid : tid - S;
       const float frac = tid < S ? px - rx : py - ry;
@@ -544,18 +531,15 @@ template <int S> struct Fused {
     }
   }
 
-  // P1: row pass B' of ifft2(probe_hat * ramp), input from global memory
-  // FROM_LDS: the probe spectrum was staged in the frame (p_stage_probe);
-  // else this thread's values were loaded into r.v at setup
-  template <class L, bool FROM_LDS = false>
-  static PTY_HD void p_probe_rowBinv(int tid, const FusedArgs& a, Regs& r, L lds, const cf* tw) {
+  // P1: row pass B' of ifft2(probe_hat * ramp), the probe spectrum staged in
+  // the frame by p_stage_probe
+  template <class L> static PTY_HD void p_probe_rowBinv(int tid, const FusedArgs& a, Regs& r, L lds, const cf* tw) {
     const int rr = F::row_line(tid), t = F::row_q(tid);
     const cf ex = lds.ld(RAMP + F::kidx(rr));
 #pragma unroll
     for (int j = 0; j < N1; ++j) {
       const cf e = cmul(ex, lds.ld(RAMP + S + F::kidx(N1 * t + j)));
-      const cf p = FROM_LDS ? lds.ld(F::at(rr, N1 * t + j)) : r.v[j];
-      r.v[j] = cmul(p, e);
+      r.v[j] = cmul(lds.ld(F::at(rr, N1 * t + j)), e);
     }
     P::template B<true>(t, r.v, tw);
 #pragma unroll
@@ -787,12 +771,6 @@ template <int S> struct Fused {
   }
 
   // ---- mixed state (ptychographic_methods.py:2696-2805) -------------------
-  // mode k > 0: this thread's probe-spectrum values for the first row pass
-  static PTY_HD void p_load_probe(int tid, const FusedArgs& a, int k, Regs& r) {
-    const cf* ph = a.probe_hat + (size_t)k * S * S + F::row_line(tid) * S + N1 * F::row_q(tid);
-#pragma unroll
-    for (int j = 0; j < N1; ++j) r.v[j] = ph[j];
-  }
 
   // row pass B completes F_k; F_k to the staging buffer; sum_k |F_k|^2 in
   // r.amp (float32, modes in order, |F| as the reference's abs then square)
@@ -935,12 +913,12 @@ PTY_HD void fused_position(Run& run, const FusedArgs& a, int b) {
   // ifft2(probe_hat * ramp): B', A' along rows, then B', A' along columns
   run.phase([&](int tid, Regs& r, auto lds, const cf*) {
     if (!PROBE_LOADED) K::p_load_probe_coalesced(tid, a, r);
-    K::p_setup(tid, a, b, r, lds, false, false, false);
+    K::p_setup(tid, a, b, r, lds);
     K::p_stage_probe(tid, r, lds);
   });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
-    K::template p_probe_rowBinv<decltype(lds), true>(tid, a, r, lds, tw);
+    K::p_probe_rowBinv(tid, a, r, lds, tw);
   });
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
@@ -988,7 +966,7 @@ template <int S, class Run> PTY_HD void projection_position(Run& run, const Fuse
   });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
-    K::template p_probe_rowBinv<decltype(lds), true>(tid, a, r, lds, tw);
+    K::p_probe_rowBinv(tid, a, r, lds, tw);
   });
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
@@ -1027,13 +1005,17 @@ template <int S, class Run> PTY_HD void mixed_position(Run& run, const FusedArgs
     FusedArgs ak = a;
     ak.probe_hat = a.probe_hat + (size_t)k * S * S;
     ak.ps_out = a.ps_out ? a.ps_out + (size_t)k * a.B * S * S : nullptr;
-    if (k == 0) {
-      run.phase([&](int tid, Regs& r, auto lds, const cf*) { K::p_setup(tid, ak, b, r, lds, false, false, true); });
-    } else {
-      run.phase([&](int tid, Regs& r, auto, const cf*) { K::p_load_probe(tid, a, k, r); });
-    }
+    // mode k's probe spectrum in coalesced pieces staged through the frame
+    // (free: the previous mode's far field has left it)
+    run.phase([&](int tid, Regs& r, auto lds, const cf*) {
+      K::p_load_probe_coalesced(tid, ak, r);
+      if (k == 0) K::p_setup(tid, ak, b, r, lds);
+      K::p_stage_probe(tid, r, lds);
+    });
     run.barrier();
-    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::p_probe_rowBinv(tid, ak, r, lds, tw); });
+    run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
+      K::p_probe_rowBinv(tid, ak, r, lds, tw);
+    });
     run.wave_barrier();
     run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
     run.barrier();
@@ -1075,7 +1057,7 @@ template <int S, class Run> PTY_HD void simulate_position(Run& run, const FusedA
   });
   run.barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) {
-    K::template p_probe_rowBinv<decltype(lds), true>(tid, a, r, lds, tw);
+    K::p_probe_rowBinv(tid, a, r, lds, tw);
   });
   run.wave_barrier();
   run.phase([&](int tid, Regs& r, auto lds, const cf* tw) { K::template p_rowA<true>(tid, r, lds, tw); });
