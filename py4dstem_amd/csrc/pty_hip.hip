@@ -653,6 +653,9 @@ __global__ __launch_bounds__(256) void k_mixed_object_update(cf* __restrict__ ob
 #define PTY_PROBE_CHUNK 64
 #endif
 constexpr int PROBE_CHUNK = PTY_PROBE_CHUNK;  // batch slots per partial sum (a multiple of 16)
+// staged launches start at multiples of 64 slots (stage_size) and index the
+// partial sums by s0 / PROBE_CHUNK: a chunk must not straddle a stage
+static_assert(PROBE_CHUNK % 16 == 0 && 64 % PROBE_CHUNK == 0, "probe chunks must tile the 64-slot stages");
 
 struct ProbePartialArgs {
   const cf* chi;
