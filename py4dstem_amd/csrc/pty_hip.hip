@@ -411,8 +411,18 @@ __device__ __forceinline__ void fft2_body(const cf* __restrict__ in, cf* __restr
 #pragma unroll
       for (int j = 0; j < F::N1; ++j) dst[kr * S + F::kidx(F::N1 * t + j)] = r.v[j];
     } else {
+      // kernel layout out through the frame: a thread's own 16 elements are
+      // 128 contiguous bytes, which would make every store instruction touch
+      // 64 cache lines; pieces of two elements per lane are contiguous
 #pragma unroll
-      for (int j = 0; j < F::N1; ++j) dst[rr * S + F::N1 * t + j] = r.v[j];
+      for (int j = 0; j < F::N1; ++j) lds.st(F::at(rr, F::N1 * t + j), r.v[j]);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < F::N1 / 2; ++k) {
+        const int e = 2 * (k * F::T + tid);
+        dst[e] = lds.ld(F::at(e / S, e % S));
+        dst[e + 1] = lds.ld(F::at(e / S, e % S) + 1);
+      }
     }
   } else {
     const int rr = F::row_line(tid), t = F::row_q(tid);
@@ -421,8 +431,16 @@ __device__ __forceinline__ void fft2_body(const cf* __restrict__ in, cf* __restr
 #pragma unroll
       for (int j = 0; j < F::N1; ++j) r.v[j] = src[kr * S + F::kidx(F::N1 * t + j)];
     } else {
+      // kernel layout in through the frame, in coalesced two-element pieces
 #pragma unroll
-      for (int j = 0; j < F::N1; ++j) r.v[j] = src[rr * S + F::N1 * t + j];
+      for (int k = 0; k < F::N1 / 2; ++k) {
+        const int e = 2 * (k * F::T + tid);
+        lds.st(F::at(e / S, e % S), src[e]);
+        lds.st(F::at(e / S, e % S) + 1, src[e + 1]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < F::N1; ++j) r.v[j] = lds.ld(F::at(rr, F::N1 * t + j));
     }
     Pass<S>::template B<true>(t, r.v, tw);
 #pragma unroll
